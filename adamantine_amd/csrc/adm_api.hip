@@ -1,0 +1,904 @@
+// C-ABI of the MI355X matrix-free heat path (declared in include/adm.h).
+//
+// The context replaces adamantine's ThermalOperatorDevice + the device
+// instantiation of ThermalPhysics for a structured box mesh; see adm.h for the
+// reference interface each entry point stands in for.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/adm.h"
+#include "adm_context.hpp"
+#include "adm_kernels.hpp"
+#include "adm_launch.hpp"
+#include "adm_step_kernels.hpp"
+
+namespace adm {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define ADM_HIP(call)                                                     \
+  do {                                                                    \
+    hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess)                                                 \
+      throw AdmError(ADM_ERR_HIP, std::string(#call) + ": " +             \
+                                      hipGetErrorString(e_));             \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// 1-D quadrature (same construction as QGauss / QGaussLobatto: Newton on
+// Legendre polynomials in extended precision)
+// ---------------------------------------------------------------------------
+
+static void legendre(int n, long double x, long double *p, long double *dp) {
+  long double p0 = 1.0L, p1 = x;
+  if (n == 0) {
+    *p = 1.0L;
+    *dp = 0.0L;
+    return;
+  }
+  for (int k = 2; k <= n; ++k) {
+    long double pk = ((2.0L * k - 1.0L) * x * p1 - (k - 1.0L) * p0) / k;
+    p0 = p1;
+    p1 = pk;
+  }
+  *p = p1;
+  *dp = n * (x * p1 - p0) / (x * x - 1.0L);
+}
+
+void gauss_points(int n, double *x, double *w) {
+  for (int i = 0; i < n; ++i) {
+    long double z = cosl(3.14159265358979323846264338327950288L * (i + 0.75L) /
+                         (n + 0.5L));
+    for (int it = 0; it < 100; ++it) {
+      long double p, dp;
+      legendre(n, z, &p, &dp);
+      long double dz = p / dp;
+      z -= dz;
+      if (fabsl(dz) < 1e-19L) break;
+    }
+    long double p, dp;
+    legendre(n, z, &p, &dp);
+    long double ww = 2.0L / ((1.0L - z * z) * dp * dp);
+    x[n - 1 - i] = (double)(0.5L + 0.5L * z);
+    w[n - 1 - i] = (double)(0.5L * ww);
+  }
+}
+
+void gauss_lobatto_points(int n, double *x, double *w) {
+  int m = n - 1;
+  x[0] = 0.0;
+  x[n - 1] = 1.0;
+  w[0] = w[n - 1] = (double)(0.5L * 2.0L / (m * (m + 1.0L)));
+  for (int i = 1; i < n - 1; ++i) {
+    long double z = -cosl(3.14159265358979323846264338327950288L * i / m);
+    for (int it = 0; it < 200; ++it) {
+      long double p, dp;
+      legendre(m, z, &p, &dp);
+      long double d2p = (2.0L * z * dp - m * (m + 1.0L) * p) / (1.0L - z * z);
+      long double dz = dp / d2p;
+      z -= dz;
+      if (fabsl(dz) < 1e-19L) break;
+    }
+    long double p, dp;
+    legendre(m, z, &p, &dp);
+    x[i] = (double)(0.5L + 0.5L * z);
+    w[i] = (double)(0.5L * 2.0L / (m * (m + 1.0L) * p * p));
+  }
+}
+
+static double lagrange(int n, const double *pts, int i, double x) {
+  double v = 1.0;
+  for (int j = 0; j < n; ++j)
+    if (j != i) v *= (x - pts[j]) / (pts[i] - pts[j]);
+  return v;
+}
+
+static double lagrange_deriv(int n, const double *pts, int k, double x) {
+  double s = 0.0;
+  for (int m = 0; m < n; ++m) {
+    if (m == k) continue;
+    double t = 1.0 / (pts[k] - pts[m]);
+    for (int j = 0; j < n; ++j)
+      if (j != k && j != m) t *= (x - pts[j]) / (pts[k] - pts[j]);
+    s += t;
+  }
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// scan path (source/ScanPath.cc:197-275) and HeatSource::update_time
+// ---------------------------------------------------------------------------
+
+static int scan_segment(const BeamData &b, double t, int &old_segment,
+                        double sp[3], double *st) {
+  int cur = old_segment;
+  while (t > b.segments[cur].end_time) ++cur;
+  if (cur > 0) {
+    *st = b.segments[cur - 1].end_time;
+    for (int d = 0; d < 3; ++d) sp[d] = b.segments[cur - 1].end_point[d];
+  } else {
+    *st = 0.0;
+    for (int d = 0; d < 3; ++d) sp[d] = b.segments[cur].end_point[d];
+  }
+  old_segment = cur;
+  return cur;
+}
+
+void beam_update(const BeamData &b, double t, int &old_segment,
+                 adm_beam_state &s) {
+  std::memset(&s, 0, sizeof(s));
+  s.type = b.desc.type;
+  if (b.desc.type == ADM_SOURCE_CUBE) {
+    s.on = (t > b.desc.start_time) && (t < b.desc.end_time);
+    s.cube_value = b.desc.value;
+    for (int d = 0; d < 3; ++d) {
+      s.cube_min[d] = b.desc.min_point[d];
+      s.cube_max[d] = b.desc.max_point[d];
+    }
+    return;
+  }
+  const adm_segment &last = b.segments.back();
+  double pmod = 0.0;
+  if (!(t > last.end_time)) {
+    double sp[3], st;
+    int cur = scan_segment(b, t, old_segment, sp, &st);
+    pmod = b.segments[cur].power_modifier;
+    // deal.II's Tensor / scalar multiplies by the reciprocal
+    const double inv = 1.0 / (b.segments[cur].end_time - st);
+    for (int d = 0; d < 3; ++d)
+      s.center[d] =
+          sp[d] + ((b.segments[cur].end_point[d] - sp[d]) * inv) * (t - st);
+  } else {
+    for (int d = 0; d < 3; ++d) s.center[d] = -DBL_MAX;
+  }
+  s.on = pmod > 0.0;
+  const double r2 = std::pow(b.desc.radius, 2);
+  const double dd = b.desc.depth;
+  s.depth = dd;
+  s.radius_squared = r2;
+  if (b.desc.type == ADM_SOURCE_GOLDAK) {
+    const double pi_over_3_to_1p5 = std::pow(M_PI / 3.0, 1.5);
+    s.alpha = 2.0 * b.desc.absorption_efficiency * b.desc.max_power * pmod /
+              (r2 * dd * pi_over_3_to_1p5);
+    s.inv_radius_squared = 1.0 / r2;
+    s.inv_depth_squared = 1.0 / (dd * dd);
+  } else {
+    const double log_01 = std::log(0.1);
+    s.alpha = -b.desc.absorption_efficiency * b.desc.max_power * pmod *
+              log_01 / (M_PI * r2 * dd);
+    s.inv_depth = 1.0 / dd;
+    s.inv_radius_squared = 1.0 / r2;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+
+static bool table_sorted(const double (*t)[2]) {
+  for (int i = 1; i < ADM_TABLE_SIZE; ++i)
+    if (t[i][0] < t[i - 1][0]) return false;
+  return true;
+}
+
+Context::Context(const adm_problem_desc &d) {
+  const adm_mesh_desc &m = d.mesh;
+  if (m.dim != 3)
+    throw AdmError(ADM_ERR_UNSUPPORTED, "the device path is three-dimensional");
+  if (m.fe_degree < 1 || m.fe_degree > 2)
+    throw AdmError(ADM_ERR_UNSUPPORTED,
+                   "fe_degree must be 1 or 2 on the device path");
+  if (m.quadrature != 0 && m.quadrature != 1)
+    throw AdmError(ADM_ERR_INVALID, "quadrature must be 0 (Gauss) or 1 "
+                                    "(Gauss-Lobatto)");
+  for (int k = 0; k < 3; ++k)
+    if (m.n_cells[k] < 1 || !(m.extent[k] > 0.0))
+      throw AdmError(ADM_ERR_INVALID, "invalid mesh size");
+  const adm_material_desc &mat = d.material;
+  if (mat.n_materials < 1 || mat.n_materials > ADM_MAX_MATERIALS)
+    throw AdmError(ADM_ERR_INVALID, "n_materials must be in 1..4");
+  if (mat.n_states < 1 || mat.n_states > 3)
+    throw AdmError(ADM_ERR_INVALID, "n_states must be 1, 2 or 3");
+  if (mat.p_order < 0 || mat.p_order > 4)
+    throw AdmError(ADM_ERR_INVALID, "p_order must be in 0..4");
+  if (d.n_boundary_ids < 1 || d.n_boundary_ids > ADM_MAX_BOUNDARY_IDS)
+    throw AdmError(ADM_ERR_INVALID, "invalid number of boundary ids");
+  if (d.n_beams < 0 || d.n_beams > ADM_MAX_BEAMS)
+    throw AdmError(ADM_ERR_INVALID, "at most 4 heat sources");
+
+  p = m.fe_degree;
+  n1 = p + 1;
+  nq = n1 * n1 * n1;
+  nfq = n1 * n1;
+  n_cells = 1;
+  n_dofs = 1;
+  for (int k = 0; k < 3; ++k) {
+    nc[k] = m.n_cells[k];
+    nn[k] = p * nc[k] + 1;
+    n_cells *= nc[k];
+    n_dofs *= nn[k];
+  }
+  n_states = mat.n_states;
+  use_table = mat.use_table != 0;
+
+  // ---- constants ----------------------------------------------------------
+  std::memset(&kc, 0, sizeof(kc));
+  double xs[kMaxN], wq[kMaxN];
+  gauss_lobatto_points(n1, xs, wgl);
+  if (m.quadrature == 1)
+    gauss_lobatto_points(n1, kc.xq, wq);
+  else
+    gauss_points(n1, kc.xq, wq);
+  for (int q = 0; q < n1; ++q)
+    for (int i = 0; i < n1; ++i) {
+      kc.S[q][i] = lagrange(n1, xs, i, kc.xq[q]);
+      kc.Dc[q][i] = lagrange_deriv(n1, kc.xq, i, kc.xq[q]);
+    }
+  det = 1.0;
+  for (int k = 0; k < 3; ++k) {
+    kc.h[k] = m.extent[k] / nc[k];
+    kc.inv_h[k] = 1.0 / kc.h[k];
+    kc.origin[k] = m.origin[k];
+    det *= kc.h[k];
+  }
+  for (int q = 0; q < nq; ++q) {
+    double w = 1.0;
+    int r = q;
+    for (int k = 0; k < 3; ++k) {
+      w *= wq[r % n1];
+      r /= n1;
+    }
+    kc.jxw[q] = det * w;
+  }
+  for (int nd = 0; nd < 3; ++nd) {
+    double fdet = 1.0;
+    for (int k = 0; k < 3; ++k)
+      if (k != nd) fdet *= kc.h[k];
+    for (int q = 0; q < nfq; ++q) {
+      double w = 1.0;
+      int r = q;
+      for (int k = 0; k < 2; ++k) {
+        w *= wq[r % n1];
+        r /= n1;
+      }
+      kc.fjxw[nd][q] = fdet * w;
+    }
+  }
+  kc.n_boundary_ids = d.n_boundary_ids;
+  int n_adiabatic = 0;
+  for (int i = 0; i < d.n_boundary_ids; ++i) {
+    kc.boundary_type[i] = d.boundary_type[i];
+    if (d.boundary_type[i] & ADM_BC_ADIABATIC) ++n_adiabatic;
+  }
+  kc.adiabatic_only = n_adiabatic == d.n_boundary_ids;
+  kc.log_01 = std::log(0.1);
+
+  MatConst &mc = kc.mat;
+  mc.n_materials = mat.n_materials;
+  mc.n_states = mat.n_states;
+  mc.use_table = mat.use_table;
+  mc.p_order = mat.p_order;
+  mc.single_layout = mat.single_material_layout;
+  std::memcpy(mc.table, mat.table, sizeof(mc.table));
+  std::memcpy(mc.poly, mat.poly, sizeof(mc.poly));
+  std::memcpy(mc.prop, mat.prop, sizeof(mc.prop));
+  for (int mid = 0; mid < ADM_MAX_MATERIALS; ++mid) {
+    mc.latent_factor[mid] = mat.prop[mid][ADM_LATENT_HEAT] /
+                            (mat.prop[mid][ADM_LIQUIDUS] -
+                             mat.prop[mid][ADM_SOLIDUS]);
+    for (int pr = 0; pr < ADM_N_STATE_PROPS; ++pr)
+      for (int s = 0; s < ADM_MAX_STATES; ++s) {
+        const double(*t)[2] = mat.table[mid][pr][s];
+        if (mat.use_table && mid < mat.n_materials && s < mat.n_states &&
+            !table_sorted(t))
+          throw AdmError(ADM_ERR_INVALID,
+                         "material tables must have increasing temperatures");
+        bool constant = true;
+        for (int i = 1; i < ADM_TABLE_SIZE; ++i)
+          if (t[i][1] != t[0][1]) constant = false;
+        mc.kind[mid][pr][s] = constant ? kTableConstant : kTableGeneral;
+      }
+    const int kx = ADM_THERMAL_CONDUCTIVITY_X;
+    bool iso = true;
+    for (int other = kx + 1; other <= ADM_THERMAL_CONDUCTIVITY_Z; ++other) {
+      if (mat.use_table) {
+        if (std::memcmp(mat.table[mid][kx], mat.table[mid][other],
+                        sizeof(mat.table[mid][kx])) != 0)
+          iso = false;
+      } else {
+        if (std::memcmp(mat.poly[mid][kx], mat.poly[mid][other],
+                        sizeof(mat.poly[mid][kx])) != 0)
+          iso = false;
+      }
+    }
+    mc.isotropic[mid] = iso ? 1 : 0;
+  }
+
+  // ---- per-cell data --------------------------------------------------------
+  active.assign(n_cells, 1);
+  material.assign(n_cells, 0);
+  dep_cos.assign(n_cells, 1.0);
+  dep_sin.assign(n_cells, 0.0);
+  for (int64_t c = 0; c < n_cells; ++c) {
+    if (d.cell_active) active[c] = d.cell_active[c] ? 1 : 0;
+    if (d.cell_material) {
+      if (d.cell_material[c] >= mat.n_materials)
+        throw AdmError(ADM_ERR_INVALID, "cell material id out of range");
+      material[c] = d.cell_material[c];
+    }
+    if (d.deposition_cos) dep_cos[c] = d.deposition_cos[c];
+    if (d.deposition_sin) dep_sin[c] = d.deposition_sin[c];
+  }
+  cell_state.assign((size_t)n_cells * n_states, 0.0);
+  for (int64_t c = 0; c < n_cells; ++c) {
+    if (d.cell_state) {
+      for (int s = 0; s < n_states; ++s)
+        cell_state[c * n_states + s] = d.cell_state[c * n_states + s];
+    } else {
+      cell_state[c * n_states] = 1.0;
+    }
+  }
+
+  for (int b = 0; b < d.n_beams; ++b) {
+    BeamData bd;
+    bd.desc = d.beams[b];
+    if (bd.desc.type != ADM_SOURCE_CUBE) {
+      if (bd.desc.n_segments < 1 || !bd.desc.segments)
+        throw AdmError(ADM_ERR_INVALID, "beam without scan path segments");
+      bd.segments.assign(bd.desc.segments,
+                         bd.desc.segments + bd.desc.n_segments);
+    } else if (bd.desc.type < 0 || bd.desc.type > 2) {
+      throw AdmError(ADM_ERR_INVALID, "unknown heat source type");
+    }
+    bd.desc.segments = nullptr;
+    beams.push_back(bd);
+  }
+  old_segment.assign(beams.size(), 0);
+  beam_states.resize(ADM_MAX_BEAMS);
+  std::memset(beam_states.data(), 0, sizeof(adm_beam_state) * ADM_MAX_BEAMS);
+
+  ADM_HIP(hipGetDevice(&device));
+  ADM_HIP(hipMalloc(&d_kc, sizeof(KConst)));
+  ADM_HIP(hipMalloc(&d_meta, sizeof(uint32_t) * n_cells));
+  ADM_HIP(hipMalloc(&d_powder_slot, sizeof(int32_t) * n_cells));
+  ADM_HIP(hipMalloc(&d_cos, sizeof(double) * n_cells));
+  ADM_HIP(hipMalloc(&d_sin, sizeof(double) * n_cells));
+  ADM_HIP(hipMalloc(&d_wgl, sizeof(double) * kMaxN));
+  ADM_HIP(hipMemcpy(d_wgl, wgl, sizeof(double) * n1, hipMemcpyHostToDevice));
+  if (n_states > 1) {
+    ADM_HIP(hipMalloc(&d_liquid, sizeof(double) * n_cells * nq));
+    ADM_HIP(hipMalloc(&d_liquid_dirty, n_cells));
+  }
+  upload_constants();
+  rebuild_cells();
+  set_cell_state(cell_state.data());
+}
+
+#define ADM_FREE(p) (void)hipFree(p)
+Context::~Context() {
+  ADM_FREE(d_kc);
+  ADM_FREE(d_meta);
+  ADM_FREE(d_powder_slot);
+  ADM_FREE(d_cos);
+  ADM_FREE(d_sin);
+  ADM_FREE(d_wgl);
+  ADM_FREE(d_liquid);
+  ADM_FREE(d_liquid_dirty);
+  ADM_FREE(d_powder);
+  ADM_FREE(d_face_powder);
+  ADM_FREE(d_R);
+  ADM_FREE(d_Y);
+  ADM_FREE(d_acc);
+  ADM_FREE(d_F);
+  ADM_FREE(d_inv_mass);
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+}
+
+void Context::upload_constants() {
+  ADM_HIP(hipMemcpy(d_kc, &kc, sizeof(KConst), hipMemcpyHostToDevice));
+}
+
+// Faces of the active domain that carry a boundary term
+// (source/ThermalOperator.templates.hh:629-652 with Boundary::get_boundary_type)
+uint32_t Context::face_mask(int64_t c) const {
+  if (kc.adiabatic_only) return 0;
+  const int cc[3] = {(int)(c % nc[0]), (int)((c / nc[0]) % nc[1]),
+                     (int)(c / ((int64_t)nc[0] * nc[1]))};
+  uint32_t mask = 0;
+  for (int f = 0; f < 6; ++f) {
+    const int dd = f / 2, side = f % 2;
+    const int nb = cc[dd] + (side ? 1 : -1);
+    unsigned bid;
+    if (nb < 0 || nb >= nc[dd]) {
+      bid = (unsigned)f;
+    } else {
+      int cn[3] = {cc[0], cc[1], cc[2]};
+      cn[dd] = nb;
+      const int64_t c2 =
+          cn[0] + (int64_t)nc[0] * (cn[1] + (int64_t)nc[1] * cn[2]);
+      if (active[c2]) continue;
+      bid = 0xffffffffu;
+    }
+    const int n = kc.n_boundary_ids;
+    const int t = bid < (unsigned)(n - 1) ? kc.boundary_type[bid]
+                                          : kc.boundary_type[n - 1];
+    if (t & (ADM_BC_RADIATIVE | ADM_BC_CONVECTIVE)) mask |= 1u << f;
+  }
+  return mask;
+}
+
+void Context::rebuild_cells() {
+  meta.assign(n_cells, 0);
+  powder_slot.assign(n_cells, -1);
+  n_powder_cells = 0;
+  for (int64_t c = 0; c < n_cells; ++c) {
+    uint32_t w = 0;
+    if (active[c]) w |= kActive;
+    w |= (uint32_t)material[c] << kMaterialShift;
+    if (active[c]) w |= face_mask(c) << kFaceShift;
+    if (dep_cos[c] != 1.0 || dep_sin[c] != 0.0) w |= kRotated;
+    if (n_states == 3 && cell_state[c * 3 + 2] > 0.0) {
+      w |= kHasPowder;
+      powder_slot[c] = (int32_t)n_powder_cells++;
+    }
+    meta[c] = w;
+  }
+  ADM_HIP(hipMemcpy(d_meta, meta.data(), sizeof(uint32_t) * n_cells,
+                    hipMemcpyHostToDevice));
+  ADM_HIP(hipMemcpy(d_powder_slot, powder_slot.data(),
+                    sizeof(int32_t) * n_cells, hipMemcpyHostToDevice));
+  ADM_HIP(hipMemcpy(d_cos, dep_cos.data(), sizeof(double) * n_cells,
+                    hipMemcpyHostToDevice));
+  ADM_HIP(hipMemcpy(d_sin, dep_sin.data(), sizeof(double) * n_cells,
+                    hipMemcpyHostToDevice));
+  ADM_FREE(d_powder);
+  ADM_FREE(d_face_powder);
+  d_powder = nullptr;
+  d_face_powder = nullptr;
+  if (n_powder_cells > 0) {
+    ADM_HIP(hipMalloc(&d_powder, sizeof(double) * n_powder_cells * nq));
+    ADM_HIP(hipMalloc(&d_face_powder,
+                      sizeof(double) * n_powder_cells * 6 * nfq));
+  }
+}
+
+// get_state_from_material_properties: every quadrature point of a cell (and
+// of its faces) takes the cell's ratios.
+void Context::set_cell_state(const double *cs) {
+  if (cs != cell_state.data())
+    std::copy(cs, cs + (size_t)n_cells * n_states, cell_state.begin());
+  // powder storage follows the cells that hold powder
+  bool need_rebuild = false;
+  if (n_states == 3) {
+    for (int64_t c = 0; c < n_cells; ++c)
+      if ((cell_state[c * 3 + 2] > 0.0) != (powder_slot[c] >= 0)) {
+        need_rebuild = true;
+        break;
+      }
+  }
+  if (need_rebuild) rebuild_cells();
+  if (n_states > 1) {
+    std::vector<double> lq((size_t)n_cells * nq);
+    std::vector<uint8_t> dirty(n_cells, 0);
+    for (int64_t c = 0; c < n_cells; ++c) {
+      const double l = cell_state[c * n_states + 1];
+      for (int q = 0; q < nq; ++q) lq[c * nq + q] = l;
+      dirty[c] = l != 0.0;
+    }
+    ADM_HIP(hipMemcpy(d_liquid, lq.data(), sizeof(double) * lq.size(),
+                      hipMemcpyHostToDevice));
+    ADM_HIP(hipMemcpy(d_liquid_dirty, dirty.data(), n_cells,
+                      hipMemcpyHostToDevice));
+  }
+  if (n_powder_cells > 0) {
+    std::vector<double> pw((size_t)n_powder_cells * nq);
+    std::vector<double> fpw((size_t)n_powder_cells * 6 * nfq);
+    for (int64_t c = 0; c < n_cells; ++c) {
+      const int64_t s = powder_slot[c];
+      if (s < 0) continue;
+      const double p = cell_state[c * 3 + 2];
+      for (int q = 0; q < nq; ++q) pw[s * nq + q] = p;
+      for (int q = 0; q < 6 * nfq; ++q) fpw[s * 6 * nfq + q] = p;
+    }
+    ADM_HIP(hipMemcpy(d_powder, pw.data(), sizeof(double) * pw.size(),
+                      hipMemcpyHostToDevice));
+    ADM_HIP(hipMemcpy(d_face_powder, fpw.data(), sizeof(double) * fpw.size(),
+                      hipMemcpyHostToDevice));
+  }
+}
+
+void Context::get_qp_state(double *liquid, double *powder) {
+  ADM_HIP(hipDeviceSynchronize());
+  if (liquid) {
+    if (n_states > 1)
+      ADM_HIP(hipMemcpy(liquid, d_liquid, sizeof(double) * n_cells * nq,
+                        hipMemcpyDeviceToHost));
+    else
+      std::fill(liquid, liquid + (size_t)n_cells * nq, 0.0);
+  }
+  if (powder) {
+    std::fill(powder, powder + (size_t)n_cells * nq, 0.0);
+    if (n_powder_cells > 0) {
+      std::vector<double> pw((size_t)n_powder_cells * nq);
+      ADM_HIP(hipMemcpy(pw.data(), d_powder, sizeof(double) * pw.size(),
+                        hipMemcpyDeviceToHost));
+      for (int64_t c = 0; c < n_cells; ++c)
+        if (powder_slot[c] >= 0)
+          std::copy(pw.begin() + powder_slot[c] * nq,
+                    pw.begin() + (powder_slot[c] + 1) * nq, powder + c * nq);
+    }
+  }
+}
+
+void Context::set_qp_state(const double *liquid, const double *powder) {
+  if (liquid && n_states > 1) {
+    ADM_HIP(hipMemcpy(d_liquid, liquid, sizeof(double) * n_cells * nq,
+                      hipMemcpyHostToDevice));
+    std::vector<uint8_t> dirty(n_cells, 0);
+    for (int64_t c = 0; c < n_cells; ++c)
+      for (int q = 0; q < nq; ++q)
+        if (liquid[c * nq + q] != 0.0) dirty[c] = 1;
+    ADM_HIP(hipMemcpy(d_liquid_dirty, dirty.data(), n_cells,
+                      hipMemcpyHostToDevice));
+  }
+  if (powder && n_states == 3) {
+    for (int64_t c = 0; c < n_cells; ++c) {
+      bool any = false;
+      for (int q = 0; q < nq; ++q) any |= powder[c * nq + q] > 0.0;
+      if (any && powder_slot[c] < 0)
+        throw AdmError(ADM_ERR_INVALID,
+                       "powder can only be set in cells that hold powder");
+    }
+    if (n_powder_cells > 0) {
+      std::vector<double> pw((size_t)n_powder_cells * nq);
+      for (int64_t c = 0; c < n_cells; ++c)
+        if (powder_slot[c] >= 0)
+          std::copy(powder + c * nq, powder + (c + 1) * nq,
+                    pw.begin() + powder_slot[c] * nq);
+      ADM_HIP(hipMemcpy(d_powder, pw.data(), sizeof(double) * pw.size(),
+                        hipMemcpyHostToDevice));
+    }
+  }
+}
+
+void Context::set_time(double t) {
+  sources_on = false;
+  for (size_t b = 0; b < beams.size(); ++b) {
+    beam_update(beams[b], t, old_segment[b], beam_states[b]);
+    if (beam_states[b].on) sources_on = true;
+  }
+}
+
+CellArgs Context::cell_args() const {
+  CellArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.meta = d_meta;
+  a.powder_slot = d_powder_slot;
+  a.powder = d_powder;
+  a.face_powder = d_face_powder;
+  a.liquid = d_liquid;
+  a.liquid_dirty = d_liquid_dirty;
+  a.dep_cos = d_cos;
+  a.dep_sin = d_sin;
+  a.kc = d_kc;
+  for (int k = 0; k < 3; ++k) {
+    a.nc[k] = nc[k];
+    a.nn[k] = nn[k];
+  }
+  a.n_beams = (int)beams.size();
+  a.sources_on = sources_on ? 1 : 0;
+  for (size_t b = 0; b < beams.size(); ++b) a.beams[b] = beam_states[b];
+  return a;
+}
+
+void Context::apply(const double *src, double *dst, hipStream_t s) {
+  const CellArgs a = cell_args();
+#define ADM_DISPATCH(P_, NS_)                                     \
+  if (p == P_ && n_states == NS_) {                                \
+    if (use_table)                                                 \
+      launch_cell<P_, NS_, true>(a, src, dst, n_cells, s);         \
+    else                                                           \
+      launch_cell<P_, NS_, false>(a, src, dst, n_cells, s);        \
+  }
+  ADM_DISPATCH(1, 1)
+  ADM_DISPATCH(1, 2)
+  ADM_DISPATCH(1, 3)
+  ADM_DISPATCH(2, 1)
+  ADM_DISPATCH(2, 2)
+  ADM_DISPATCH(2, 3)
+#undef ADM_DISPATCH
+  ADM_HIP(hipGetLastError());
+}
+
+void Context::fill(double *x, double v, int64_t n, hipStream_t s) {
+  const int block = 256;
+  hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + block - 1) / block)),
+                     dim3(block), 0, s, x, v, n);
+  ADM_HIP(hipGetLastError());
+}
+
+void Context::compute_inverse_mass(double *out, hipStream_t s) {
+  const int block = 256;
+  const unsigned grid = (unsigned)((n_dofs + block - 1) / block);
+  if (p == 1)
+    hipLaunchKernelGGL(inverse_mass_kernel<1>, dim3(grid), dim3(block), 0, s,
+                       d_meta, d_wgl, det, nc[0], nc[1], nc[2], n_dofs, out);
+  else
+    hipLaunchKernelGGL(inverse_mass_kernel<2>, dim3(grid), dim3(block), 0, s,
+                       d_meta, d_wgl, det, nc[0], nc[1], nc[2], n_dofs, out);
+  ADM_HIP(hipGetLastError());
+}
+
+void Context::ensure_step_buffers(int method) {
+  if (!d_R) {
+    ADM_HIP(hipMalloc(&d_R, sizeof(double) * n_dofs));
+    ADM_HIP(hipMemset(d_R, 0, sizeof(double) * n_dofs));
+  }
+  if (!d_inv_mass) {
+    ADM_HIP(hipMalloc(&d_inv_mass, sizeof(double) * n_dofs));
+    compute_inverse_mass(d_inv_mass, nullptr);
+    ADM_HIP(hipDeviceSynchronize());
+  }
+  if (method != ADM_FORWARD_EULER && !d_Y) {
+    ADM_HIP(hipMalloc(&d_Y, sizeof(double) * n_dofs));
+    ADM_HIP(hipMalloc(&d_acc, sizeof(double) * n_dofs));
+  }
+  if (method == ADM_RK_THIRD_ORDER && !d_F)
+    ADM_HIP(hipMalloc(&d_F, sizeof(double) * n_dofs * 3));
+}
+
+// One evolve_one_time_step on the device, enqueued on s.
+void Context::step(int method, double t, double dt, double *T, hipStream_t s) {
+  const int block = 256;
+  const unsigned grid = (unsigned)((n_dofs + block - 1) / block);
+  if (method == ADM_FORWARD_EULER) {
+    set_time(t);
+    apply(T, d_R, s);
+    hipLaunchKernelGGL(fe_update_kernel, dim3(grid), dim3(block), 0, s, T, d_R,
+                       d_inv_mass, dt, n_dofs);
+  } else if (method == ADM_RK_FOURTH_ORDER) {
+    static const double b[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+    static const double c[4] = {0.0, 0.5, 0.5, 1.0};
+    static const double anext[4] = {0.5, 0.5, 1.0, 0.0};
+    for (int i = 0; i < 4; ++i) {
+      set_time(t + c[i] * dt);
+      apply(i == 0 ? T : d_Y, d_R, s);
+      hipLaunchKernelGGL(rk4_stage_kernel, dim3(grid), dim3(block), 0, s, T,
+                         d_R, d_inv_mass, d_acc, d_Y, dt * b[i],
+                         dt * anext[i], i == 0 ? 1 : 0, i == 3 ? 1 : 0,
+                         n_dofs);
+    }
+  } else {
+    static const double a[3][3] = {{0, 0, 0}, {0.5, 0, 0}, {-1.0, 2.0, 0}};
+    static const double b[3] = {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0};
+    static const double c[3] = {0.0, 0.5, 1.0};
+    for (int i = 0; i < 3; ++i) {
+      set_time(t + c[i] * dt);
+      apply(i == 0 ? T : d_Y, d_R, s);
+      const double *an = i + 1 < 3 ? a[i + 1] : a[0];
+      hipLaunchKernelGGL(rk_stage_kernel, dim3(grid), dim3(block), 0, s, T,
+                         d_R, d_inv_mass, d_F, i, 3, dt * an[0], dt * an[1],
+                         dt * an[2], d_Y, dt * b[0], dt * b[1], dt * b[2],
+                         n_dofs);
+    }
+  }
+  ADM_HIP(hipGetLastError());
+}
+
+}  // namespace adm
+
+using adm::AdmError;
+using adm::Context;
+
+#define ADM_API_BEGIN try {
+#define ADM_API_END                                              \
+  }                                                              \
+  catch (const AdmError &e) {                                    \
+    return adm::set_error(e.code, e.what());                     \
+  }                                                              \
+  catch (const std::exception &e) {                              \
+    return adm::set_error(ADM_ERR_INVALID, e.what());            \
+  }                                                              \
+  return ADM_OK;
+
+#define ADM_CTX(ptr)                                              \
+  if (!(ptr)) return adm::set_error(ADM_ERR_STATE, "null context"); \
+  Context &ctx = *static_cast<Context *>(ptr);
+
+extern "C" {
+
+int adm_create(const adm_problem_desc *desc, void **out) {
+  ADM_API_BEGIN
+  if (!desc || !out) return adm::set_error(ADM_ERR_INVALID, "null argument");
+  *out = new Context(*desc);
+  ADM_API_END
+}
+
+int adm_destroy(void *ptr) {
+  ADM_API_BEGIN
+  delete static_cast<Context *>(ptr);
+  ADM_API_END
+}
+
+const char *adm_last_error(void) { return adm::g_last_error.c_str(); }
+
+int adm_n_dofs(const void *ptr, int64_t *n) {
+  ADM_API_BEGIN
+  if (!ptr || !n) return adm::set_error(ADM_ERR_INVALID, "null argument");
+  *n = static_cast<const Context *>(ptr)->n_dofs;
+  ADM_API_END
+}
+
+int adm_sizes(const void *ptr, int64_t *n_cells, int32_t *n_q, int32_t *n_fq) {
+  ADM_API_BEGIN
+  if (!ptr) return adm::set_error(ADM_ERR_INVALID, "null argument");
+  const Context &c = *static_cast<const Context *>(ptr);
+  if (n_cells) *n_cells = c.n_cells;
+  if (n_q) *n_q = c.nq;
+  if (n_fq) *n_fq = c.nfq;
+  ADM_API_END
+}
+
+int adm_compute_inverse_mass(void *ptr, double *inv_mass, void *stream) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  if (!inv_mass) return adm::set_error(ADM_ERR_INVALID, "null output");
+  ctx.compute_inverse_mass(inv_mass, static_cast<hipStream_t>(stream));
+  ADM_API_END
+}
+
+int adm_set_time(void *ptr, double t) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  ctx.set_time(t);
+  ADM_API_END
+}
+
+int adm_vmult(void *ptr, const double *src, double *dst, int add,
+              void *stream) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  if (!src || !dst) return adm::set_error(ADM_ERR_INVALID, "null vector");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!add) ctx.fill(dst, 0.0, ctx.n_dofs, s);
+  ctx.apply(src, dst, s);
+  ADM_API_END
+}
+
+int adm_evolve(void *ptr, int method, double t, double dt, int n_steps,
+               double *T, int use_graph, double *t_end, void *stream) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  if (!T) return adm::set_error(ADM_ERR_INVALID, "null solution vector");
+  if (method < ADM_FORWARD_EULER || method > ADM_RK_FOURTH_ORDER)
+    return adm::set_error(ADM_ERR_INVALID, "unknown time stepping method");
+  if (n_steps < 0) return adm::set_error(ADM_ERR_INVALID, "n_steps < 0");
+  (void)use_graph;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  ctx.ensure_step_buffers(method);
+  for (int k = 0; k < n_steps; ++k) {
+    ctx.step(method, t, dt, T, s);
+    t = t + dt;
+  }
+  if (t_end) *t_end = t;
+  ADM_API_END
+}
+
+int adm_set_cell_state(void *ptr, const double *cell_state) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  if (!cell_state) return adm::set_error(ADM_ERR_INVALID, "null state");
+  ADM_HIP(hipDeviceSynchronize());
+  ctx.set_cell_state(cell_state);
+  ADM_API_END
+}
+
+int adm_get_cell_state(void *ptr, double *cs) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  if (!cs) return adm::set_error(ADM_ERR_INVALID, "null state");
+  const int ns = ctx.n_states;
+  const int nq = ctx.nq;
+  std::vector<double> lq((size_t)ctx.n_cells * nq), pw((size_t)ctx.n_cells * nq);
+  ctx.get_qp_state(lq.data(), pw.data());
+  // MaterialProperty::set_state (source/MaterialProperty.templates.hh:537-620)
+  for (int64_t c = 0; c < ctx.n_cells; ++c) {
+    if (ns == 1) {
+      cs[c] = 1.0;
+      continue;
+    }
+    double ls = 0.0, ps = 0.0;
+    for (int q = 0; q < nq; ++q) {
+      ls += lq[c * nq + q];
+      ps += pw[c * nq + q];
+    }
+    const double l = ls / nq;
+    if (ns == 2) {
+      cs[c * 2 + 1] = l;
+      cs[c * 2] = 1. - l;
+    } else {
+      const double pp = ps / nq;
+      cs[c * 3 + 1] = l;
+      cs[c * 3 + 2] = pp;
+      cs[c * 3] = 1. - l - pp;
+    }
+  }
+  ADM_API_END
+}
+
+int adm_get_qp_state(void *ptr, double *liquid, double *powder) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  ctx.get_qp_state(liquid, powder);
+  ADM_API_END
+}
+
+int adm_set_qp_state(void *ptr, const double *liquid, const double *powder) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  ADM_HIP(hipDeviceSynchronize());
+  ctx.set_qp_state(liquid, powder);
+  ADM_API_END
+}
+
+int adm_set_deposition(void *ptr, const double *c, const double *s) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  if (!c || !s) return adm::set_error(ADM_ERR_INVALID, "null argument");
+  ADM_HIP(hipDeviceSynchronize());
+  std::copy(c, c + ctx.n_cells, ctx.dep_cos.begin());
+  std::copy(s, s + ctx.n_cells, ctx.dep_sin.begin());
+  // keep the quadrature-point state: rebuild only the meta words
+  std::vector<double> lq((size_t)ctx.n_cells * ctx.nq),
+      pw((size_t)ctx.n_cells * ctx.nq);
+  ctx.get_qp_state(lq.data(), pw.data());
+  std::vector<double> fpw;
+  if (ctx.n_powder_cells > 0) {
+    fpw.resize((size_t)ctx.n_powder_cells * 6 * ctx.nfq);
+    ADM_HIP(hipMemcpy(fpw.data(), ctx.d_face_powder,
+                      sizeof(double) * fpw.size(), hipMemcpyDeviceToHost));
+  }
+  ctx.rebuild_cells();
+  ctx.set_qp_state(lq.data(), pw.data());
+  if (!fpw.empty())
+    ADM_HIP(hipMemcpy(ctx.d_face_powder, fpw.data(),
+                      sizeof(double) * fpw.size(), hipMemcpyHostToDevice));
+  ADM_API_END
+}
+
+int adm_set_beam_parameters(void *ptr, int beam, double absorption,
+                            double max_power, double radius, double depth) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  if (beam < 0 || beam >= (int)ctx.beams.size())
+    return adm::set_error(ADM_ERR_INVALID, "beam index out of range");
+  auto &d = ctx.beams[beam].desc;
+  d.absorption_efficiency = absorption;
+  d.max_power = max_power;
+  d.radius = radius;
+  d.depth = depth;
+  ADM_API_END
+}
+
+int adm_beam_state_at(void *ptr, int beam, double t, adm_beam_state *out) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  if (beam < 0 || beam >= (int)ctx.beams.size() || !out)
+    return adm::set_error(ADM_ERR_INVALID, "invalid beam query");
+  adm::beam_update(ctx.beams[beam], t, ctx.old_segment[beam], *out);
+  ADM_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,87 @@
+// Host-side context of one thermal operator (one mesh, FE degree, material,
+// heat sources and boundary), the object behind the opaque handle of adm.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/adm.h"
+#include "adm_device.hpp"
+
+namespace adm {
+
+struct AdmError : std::runtime_error {
+  int code;
+  AdmError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+int set_error(int code, const std::string &msg);
+
+struct BeamData {
+  adm_beam_desc desc;
+  std::vector<adm_segment> segments;
+};
+
+void beam_update(const BeamData &b, double t, int &old_segment,
+                 adm_beam_state &s);
+void gauss_points(int n, double *x, double *w);
+void gauss_lobatto_points(int n, double *x, double *w);
+
+struct Context {
+  explicit Context(const adm_problem_desc &d);
+  ~Context();
+  Context(const Context &) = delete;
+  Context &operator=(const Context &) = delete;
+
+  void upload_constants();
+  uint32_t face_mask(int64_t c) const;
+  void rebuild_cells();
+  void set_cell_state(const double *cs);
+  void get_qp_state(double *liquid, double *powder);
+  void set_qp_state(const double *liquid, const double *powder);
+  void set_time(double t);
+  CellArgs cell_args() const;
+  void apply(const double *src, double *dst, hipStream_t s);
+  void fill(double *x, double v, int64_t n, hipStream_t s);
+  void compute_inverse_mass(double *out, hipStream_t s);
+  void ensure_step_buffers(int method);
+  void step(int method, double t, double dt, double *T, hipStream_t s);
+
+  int device = 0;
+  int p = 1, n1 = 2, nq = 8, nfq = 4;
+  int nc[3] = {0, 0, 0}, nn[3] = {0, 0, 0};
+  int64_t n_cells = 0, n_dofs = 0;
+  int n_states = 1;
+  bool use_table = false;
+  double det = 1.0;
+  double wgl[kMaxN] = {0, 0, 0};
+  KConst kc;
+
+  std::vector<uint8_t> active, material;
+  std::vector<double> dep_cos, dep_sin, cell_state;
+  std::vector<uint32_t> meta;
+  std::vector<int32_t> powder_slot;
+  int64_t n_powder_cells = 0;
+
+  std::vector<BeamData> beams;
+  std::vector<int> old_segment;
+  std::vector<adm_beam_state> beam_states;
+  bool sources_on = false;
+
+  // device buffers
+  KConst *d_kc = nullptr;
+  uint32_t *d_meta = nullptr;
+  int32_t *d_powder_slot = nullptr;
+  double *d_cos = nullptr, *d_sin = nullptr, *d_wgl = nullptr;
+  double *d_liquid = nullptr;
+  uint8_t *d_liquid_dirty = nullptr;
+  double *d_powder = nullptr, *d_face_powder = nullptr;
+  double *d_R = nullptr, *d_Y = nullptr, *d_acc = nullptr, *d_F = nullptr;
+  double *d_inv_mass = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+};
+
+}  // namespace adm

@@ -1,0 +1,246 @@
+// Device-side data layout and physics of the matrix-free heat operator.
+//
+// Everything here is read by the HIP kernels; the host builds it once per
+// context. Physics follows the host ThermalOperator of adamantine with one
+// SIMD lane per quadrature point (reference paths relative to its tree):
+//   state update       source/ThermalOperator.templates.hh:227-317
+//   1/(rho c_p)        source/ThermalOperator.templates.hh:415-477
+//   properties         source/MaterialProperty.hh:529-800,
+//                      source/MaterialProperty.templates.hh:1158-1212
+//   heat sources       source/GoldakHeatSource.cc:91-148,
+//                      source/ElectronBeamHeatSource.cc:95-158,
+//                      source/CubeHeatSource.cc:73-107
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/adm.h"
+
+namespace adm {
+
+constexpr int kMaxN = 3;          // fe_degree + 1 <= 3 on the device path
+constexpr int kMaxQ = 27;
+constexpr int kMaxFQ = 9;
+constexpr double kStefanBoltzmann = 5.670374419e-8;  // source/types.hh:158
+
+// cell meta word bits
+constexpr uint32_t kActive = 1u;
+constexpr uint32_t kMaterialShift = 1;   // 2 bits
+constexpr uint32_t kFaceShift = 3;       // 6 bits: faces with a boundary term
+constexpr uint32_t kHasPowder = 1u << 9;
+constexpr uint32_t kRotated = 1u << 10;
+
+// per-(material, property, state) table kind
+constexpr uint8_t kTableGeneral = 0;
+constexpr uint8_t kTableConstant = 1;
+
+struct MatConst {
+  int32_t n_materials, n_states, use_table, p_order, single_layout, pad_;
+  double table[ADM_MAX_MATERIALS][ADM_N_STATE_PROPS][ADM_MAX_STATES]
+              [ADM_TABLE_SIZE][2];
+  double poly[ADM_MAX_MATERIALS][ADM_N_STATE_PROPS][ADM_MAX_STATES]
+             [ADM_MAX_POLY];
+  double prop[ADM_MAX_MATERIALS][ADM_N_PROPS];
+  // latent_heat / (liquidus - solidus), precomputed with the same division
+  double latent_factor[ADM_MAX_MATERIALS];
+  uint8_t kind[ADM_MAX_MATERIALS][ADM_N_STATE_PROPS][ADM_MAX_STATES];
+  // thermal_conductivity_x/y/z identical for this material
+  uint8_t isotropic[ADM_MAX_MATERIALS];
+};
+
+struct KConst {
+  double S[kMaxN][kMaxN];    // shape values  S[q][i] = l_i(x_q)
+  double Dc[kMaxN][kMaxN];   // collocation derivative at the quadrature points
+  double xq[kMaxN];
+  double jxw[kMaxQ];
+  double fjxw[3][kMaxFQ];    // face JxW per normal direction
+  double h[3], inv_h[3], origin[3];
+  int32_t n_boundary_ids;
+  int32_t boundary_type[ADM_MAX_BOUNDARY_IDS];
+  int32_t adiabatic_only;
+  double log_01;             // std::log(0.1) on the host
+  MatConst mat;
+};
+
+// Arguments of the cell kernel (by value).
+struct CellArgs {
+  const uint32_t *meta;       // [n_cells]
+  const int32_t *powder_slot; // [n_cells], -1 without powder storage
+  double *powder;             // [n_powder_cells][n_q]
+  double *face_powder;        // [n_powder_cells][6][n_face_q]
+  double *liquid;             // [n_cells][n_q] (tracked sparsely)
+  uint8_t *liquid_dirty;      // [n_cells]
+  const double *dep_cos;      // [n_cells]
+  const double *dep_sin;
+  const KConst *kc;
+  int32_t nc[3];
+  int32_t nn[3];
+  int32_t n_beams;
+  int32_t sources_on;
+  adm_beam_state beams[ADM_MAX_BEAMS];
+};
+
+// ---------------------------------------------------------------------------
+// material properties
+// ---------------------------------------------------------------------------
+
+// MaterialProperty::compute_property_from_table. Tables are sorted (checked
+// at context creation), so the index of the first breakpoint above T is the
+// number of breakpoints not above T.
+__device__ __forceinline__ double table_value(const double (*t)[2], double T) {
+  if (T <= t[0][0]) return t[0][1];
+  int i = 0;
+#pragma unroll
+  for (int k = 0; k < ADM_TABLE_SIZE; ++k) i += !(T < t[k][0]) ? 1 : 0;
+  if (i >= ADM_TABLE_SIZE - 1) return t[ADM_TABLE_SIZE - 1][1];
+  const double t0 = t[i - 1][0], t1 = t[i][0];
+  const double v0 = t[i - 1][1], v1 = t[i][1];
+  return v0 + (T - t0) * (v1 - v0) / (t1 - t0);
+}
+
+// compute_material_property (vectorized host form). States with a zero ratio
+// add r*X = +-0 in the reference, so skipping them changes no bit.
+template <int NS, bool TABLE>
+__device__ __forceinline__ double property(const MatConst &m, int prop, int mid,
+                                           const double (&r)[NS], double T,
+                                           const double (&tp)[ADM_MAX_POLY]) {
+  double value = 0.0;
+  if constexpr (TABLE) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      if (r[s] == 0.0) continue;
+      double pv;
+      if (m.kind[mid][prop][s] == kTableConstant)
+        pv = m.table[mid][prop][s][0][1];
+      else
+        pv = table_value(m.table[mid][prop][s], T);
+      value += r[s] * pv;
+    }
+  } else {
+    if (m.single_layout) {
+#pragma unroll
+      for (int i = 0; i < ADM_MAX_POLY; ++i) {
+        if (i > m.p_order) break;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+          if (r[s] == 0.0) continue;
+          value += r[s] * m.poly[mid][prop][s][i] * tp[i];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        if (r[s] == 0.0) continue;
+#pragma unroll
+        for (int i = 0; i < ADM_MAX_POLY; ++i) {
+          if (i > m.p_order) break;
+          value += r[s] * m.poly[mid][prop][s][i] * tp[i];
+        }
+      }
+    }
+  }
+  return value;
+}
+
+// temperature powers T^i (source/ThermalOperator.templates.hh:524-530)
+__device__ __forceinline__ void temperature_powers(const MatConst &m, double T,
+                                                   double (&tp)[ADM_MAX_POLY]) {
+  tp[0] = 1.0;
+#pragma unroll
+  for (int i = 1; i < ADM_MAX_POLY; ++i) tp[i] = i <= m.p_order ? tp[i - 1] * T : 0.0;
+}
+
+// update_state_ratios for one quadrature point. Returns true if the powder
+// ratio changed.
+template <int NS>
+__device__ __forceinline__ bool state_ratios(const MatConst &m, int mid,
+                                             double T, double &powder,
+                                             double &liquid,
+                                             double (&r)[NS]) {
+  if constexpr (NS == 1) {
+    r[0] = 1.0;
+    liquid = 0.0;
+    return false;
+  } else {
+    const double solidus = m.prop[mid][ADM_SOLIDUS];
+    const double liquidus = m.prop[mid][ADM_LIQUIDUS];
+    double l;
+    if (T < solidus)
+      l = 0.0;
+    else if (T > liquidus)
+      l = 1.0;
+    else
+      l = (T - solidus) / (liquidus - solidus);
+    liquid = l;
+    if constexpr (NS == 2) {
+      r[1] = l;
+      r[0] = 1.0 - l;
+      return false;
+    } else {
+      const double a = 1.0 - l;
+      const double p = (powder < a) ? powder : a;
+      const bool changed = p != powder;
+      powder = p;
+      r[1] = l;
+      r[2] = p;
+      r[0] = 1.0 - l - p;
+      return changed;
+    }
+  }
+}
+
+template <int NS, bool TABLE>
+__device__ __forceinline__ double inv_rho_cp(const MatConst &m, int mid,
+                                             const double (&r)[NS], double T,
+                                             const double (&tp)[ADM_MAX_POLY]) {
+  const double density = property<NS, TABLE>(m, ADM_DENSITY, mid, r, T, tp);
+  double specific_heat =
+      property<NS, TABLE>(m, ADM_SPECIFIC_HEAT, mid, r, T, tp);
+  if constexpr (NS > 1) {
+    const double is_mushy = r[1] * r[0];
+    if (is_mushy > 0.0) specific_heat += m.latent_factor[mid];
+  }
+  return 1.0 / (density * specific_heat);
+}
+
+// ---------------------------------------------------------------------------
+// heat sources (one lane per point)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double beam_value(const adm_beam_state &s, double x,
+                                             double y, double z_,
+                                             double log_01) {
+  if (s.type == ADM_SOURCE_CUBE) {
+    if (!s.on) return 0.0;
+    if (!(x >= s.cube_min[0] && x <= s.cube_max[0])) return 0.0;
+    if (!(y >= s.cube_min[1] && y <= s.cube_max[1])) return 0.0;
+    if (!(z_ >= s.cube_min[2] && z_ <= s.cube_max[2])) return 0.0;
+    return 1.0 * s.cube_value;
+  }
+  const double z = z_ - s.center[2];
+  if (z + s.depth < 0.0) return 0.0;
+  const double dx = x - s.center[0];
+  const double dy = y - s.center[1];
+  double xpy_squared = dx * dx;
+  xpy_squared += dy * dy;
+  const double xpy_area = 5. * s.radius_squared - xpy_squared;
+  if (xpy_area < 0.) return 0.0;
+  if (s.type == ADM_SOURCE_GOLDAK) {
+    double exponent = z * z * s.inv_depth_squared;
+    exponent += xpy_squared * s.inv_radius_squared;
+    exponent *= -3.0;
+    return 1.0 * s.alpha * exp(exponent);
+  }
+  const double z_rel = z * s.inv_depth;
+  double dist = z_rel * z_rel;
+  dist *= -3.0;
+  dist -= z_rel;
+  dist -= z_rel;
+  dist += 1.0;
+  double exponent = log_01 * xpy_squared;
+  exponent *= s.inv_radius_squared;
+  return 1.0 * s.alpha * exp(exponent) * dist;
+}
+
+}  // namespace adm

@@ -1,0 +1,240 @@
+"""Host-side mirror of adamantine's thermal operator / physics API over the
+C-ABI of libadm.so (include/adm.h).
+
+Class and method names follow the reference (source/ThermalOperatorBase.hh,
+source/ThermalPhysicsInterface.hh) so that code written against
+``ThermalOperatorDevice`` / ``ThermalPhysics<..., MemorySpace::Default>``
+maps one-to-one. Device vectors are torch float64 tensors on the current HIP
+device; torch only provides memory and streams, every operator call runs the
+hand-written HIP kernels of libadm.so.
+
+The library must be built (``python -m adamantine_amd.build``); there is no
+fallback: a missing or unloadable library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from .problem import BeamState, Problem, ProblemDesc
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libadm.so")
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+
+class AdmError(RuntimeError):
+    """Error reported by the C-ABI (the reference's ASSERT_THROW)."""
+
+
+def load_library():
+    """Load libadm.so. Raises if it is missing: the HIP path is the product,
+    there is no CPU fallback."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise AdmError("libadm.so not built: run python -m adamantine_amd.build")
+    L = C.CDLL(LIB_PATH)
+    L.adm_last_error.restype = C.c_char_p
+    L.adm_create.argtypes = [C.POINTER(ProblemDesc), C.POINTER(_vp)]
+    L.adm_destroy.argtypes = [_vp]
+    L.adm_n_dofs.argtypes = [_vp, C.POINTER(C.c_int64)]
+    L.adm_sizes.argtypes = [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                            C.POINTER(C.c_int32)]
+    L.adm_compute_inverse_mass.argtypes = [_vp, _vp, _vp]
+    L.adm_set_time.argtypes = [_vp, C.c_double]
+    L.adm_vmult.argtypes = [_vp, _vp, _vp, C.c_int, _vp]
+    L.adm_evolve.argtypes = [_vp, C.c_int, C.c_double, C.c_double, C.c_int,
+                             _vp, C.c_int, _dp, _vp]
+    L.adm_set_cell_state.argtypes = [_vp, _dp]
+    L.adm_get_cell_state.argtypes = [_vp, _dp]
+    L.adm_get_qp_state.argtypes = [_vp, _dp, _dp]
+    L.adm_set_qp_state.argtypes = [_vp, _dp, _dp]
+    L.adm_set_deposition.argtypes = [_vp, _dp, _dp]
+    L.adm_set_beam_parameters.argtypes = [_vp, C.c_int, C.c_double, C.c_double,
+                                          C.c_double, C.c_double]
+    L.adm_beam_state_at.argtypes = [_vp, C.c_int, C.c_double,
+                                    C.POINTER(BeamState)]
+    _lib = L
+    return L
+
+
+def check(rc: int):
+    if rc != 0:
+        raise AdmError(load_library().adm_last_error().decode())
+
+
+def _stream_ptr(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def _dev_ptr(t):
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and
+            t.dtype == torch.float64 and t.is_contiguous()):
+        raise AdmError("device vectors must be contiguous float64 HIP tensors")
+    return C.c_void_p(t.data_ptr())
+
+
+def _host(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class ThermalOperatorDevice:
+    """Drop-in counterpart of ThermalOperatorDevice
+    (source/ThermalOperatorDevice.hh) with the host operator's semantics."""
+
+    def __init__(self, problem: Problem):
+        L = load_library()
+        self.problem = problem
+        self._desc, self._keep = problem.to_c()
+        h = _vp()
+        check(L.adm_create(C.byref(self._desc), C.byref(h)))
+        self._h = h
+        n = C.c_int64()
+        check(L.adm_n_dofs(h, C.byref(n)))
+        self._n_dofs = n.value
+        nc, nq, nfq = C.c_int64(), C.c_int32(), C.c_int32()
+        check(L.adm_sizes(h, C.byref(nc), C.byref(nq), C.byref(nfq)))
+        self.n_cells, self.n_q, self.n_face_q = nc.value, nq.value, nfq.value
+        self._inverse_mass = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            load_library().adm_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    # ThermalOperatorBase ---------------------------------------------------
+    def m(self) -> int:
+        return self._n_dofs
+
+    def n(self) -> int:
+        return self._n_dofs
+
+    def initialize_dof_vector(self, value: float = 0.0, device=None):
+        import torch
+        return torch.full((self._n_dofs,), float(value), dtype=torch.float64,
+                          device=device or "cuda")
+
+    def vmult(self, dst, src, stream=None):
+        check(load_library().adm_vmult(self._h, _dev_ptr(src), _dev_ptr(dst), 0,
+                                       _stream_ptr(stream)))
+
+    def vmult_add(self, dst, src, stream=None):
+        check(load_library().adm_vmult(self._h, _dev_ptr(src), _dev_ptr(dst), 1,
+                                       _stream_ptr(stream)))
+
+    def compute_inverse_mass_matrix(self, stream=None):
+        self._inverse_mass = self.initialize_dof_vector()
+        check(load_library().adm_compute_inverse_mass(
+            self._h, _dev_ptr(self._inverse_mass), _stream_ptr(stream)))
+
+    def get_inverse_mass_matrix(self):
+        if self._inverse_mass is None:
+            self.compute_inverse_mass_matrix()
+        return self._inverse_mass
+
+    def set_time(self, t: float):
+        check(load_library().adm_set_time(self._h, float(t)))
+
+    def get_state_from_material_properties(self, cell_state):
+        check(load_library().adm_set_cell_state(
+            self._h, _host(cell_state).ctypes.data_as(_dp)))
+
+    def set_state_to_material_properties(self):
+        ns = self.problem.material.n_states
+        out = np.zeros((self.n_cells, ns))
+        check(load_library().adm_get_cell_state(self._h, out.ctypes.data_as(_dp)))
+        return out
+
+    def set_material_deposition_orientation(self, deposition_cos,
+                                            deposition_sin):
+        c = _host(deposition_cos)
+        s = _host(deposition_sin)
+        check(load_library().adm_set_deposition(self._h, c.ctypes.data_as(_dp),
+                                                s.ctypes.data_as(_dp)))
+
+    # quadrature-point state --------------------------------------------------
+    def qp_state(self):
+        liq = np.zeros((self.n_cells, self.n_q))
+        pw = np.zeros((self.n_cells, self.n_q))
+        check(load_library().adm_get_qp_state(self._h, liq.ctypes.data_as(_dp),
+                                              pw.ctypes.data_as(_dp)))
+        return liq, pw
+
+    def set_qp_state(self, liquid=None, powder=None):
+        lq = None if liquid is None else _host(liquid)
+        pw = None if powder is None else _host(powder)
+        check(load_library().adm_set_qp_state(
+            self._h, None if lq is None else lq.ctypes.data_as(_dp),
+            None if pw is None else pw.ctypes.data_as(_dp)))
+
+    def beam_state_at(self, beam: int, t: float) -> BeamState:
+        s = BeamState()
+        check(load_library().adm_beam_state_at(self._h, beam, float(t),
+                                               C.byref(s)))
+        return s
+
+    def set_beam_parameters(self, beam, absorption_efficiency, max_power,
+                            radius, depth):
+        check(load_library().adm_set_beam_parameters(
+            self._h, int(beam), float(absorption_efficiency), float(max_power),
+            float(radius), float(depth)))
+
+
+class ThermalPhysicsDevice:
+    """Counterpart of ThermalPhysics<dim, ..., MemorySpace::Default>
+    (source/ThermalPhysics.hh) for the explicit steppers."""
+
+    def __init__(self, problem: Problem, method: Optional[int] = None):
+        self.problem = problem
+        self.method = problem.method if method is None else method
+        self.operator = ThermalOperatorDevice(problem)
+
+    def setup(self):
+        """setup(): DoFs, deposition orientation, inverse mass, state
+        (source/ThermalPhysics.templates.hh:496-506)."""
+        self.operator.compute_inverse_mass_matrix()
+
+    def initialize_dof_vector(self, value: float = 0.0):
+        return self.operator.initialize_dof_vector(value)
+
+    def evolve_one_time_step(self, t: float, delta_t: float, solution,
+                             stream=None) -> float:
+        return self.evolve(t, delta_t, 1, solution, stream=stream)
+
+    def evolve(self, t: float, delta_t: float, n_steps: int, solution,
+               use_graph: bool = False, stream=None) -> float:
+        t_end = C.c_double()
+        check(load_library().adm_evolve(
+            self.operator.handle, int(self.method), float(t), float(delta_t),
+            int(n_steps), _dev_ptr(solution), int(use_graph), C.byref(t_end),
+            _stream_ptr(stream)))
+        return t_end.value
+
+    def update_physics_parameters(self, beam: int, absorption_efficiency,
+                                  max_power, radius, depth):
+        """update_physics_parameters (source/ThermalPhysics.templates.hh:
+        827-852) for one beam."""
+        self.operator.set_beam_parameters(beam, absorption_efficiency,
+                                          max_power, radius, depth)
+
+    def get_state_from_material_properties(self, cell_state):
+        self.operator.get_state_from_material_properties(cell_state)
+
+    def set_state_to_material_properties(self):
+        return self.operator.set_state_to_material_properties()

@@ -1,0 +1,133 @@
+"""The five workloads of the MI355X heat path, as Problem builders.
+
+Every choice the workload statements leave open is fixed here (seeded RNGs
+are numpy's legacy RandomState streams, which numpy keeps stable):
+
+1. ``plain_small``  32x16x8 Q1 box 2x1x0.5 mm, Solid, polynomial k(T),
+   Goldak 200 W moving along x at 1 m/s, forward Euler 1e-6 s x 100.
+2. ``plain_large``  128x64x32 Q2 box 4x2x1 mm (2,154,945 nodes), SLP with
+   8-breakpoint k(T), c_p(T) tables, top 4 cell layers powder, Goldak 400 W
+   on a 5-track serpentine, radiative + convective top face, RK4 5e-7 s.
+3. ``ensemble``     64x32x16 Q2, same material model, 64 members.
+4. ``adjoint``      64x32x16 Q2, 200 RK4 steps.
+5. ``deposition``   96x96x40 Q2, electron beam, layer activation.
+"""
+from __future__ import annotations
+
+from typing import List
+
+import numpy as np
+
+from .problem import (BC_ADIABATIC, BC_CONVECTIVE, BC_RADIATIVE,
+                      FORWARD_EULER, RK_FOURTH_ORDER, SOURCE_ELECTRON_BEAM,
+                      SOURCE_GOLDAK, Beam, Material, Problem,
+                      segments_from_path)
+
+MM = 1e-3
+UM = 1e-6
+
+
+def plain_small(n_cells=(32, 16, 8)) -> Problem:
+    """Q1, Solid, rho = 7904, c_p = 714, k = 11.8 + 0.0138 T; Goldak 200 W,
+    absorption 0.3, 50 um radius and depth, along x at mid-y on the top
+    surface at 1 m/s; adiabatic; T0 = 300 K."""
+    m = Material(n_materials=1, n_states=1, use_table=False, p_order=1,
+                 single_material_layout=True)
+    m.set_poly(0, "density", "solid", [7904.0])
+    m.set_poly(0, "specific_heat", "solid", [714.0])
+    for k in ("thermal_conductivity_x", "thermal_conductivity_y",
+              "thermal_conductivity_z"):
+        m.set_poly(0, k, "solid", [11.8, 0.0138])
+    path = segments_from_path([
+        {"mode": "point", "xyz": (0.0, 0.5 * MM, 0.5 * MM), "pmod": 0.0,
+         "param": 1e-9},
+        {"mode": "line", "xyz": (2.0 * MM, 0.5 * MM, 0.5 * MM), "pmod": 1.0,
+         "param": 1.0},
+    ])
+    beam = Beam(type=SOURCE_GOLDAK, absorption_efficiency=0.3, max_power=200.0,
+                radius=50 * UM, depth=50 * UM, segments=path)
+    return Problem(n_cells=tuple(n_cells), extent=(2 * MM, 1 * MM, 0.5 * MM),
+                   fe_degree=1, quadrature=0, dim=3, material=m,
+                   boundary_types=[BC_ADIABATIC] * 6, beams=[beam],
+                   method=FORWARD_EULER, initial_temperature=300.0)
+
+
+def slp_tables(seed_breaks: int = 2, seed_k: int = 3, seed_cp: int = 4,
+               k_scale: float = 1.0) -> Material:
+    """SolidLiquidPowder tables: 8 breakpoints from 300 K to 3000 K (6 sorted
+    uniform interior draws for k and 6 for c_p), k = 11.8 + 0.0138 T + U(-1,1),
+    c_p = 714 + 0.1 (T - 300) + U(-10,10); liquid uses the solid tables;
+    powder has 0.6 rho and 0.01 k; solidus 1675 K, liquidus 1708 K, latent
+    heat 2.6e5 J/kg; emissivity 0.3 and h = 10 W/m^2K; T_inf = 300 K."""
+    rb = np.random.RandomState(seed_breaks)
+    tk = np.concatenate([[300.0], np.sort(rb.uniform(300.0, 3000.0, 6)),
+                         [3000.0]])
+    tc = np.concatenate([[300.0], np.sort(rb.uniform(300.0, 3000.0, 6)),
+                         [3000.0]])
+    kv = 11.8 + 0.0138 * tk + np.random.RandomState(seed_k).uniform(-1, 1, 8)
+    cv = 714.0 + 0.1 * (tc - 300.0) + \
+        np.random.RandomState(seed_cp).uniform(-10, 10, 8)
+    m = Material(n_materials=1, n_states=3, use_table=True, p_order=0,
+                 single_material_layout=True)
+    for state, rho_f, k_f in (("solid", 1.0, 1.0), ("liquid", 1.0, 1.0),
+                              ("powder", 0.6, 0.01)):
+        m.set_table(0, "density", state, [(300.0, 7904.0 * rho_f)])
+        m.set_table(0, "specific_heat", state, list(zip(tc, cv)))
+        for kname in ("thermal_conductivity_x", "thermal_conductivity_y",
+                      "thermal_conductivity_z"):
+            m.set_table(0, kname, state,
+                        list(zip(tk, kv * k_f * k_scale)))
+        m.set_table(0, "emissivity", state, [(300.0, 0.3)])
+        m.set_table(0, "convection_heat_transfer_coef", state, [(300.0, 10.0)])
+    m.set_property(0, "solidus", 1675.0)
+    m.set_property(0, "liquidus", 1708.0)
+    m.set_property(0, "latent_heat", 2.6e5)
+    m.set_property(0, "radiation_temperature_infty", 300.0)
+    m.set_property(0, "convection_temperature_infty", 300.0)
+    return m
+
+
+def serpentine(n_tracks=5, x0=0.5 * MM, x1=3.5 * MM, y0=0.8 * MM,
+               hatch=0.1 * MM, z=1.0 * MM, speed=0.8) -> List[tuple]:
+    """Tracks along x at y0 + k*hatch, alternating direction; the hatch jumps
+    are beam-off line segments."""
+    pts = [{"mode": "point", "xyz": (x0, y0, z), "pmod": 0.0, "param": 1e-9}]
+    x_end = x0
+    for k in range(n_tracks):
+        y = y0 + k * hatch
+        if k > 0:
+            pts.append({"mode": "line", "xyz": (x_end, y, z), "pmod": 0.0,
+                        "param": speed})
+        x_end = x1 if k % 2 == 0 else x0
+        pts.append({"mode": "line", "xyz": (x_end, y, z), "pmod": 1.0,
+                    "param": speed})
+    return segments_from_path(pts)
+
+
+def plain_large(n_cells=(128, 64, 32), extent=(4 * MM, 2 * MM, 1 * MM),
+                powder_layers: int = 4, method=RK_FOURTH_ORDER) -> Problem:
+    m = slp_tables()
+    beam = Beam(type=SOURCE_GOLDAK, absorption_efficiency=0.3, max_power=400.0,
+                radius=100 * UM, depth=60 * UM,
+                segments=serpentine(z=extent[2]))
+    p = Problem(n_cells=tuple(n_cells), extent=tuple(extent), fe_degree=2,
+                quadrature=0, dim=3, material=m,
+                boundary_types=[BC_ADIABATIC] * 5 +
+                [BC_RADIATIVE | BC_CONVECTIVE],
+                beams=[beam], method=method, initial_temperature=300.0)
+    hz = extent[2] / n_cells[2]
+    p.set_powder_layer(extent[2], powder_layers * hz)
+    return p
+
+
+def random_temperature(problem: Problem, seed: int = 1, low=300.0,
+                       high=2000.0) -> np.ndarray:
+    return np.random.RandomState(seed).uniform(low, high, problem.n_dofs)
+
+
+def smooth_temperature(problem: Problem) -> np.ndarray:
+    """T = 300 + 50 sin(pi x / L) cos(pi y / W) (+ a z ramp)."""
+    X = problem.node_coordinates()
+    L, W, H = problem.extent
+    return (300.0 + 50.0 * np.sin(np.pi * X[:, 0] / L) *
+            np.cos(np.pi * X[:, 1] / W) + 10.0 * X[:, 2] / H)

@@ -1,0 +1,283 @@
+/*
+ * adm.h -- C-ABI of the MI355X-native matrix-free heat-equation path.
+ *
+ * This header is the drop-in boundary that replaces the *Dev instantiations of
+ * adamantine's ThermalOperatorDevice / ThermalPhysics (memory_space device).
+ * Every entry point below names the reference interface it replaces
+ * (paths relative to the adamantine source tree).
+ *
+ * Conventions
+ *   - every function returns int: 0 = success, non-zero = error code;
+ *     adm_last_error() returns a message for the calling thread's last error
+ *     (the reference throws std::runtime_error through ASSERT_THROW,
+ *     source/utils.hh:70-74; no C++ exception crosses this boundary);
+ *   - device buffers are caller-owned (hipMalloc / torch) and passed as plain
+ *     pointers; host buffers are marked "host";
+ *   - work is enqueued on the hipStream_t passed as `stream` (NULL = default
+ *     stream) and is stream-ordered; functions that return data to host
+ *     buffers synchronise the stream before returning;
+ *   - one context per (mesh, FE degree, material, sources, boundary) like the
+ *     reference's ThermalOperator; a context is not thread-safe.
+ *
+ * DoF numbering: the mesh is the structured box of
+ * dealii::GridGenerator::subdivided_hyper_rectangle (source/Geometry.cc:252).
+ * Nodes of the FE_Q(p) space (Gauss-Lobatto support points) are numbered
+ * lexicographically, x fastest:  node = ix + Nx*(iy + Ny*iz),
+ * N_d = p*n_cells_d + 1.  Cells are numbered c = cx + nx*(cy + ny*cz).
+ * INTEGRATION.md gives the permutation to deal.II's DoF order.
+ */
+#ifndef ADM_H
+#define ADM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ADM_MAX_MATERIALS 4
+#define ADM_TABLE_SIZE 12     /* MaterialProperty::table_size */
+#define ADM_N_STATE_PROPS 8   /* thermal StateProperty entries, source/types.hh:40-64 */
+#define ADM_N_PROPS 5         /* Property entries, source/types.hh:88-96 */
+#define ADM_MAX_STATES 3
+#define ADM_MAX_POLY 5        /* p_order <= 4, source/MaterialProperty.hh:543 */
+#define ADM_MAX_BEAMS 4
+#define ADM_MAX_BOUNDARY_IDS 8
+
+/* StateProperty (source/types.hh:40-49) */
+enum adm_state_property {
+  ADM_DENSITY = 0,
+  ADM_SPECIFIC_HEAT = 1,
+  ADM_THERMAL_CONDUCTIVITY_X = 2,
+  ADM_THERMAL_CONDUCTIVITY_Y = 3,
+  ADM_THERMAL_CONDUCTIVITY_Z = 4,
+  ADM_EMISSIVITY = 5,
+  ADM_RADIATION_HEAT_TRANSFER_COEF = 6,
+  ADM_CONVECTION_HEAT_TRANSFER_COEF = 7
+};
+
+/* Property (source/types.hh:88-96) */
+enum adm_property {
+  ADM_LIQUIDUS = 0,
+  ADM_SOLIDUS = 1,
+  ADM_LATENT_HEAT = 2,
+  ADM_RADIATION_TEMPERATURE_INFTY = 3,
+  ADM_CONVECTION_TEMPERATURE_INFTY = 4
+};
+
+/* BoundaryType bits (source/Boundary.hh:21-29) */
+enum adm_boundary_type {
+  ADM_BC_ADIABATIC = 0x1,
+  ADM_BC_RADIATIVE = 0x2,
+  ADM_BC_CONVECTIVE = 0x4
+};
+
+/* heat source kinds (source/ThermalPhysics.templates.hh:390-410) */
+enum adm_source_type {
+  ADM_SOURCE_GOLDAK = 0,
+  ADM_SOURCE_ELECTRON_BEAM = 1,
+  ADM_SOURCE_CUBE = 2
+};
+
+/* explicit time steppers (source/ThermalPhysics.templates.hh:456-471) */
+enum adm_time_stepping {
+  ADM_FORWARD_EULER = 0,
+  ADM_RK_THIRD_ORDER = 1,
+  ADM_RK_FOURTH_ORDER = 2
+};
+
+/* error codes */
+enum adm_status {
+  ADM_OK = 0,
+  ADM_ERR_INVALID = 1,   /* invalid argument or descriptor */
+  ADM_ERR_HIP = 2,       /* HIP runtime error */
+  ADM_ERR_UNSUPPORTED = 3,
+  ADM_ERR_STATE = 4      /* call out of order (e.g. vmult before create) */
+};
+
+/* Material model: the data of MaterialProperty after fill_properties
+ * (source/MaterialProperty.templates.hh:796-1119), i.e. tables padded to 12
+ * entries with the last pair, missing tables zero, missing state-independent
+ * properties DBL_MAX. */
+typedef struct adm_material_desc {
+  int32_t n_materials;   /* runtime number of materials (1..4) */
+  int32_t n_states;      /* 1 Solid, 2 SolidLiquid, 3 SolidLiquidPowder */
+  int32_t use_table;     /* property_format table (1) or polynomial (0) */
+  int32_t p_order;       /* polynomial order template parameter (0..4) */
+  int32_t single_material_layout; /* 1: the reference's n_materials == 1
+                                      instantiation (power-major polynomial
+                                      sum, MaterialProperty.hh:572-777);
+                                      0: n_materials == -1 (state-major,
+                                      MaterialProperty.hh:778-796) */
+  int32_t pad_;
+  double table[ADM_MAX_MATERIALS][ADM_N_STATE_PROPS][ADM_MAX_STATES]
+              [ADM_TABLE_SIZE][2];
+  double poly[ADM_MAX_MATERIALS][ADM_N_STATE_PROPS][ADM_MAX_STATES]
+             [ADM_MAX_POLY];
+  double prop[ADM_MAX_MATERIALS][ADM_N_PROPS];
+} adm_material_desc;
+
+/* Structured box mesh (source/Geometry.cc:186-260) and FE space
+ * (source/ThermalPhysics.templates.hh:364-369). */
+typedef struct adm_mesh_desc {
+  int32_t dim;           /* 3 (the CPU oracle also accepts 2: axes x, z) */
+  int32_t fe_degree;     /* 1..4 */
+  int32_t quadrature;    /* 0 = QGauss(p+1), 1 = QGaussLobatto(p+1) */
+  int32_t n_cells[3];    /* length_divisions, width_divisions, height_divisions */
+  double origin[3];
+  double extent[3];      /* length, width, height */
+} adm_mesh_desc;
+
+/* One scan-path segment (source/ScanPath.hh:40-48); end_time is absolute. */
+typedef struct adm_segment {
+  double end_time;
+  double power_modifier;
+  double end_point[3];
+} adm_segment;
+
+/* One heat source (source/BeamHeatSourceProperties.hh:55-66,
+ * source/CubeHeatSource.cc:31-42). */
+typedef struct adm_beam_desc {
+  int32_t type;          /* adm_source_type */
+  int32_t n_segments;
+  double absorption_efficiency;
+  double max_power;
+  double radius;         /* diameter / 2 */
+  double depth;
+  /* cube source */
+  double start_time, end_time, value;
+  double min_point[3], max_point[3];
+  const adm_segment *segments;  /* host; n_segments entries */
+} adm_beam_desc;
+
+/* A beam evaluated at one time (HeatSource::update_time,
+ * source/GoldakHeatSource.cc:24-44, source/ElectronBeamHeatSource.cc:24-45). */
+typedef struct adm_beam_state {
+  int32_t type;
+  int32_t on;            /* power modifier > 0 (is_source_on) */
+  double center[3];
+  double alpha;
+  double depth;
+  double inv_depth;
+  double inv_depth_squared;
+  double radius_squared;
+  double inv_radius_squared;
+  double cube_value;
+  double cube_min[3];
+  double cube_max[3];
+} adm_beam_state;
+
+/* Everything the reference's ThermalPhysics/ThermalOperator constructors and
+ * setup() gather from Geometry, Boundary, MaterialProperty and the sources.
+ * Per-cell arrays are host arrays in cell order, NULL meaning the default. */
+typedef struct adm_problem_desc {
+  adm_mesh_desc mesh;
+  adm_material_desc material;
+  int32_t n_boundary_ids;                       /* 2*dim for a box */
+  int32_t boundary_type[ADM_MAX_BOUNDARY_IDS];  /* Boundary::_types */
+  int32_t n_beams;
+  int32_t pad_;
+  adm_beam_desc beams[ADM_MAX_BEAMS];
+  const uint8_t *cell_active;     /* 1 = FE_Q, 0 = FE_Nothing; NULL: all 1 */
+  const uint8_t *cell_material;   /* material_id; NULL: 0 */
+  const double *cell_state;       /* [n_cells][n_states] ratios; NULL: solid */
+  const double *deposition_cos;   /* NULL: 1 */
+  const double *deposition_sin;   /* NULL: 0 */
+} adm_problem_desc;
+
+/* ------------------------------------------------------------------------ */
+/* Context life cycle                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces the ThermalOperatorDevice constructor + reinit() +
+ * set_material_deposition_orientation() + get_state_from_material_properties()
+ * as called by ThermalPhysics::setup() (source/ThermalPhysics.templates.hh:
+ * 496-524, source/ThermalOperatorDevice.templates.hh:520-579). Uses the
+ * current HIP device. */
+int adm_create(const adm_problem_desc *desc, void **ctx);
+
+/* Replaces ThermalOperatorDevice::clear() + destruction
+ * (source/ThermalOperatorDevice.templates.hh:631-640). */
+int adm_destroy(void *ctx);
+
+/* Message of the calling thread's last error (ASSERT_THROW text). */
+const char *adm_last_error(void);
+
+/* m() / n() (source/ThermalOperatorBase.hh:26-32): number of nodes of the
+ * structured numbering (nodes of inactive cells included; they are inert). */
+int adm_n_dofs(const void *ctx, int64_t *n_dofs);
+
+/* Number of cells and of quadrature points per cell. */
+int adm_sizes(const void *ctx, int64_t *n_cells, int32_t *n_q_per_cell,
+              int32_t *n_face_q);
+
+/* ------------------------------------------------------------------------ */
+/* Operator                                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* compute_inverse_mass_matrix() + get_inverse_mass_matrix()
+ * (source/ThermalOperator.templates.hh:118-164): Gauss-Lobatto lumped mass
+ * over active cells, inverted. Nodes touched by no active cell get 0 (the
+ * guard of source/ThermalOperatorDevice.templates.hh:624-627).
+ * inv_mass: device, n_dofs doubles. */
+int adm_compute_inverse_mass(void *ctx, double *inv_mass, void *stream);
+
+/* set_time(t) (source/ThermalOperator.hh:298-308): evaluates every beam at t
+ * for the next vmult. */
+int adm_set_time(void *ctx, double t);
+
+/* vmult / vmult_add (source/ThermalOperator.templates.hh:176-225):
+ * dst = A(src) (add = 0) or dst += A(src) (add = 1), with A the host
+ * ThermalOperator's cell term (diffusion + heat sources, per-quadrature-point
+ * 1/(rho c_p)) plus its face term on non-adiabatic boundaries of the active
+ * domain. Mutates the per-quadrature-point liquid/powder state like the
+ * reference's cell_local_apply. src, dst: device, n_dofs doubles. */
+int adm_vmult(void *ctx, const double *src, double *dst, int add,
+              void *stream);
+
+/* Repeated evolve_one_time_step (source/ThermalPhysics.templates.hh:854-888):
+ * n_steps steps of `method` from time t with step dt on the device solution
+ * vector T (n_dofs doubles, updated in place). With use_graph != 0 the step
+ * loop is captured once in a hipGraph and replayed. t_end (host, may be NULL)
+ * receives the final time. */
+int adm_evolve(void *ctx, int method, double t, double dt, int n_steps,
+               double *T, int use_graph, double *t_end, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* State                                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* get_state_from_material_properties (source/ThermalOperator.templates.hh:
+ * 753-884): set every quadrature point (cells and boundary faces) from the
+ * per-cell state ratios (host, [n_cells][n_states]). */
+int adm_set_cell_state(void *ctx, const double *cell_state);
+
+/* set_state_to_material_properties (source/MaterialProperty.templates.hh:
+ * 537-733): cell averages of the quadrature-point ratios
+ * (host out, [n_cells][n_states]; inactive cells get zeros). */
+int adm_get_cell_state(void *ctx, double *cell_state);
+
+/* Per-quadrature-point liquid and powder ratios (host, [n_cells][n_q];
+ * either pointer may be NULL). Inactive cells read as zero. */
+int adm_get_qp_state(void *ctx, double *liquid, double *powder);
+int adm_set_qp_state(void *ctx, const double *liquid, const double *powder);
+
+/* set_material_deposition_orientation (source/ThermalOperator.templates.hh:
+ * 897-940): per-cell cos/sin (host, n_cells each). */
+int adm_set_deposition(void *ctx, const double *cos_, const double *sin_);
+
+/* Beam parameter update (ThermalPhysics::update_physics_parameters,
+ * source/ThermalPhysics.templates.hh:827-852): new absorption efficiency,
+ * max power, radius and depth for beam `beam`. */
+int adm_set_beam_parameters(void *ctx, int beam, double absorption_efficiency,
+                            double max_power, double radius, double depth);
+
+/* Evaluate beam `beam` of the context at time t on the host, with the
+ * forward-only segment search of ScanPath (source/ScanPath.cc:197-275). */
+int adm_beam_state_at(void *ctx, int beam, double t, adm_beam_state *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ADM_H */

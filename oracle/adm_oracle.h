@@ -1,0 +1,77 @@
+/*
+ * adm_oracle.h -- CPU restatement of adamantine's host ThermalOperator /
+ * ThermalPhysics path (TEST INFRASTRUCTURE: the checker for the HIP library,
+ * never part of the product path).
+ *
+ * Semantics are those of the host (MemorySpace::Host) classes with one SIMD
+ * lane (VectorizedArray width 1): the powder update of
+ * source/ThermalOperator.templates.hh:294 and the heat-source cut-off of
+ * source/GoldakHeatSource.cc:129-139 act per quadrature point.
+ */
+#ifndef ADM_ORACLE_H
+#define ADM_ORACLE_H
+
+#include "../include/adm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle oracle_t;
+
+oracle_t *oracle_create(const adm_problem_desc *desc);
+void oracle_destroy(oracle_t *o);
+int64_t oracle_n_dofs(const oracle_t *o);
+int64_t oracle_n_cells(const oracle_t *o);
+int oracle_n_q(const oracle_t *o);
+int oracle_n_face_q(const oracle_t *o);
+void oracle_set_threads(int n);
+
+/* ThermalOperator::compute_inverse_mass_matrix result (n_dofs; 0 on nodes of
+ * no active cell). */
+void oracle_inverse_mass(const oracle_t *o, double *out);
+/* ThermalOperator::set_time */
+void oracle_set_time(oracle_t *o, double t);
+/* ThermalOperator::vmult (add = 0) / vmult_add (add = 1) */
+void oracle_vmult(oracle_t *o, const double *src, double *dst, int add);
+/* evaluate_thermal_physics: out = Minv * A(y) at time t */
+void oracle_evaluate(oracle_t *o, double t, const double *y, double *out);
+/* evolve_one_time_step repeated n_steps times; returns the final time */
+double oracle_evolve(oracle_t *o, int method, double t, double dt, int n_steps,
+                     double *T);
+
+void oracle_get_qp_state(const oracle_t *o, double *liquid, double *powder);
+void oracle_set_qp_state(oracle_t *o, const double *liquid,
+                         const double *powder);
+void oracle_get_face_powder(const oracle_t *o, double *face_powder);
+void oracle_set_cell_state(oracle_t *o, const double *cell_state);
+void oracle_get_cell_state(const oracle_t *o, double *cell_state);
+void oracle_set_deposition(oracle_t *o, const double *c, const double *s);
+void oracle_set_beam_parameters(oracle_t *o, int beam, double absorption,
+                                double max_power, double radius, double depth);
+
+/* ScanPath / HeatSource pieces, exposed for the reference's unit tests */
+void oracle_beam_update(const adm_beam_desc *b, double t, int *old_segment,
+                        adm_beam_state *out);
+void oracle_scan_path_position(const adm_beam_desc *b, double t,
+                               int *old_segment, double out[3]);
+double oracle_scan_path_power_modifier(const adm_beam_desc *b, double t,
+                                       int *old_segment);
+double oracle_beam_value(const adm_beam_state *s, int dim, const double *pt);
+/* MaterialProperty::compute_material_property (vectorized host form) */
+double oracle_material_property(const adm_material_desc *m, int prop,
+                                int material_id, const double *state_ratios,
+                                double T);
+double oracle_property_from_table(const adm_material_desc *m, int material_id,
+                                  int prop, int state, double T);
+
+/* 1-D tables used by the operator (n = fe_degree + 1 entries each) */
+void oracle_basis_1d(const oracle_t *o, double *support, double *qpoints,
+                     double *qweights, double *shape, double *coll_deriv,
+                     double *gl_weights);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
