@@ -14,8 +14,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libadm.so")
 
 # translation units: the API plus one per cell-kernel configuration
-# (fe_degree, number of material states, table/polynomial properties)
-CELL_CONFIGS = [(p, ns, t) for p in (1, 2) for ns in (1, 2, 3) for t in (0, 1)]
+# (fe_degree, number of material states, property evaluator)
+CELL_CONFIGS = [(p, ns, k) for p in (1, 2) for ns in (1, 2, 3) for k in (0, 1, 2)]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -56,7 +56,7 @@ def build_library(verbose: bool = True, jobs: int = 0) -> str:
     for (p, ns, t) in CELL_CONFIGS:
         obj = os.path.join(objdir, "adm_cell_p%d_s%d_t%d.o" % (p, ns, t))
         jobs_list.append(([hipcc] + HIPCC_FLAGS + inc + [
-            "-DADM_P=%d" % p, "-DADM_NS=%d" % ns, "-DADM_TABLE=%d" % t,
+            "-DADM_P=%d" % p, "-DADM_NS=%d" % ns, "-DADM_PK=%d" % t,
             "-c", "-o", obj, os.path.join(CSRC, "adm_cell_inst.hip")],
             verbose))
         objs.append(obj)

@@ -193,6 +193,78 @@ static bool table_sorted(const double (*t)[2]) {
   return true;
 }
 
+// A table has the regular form when its temperatures increase strictly up to
+// some entry and the remaining entries repeat that last pair (the padding of
+// MaterialProperty::fill_properties, source/MaterialProperty.templates.hh:
+// 975-1004).
+static bool table_regular(const double (*t)[2]) {
+  int n = 1;
+  while (n < ADM_TABLE_SIZE && t[n][0] > t[n - 1][0]) ++n;
+  for (int k = n; k < ADM_TABLE_SIZE; ++k)
+    if (t[k][0] != t[n - 1][0] || t[k][1] != t[n - 1][1]) return false;
+  return true;
+}
+
+// Merged piecewise-linear form (see MergedTable): in interval j, i.e.
+// [B_{j-1}, B_j), each table's index in compute_property_from_table is fixed,
+// so the table is (value at B_{j-1}) + (T - B_{j-1}) * slope.
+void Context::build_merged_tables() {
+  MatConst &mc = kc.mat;
+  mc.merged_ok = mc.use_table ? 1 : 0;
+  if (!mc.use_table) return;
+  for (int mid = 0; mid < mc.n_materials; ++mid) {
+    std::vector<double> B;
+    for (int pr = 0; pr < ADM_N_STATE_PROPS; ++pr)
+      for (int s = 0; s < mc.n_states; ++s) {
+        const double(*t)[2] = mc.table[mid][pr][s];
+        if (!table_regular(t)) mc.merged_ok = 0;
+        for (int k = 0; k < ADM_TABLE_SIZE; ++k) B.push_back(t[k][0]);
+      }
+    std::sort(B.begin(), B.end());
+    B.erase(std::unique(B.begin(), B.end()), B.end());
+    if ((int)B.size() > kMergedMax - 1) {
+      mc.merged_ok = 0;
+      continue;
+    }
+    MergedTable &mt = mc.merged[mid];
+    const int nB = (int)B.size();
+    for (int k = 0; k < kMergedMax; ++k)
+      mt.B[k] = k < nB ? B[k] : HUGE_VAL;
+    for (int pr = 0; pr < ADM_N_STATE_PROPS; ++pr)
+      for (int j = 0; j < kMergedMax; ++j)
+        for (int s = 0; s < ADM_MAX_STATES; ++s) {
+          double A = 0.0, S = 0.0;
+          if (s < mc.n_states && j <= nB) {
+            const double(*t)[2] = mc.table[mid][pr][s];
+            if (j == 0) {
+              A = t[0][1];
+            } else {
+              const double Tl = B[j - 1];
+              int i = 0;
+              for (int k = 0; k < ADM_TABLE_SIZE; ++k) i += t[k][0] <= Tl;
+              if (i == 0) {
+                A = t[0][1];
+              } else if (i >= ADM_TABLE_SIZE - 1) {
+                A = t[ADM_TABLE_SIZE - 1][1];
+              } else {
+                S = (t[i][1] - t[i - 1][1]) / (t[i][0] - t[i - 1][0]);
+                A = (double)((long double)t[i - 1][1] +
+                             ((long double)Tl - (long double)t[i - 1][0]) *
+                                 (long double)S);
+              }
+            }
+          }
+          mt.coef[pr][j][s][0] = A;
+          mt.coef[pr][j][s][1] = S;
+        }
+  }
+}
+
+int Context::property_kind() const {
+  if (!use_table) return 0;
+  return kc.mat.merged_ok ? 2 : 1;
+}
+
 Context::Context(const adm_problem_desc &d) {
   const adm_mesh_desc &m = d.mesh;
   if (m.dim != 3)
@@ -325,6 +397,7 @@ Context::Context(const adm_problem_desc &d) {
     }
     mc.isotropic[mid] = iso ? 1 : 0;
   }
+  build_merged_tables();
 
   // ---- per-cell data --------------------------------------------------------
   active.assign(n_cells, 1);
@@ -398,6 +471,7 @@ Context::~Context() {
   ADM_FREE(d_liquid_dirty);
   ADM_FREE(d_powder);
   ADM_FREE(d_face_powder);
+  ADM_FREE(d_faces);
   ADM_FREE(d_R);
   ADM_FREE(d_Y);
   ADM_FREE(d_acc);
@@ -443,11 +517,29 @@ void Context::rebuild_cells() {
   meta.assign(n_cells, 0);
   powder_slot.assign(n_cells, -1);
   n_powder_cells = 0;
+  faces.clear();
   for (int64_t c = 0; c < n_cells; ++c) {
     uint32_t w = 0;
     if (active[c]) w |= kActive;
     w |= (uint32_t)material[c] << kMaterialShift;
-    if (active[c]) w |= face_mask(c) << kFaceShift;
+    if (active[c]) {
+      const uint32_t fm = face_mask(c);
+      for (int f = 0; f < 6; ++f) {
+        if (!(fm & (1u << f))) continue;
+        const int cc[3] = {(int)(c % nc[0]), (int)((c / nc[0]) % nc[1]),
+                           (int)(c / ((int64_t)nc[0] * nc[1]))};
+        const int dd = f / 2, side = f % 2;
+        const bool domain = side ? (cc[dd] == nc[dd] - 1) : (cc[dd] == 0);
+        const unsigned bid = domain ? (unsigned)f : 0xffffffffu;
+        const int nb = kc.n_boundary_ids;
+        const int bt = bid < (unsigned)(nb - 1) ? kc.boundary_type[bid]
+                                                : kc.boundary_type[nb - 1];
+        FaceRef fr;
+        fr.cell = (int32_t)c;
+        fr.info = f | ((bt & 31) << 3) | ((int)material[c] << 8);
+        faces.push_back(fr);
+      }
+    }
     if (dep_cos[c] != 1.0 || dep_sin[c] != 0.0) w |= kRotated;
     if (n_states == 3 && cell_state[c * 3 + 2] > 0.0) {
       w |= kHasPowder;
@@ -465,12 +557,18 @@ void Context::rebuild_cells() {
                     hipMemcpyHostToDevice));
   ADM_FREE(d_powder);
   ADM_FREE(d_face_powder);
+  ADM_FREE(d_faces);
   d_powder = nullptr;
   d_face_powder = nullptr;
-  if (n_powder_cells > 0) {
+  d_faces = nullptr;
+  if (n_powder_cells > 0)
     ADM_HIP(hipMalloc(&d_powder, sizeof(double) * n_powder_cells * nq));
-    ADM_HIP(hipMalloc(&d_face_powder,
-                      sizeof(double) * n_powder_cells * 6 * nfq));
+  if (!faces.empty()) {
+    ADM_HIP(hipMalloc(&d_faces, sizeof(FaceRef) * faces.size()));
+    ADM_HIP(hipMemcpy(d_faces, faces.data(), sizeof(FaceRef) * faces.size(),
+                      hipMemcpyHostToDevice));
+    if (n_states == 3)
+      ADM_HIP(hipMalloc(&d_face_powder, sizeof(double) * faces.size() * nfq));
   }
 }
 
@@ -504,16 +602,20 @@ void Context::set_cell_state(const double *cs) {
   }
   if (n_powder_cells > 0) {
     std::vector<double> pw((size_t)n_powder_cells * nq);
-    std::vector<double> fpw((size_t)n_powder_cells * 6 * nfq);
     for (int64_t c = 0; c < n_cells; ++c) {
       const int64_t s = powder_slot[c];
       if (s < 0) continue;
       const double p = cell_state[c * 3 + 2];
       for (int q = 0; q < nq; ++q) pw[s * nq + q] = p;
-      for (int q = 0; q < 6 * nfq; ++q) fpw[s * 6 * nfq + q] = p;
     }
     ADM_HIP(hipMemcpy(d_powder, pw.data(), sizeof(double) * pw.size(),
                       hipMemcpyHostToDevice));
+  }
+  if (n_states == 3 && !faces.empty()) {
+    std::vector<double> fpw(faces.size() * nfq);
+    for (size_t f = 0; f < faces.size(); ++f)
+      for (int q = 0; q < nfq; ++q)
+        fpw[f * nfq + q] = cell_state[(int64_t)faces[f].cell * 3 + 2];
     ADM_HIP(hipMemcpy(d_face_powder, fpw.data(), sizeof(double) * fpw.size(),
                       hipMemcpyHostToDevice));
   }
@@ -603,14 +705,38 @@ CellArgs Context::cell_args() const {
   return a;
 }
 
+FaceArgs Context::face_args() const {
+  FaceArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.faces = d_faces;
+  a.face_powder = d_face_powder;
+  a.kc = d_kc;
+  for (int k = 0; k < 3; ++k) {
+    a.nc[k] = nc[k];
+    a.nn[k] = nn[k];
+  }
+  a.n_faces = (int64_t)faces.size();
+  return a;
+}
+
 void Context::apply(const double *src, double *dst, hipStream_t s) {
   const CellArgs a = cell_args();
-#define ADM_DISPATCH(P_, NS_)                                     \
-  if (p == P_ && n_states == NS_) {                                \
-    if (use_table)                                                 \
-      launch_cell<P_, NS_, true>(a, src, dst, n_cells, s);         \
-    else                                                           \
-      launch_cell<P_, NS_, false>(a, src, dst, n_cells, s);        \
+  const int pk = property_kind();
+  const size_t lds = pk == 2 ? sizeof(double) * kMergedLds * kc.mat.n_materials
+                             : 0;
+  const FaceArgs fa = face_args();
+#define ADM_DISPATCH(P_, NS_)                                           \
+  if (p == P_ && n_states == NS_) {                                      \
+    if (pk == 0) {                                                       \
+      launch_cell<P_, NS_, 0>(a, src, dst, n_cells, lds, s);             \
+      if (fa.n_faces) launch_face<P_, NS_, 0>(fa, src, dst, s);          \
+    } else if (pk == 1) {                                                \
+      launch_cell<P_, NS_, 1>(a, src, dst, n_cells, lds, s);             \
+      if (fa.n_faces) launch_face<P_, NS_, 1>(fa, src, dst, s);          \
+    } else {                                                             \
+      launch_cell<P_, NS_, 2>(a, src, dst, n_cells, lds, s);             \
+      if (fa.n_faces) launch_face<P_, NS_, 2>(fa, src, dst, s);          \
+    }                                                                    \
   }
   ADM_DISPATCH(1, 1)
   ADM_DISPATCH(1, 2)
@@ -853,6 +979,27 @@ int adm_set_qp_state(void *ptr, const double *liquid, const double *powder) {
   ADM_API_END
 }
 
+int adm_get_face_state(void *ptr, double *face_powder) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  if (!face_powder) return adm::set_error(ADM_ERR_INVALID, "null output");
+  const int nf = ctx.nfq;
+  std::fill(face_powder, face_powder + (size_t)ctx.n_cells * 6 * nf, 0.0);
+  if (ctx.d_face_powder && !ctx.faces.empty()) {
+    ADM_HIP(hipDeviceSynchronize());
+    std::vector<double> fpw(ctx.faces.size() * nf);
+    ADM_HIP(hipMemcpy(fpw.data(), ctx.d_face_powder,
+                      sizeof(double) * fpw.size(), hipMemcpyDeviceToHost));
+    for (size_t f = 0; f < ctx.faces.size(); ++f) {
+      const int64_t c = ctx.faces[f].cell;
+      const int fid = ctx.faces[f].info & 7;
+      for (int q = 0; q < nf; ++q)
+        face_powder[(c * 6 + fid) * nf + q] = fpw[f * nf + q];
+    }
+  }
+  ADM_API_END
+}
+
 int adm_set_deposition(void *ptr, const double *c, const double *s) {
   ADM_API_BEGIN
   ADM_CTX(ptr)
@@ -865,14 +1012,14 @@ int adm_set_deposition(void *ptr, const double *c, const double *s) {
       pw((size_t)ctx.n_cells * ctx.nq);
   ctx.get_qp_state(lq.data(), pw.data());
   std::vector<double> fpw;
-  if (ctx.n_powder_cells > 0) {
-    fpw.resize((size_t)ctx.n_powder_cells * 6 * ctx.nfq);
+  if (ctx.d_face_powder && !ctx.faces.empty()) {
+    fpw.resize(ctx.faces.size() * ctx.nfq);
     ADM_HIP(hipMemcpy(fpw.data(), ctx.d_face_powder,
                       sizeof(double) * fpw.size(), hipMemcpyDeviceToHost));
   }
   ctx.rebuild_cells();
   ctx.set_qp_state(lq.data(), pw.data());
-  if (!fpw.empty())
+  if (!fpw.empty() && ctx.d_face_powder)
     ADM_HIP(hipMemcpy(ctx.d_face_powder, fpw.data(),
                       sizeof(double) * fpw.size(), hipMemcpyHostToDevice));
   ADM_API_END

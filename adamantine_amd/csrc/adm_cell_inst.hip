@@ -1,20 +1,33 @@
-// One instantiation of the cell kernel, selected by -DADM_P, -DADM_NS and
-// -DADM_TABLE at compile time.
+// One instantiation of the cell and face kernels, selected by -DADM_P,
+// -DADM_NS and -DADM_PK at compile time.
 #include "adm_kernels.hpp"
 #include "adm_launch.hpp"
 
 namespace adm {
 
-template <int P, int NS, bool TABLE>
+template <int P, int NS, int PK>
 void launch_cell(const CellArgs &a, const double *src, double *dst,
-                 int64_t n_cells, hipStream_t s) {
+                 int64_t n_cells, size_t lds_bytes, hipStream_t s) {
   const int block = 256;
   const int64_t grid = (n_cells + block - 1) / block;
-  hipLaunchKernelGGL((cell_kernel<P, NS, TABLE>), dim3((unsigned)grid),
-                     dim3(block), 0, s, a, src, dst, n_cells);
+  hipLaunchKernelGGL((cell_kernel<P, NS, PK>), dim3((unsigned)grid),
+                     dim3(block), lds_bytes, s, a, src, dst, n_cells);
 }
 
-template void launch_cell<ADM_P, ADM_NS, (ADM_TABLE != 0)>(
-    const CellArgs &, const double *, double *, int64_t, hipStream_t);
+template <int P, int NS, int PK>
+void launch_face(const FaceArgs &a, const double *src, double *dst,
+                 hipStream_t s) {
+  const int block = 256;
+  const int64_t grid = (a.n_faces + block - 1) / block;
+  hipLaunchKernelGGL((face_kernel<P, NS, PK>), dim3((unsigned)grid),
+                     dim3(block), 0, s, a, src, dst);
+}
+
+template void launch_cell<ADM_P, ADM_NS, ADM_PK>(const CellArgs &,
+                                                 const double *, double *,
+                                                 int64_t, size_t, hipStream_t);
+template void launch_face<ADM_P, ADM_NS, ADM_PK>(const FaceArgs &,
+                                                 const double *, double *,
+                                                 hipStream_t);
 
 }  // namespace adm

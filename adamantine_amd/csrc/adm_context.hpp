@@ -44,6 +44,9 @@ struct Context {
   void set_qp_state(const double *liquid, const double *powder);
   void set_time(double t);
   CellArgs cell_args() const;
+  FaceArgs face_args() const;
+  void build_merged_tables();
+  int property_kind() const;
   void apply(const double *src, double *dst, hipStream_t s);
   void fill(double *x, double v, int64_t n, hipStream_t s);
   void compute_inverse_mass(double *out, hipStream_t s);
@@ -65,6 +68,7 @@ struct Context {
   std::vector<uint32_t> meta;
   std::vector<int32_t> powder_slot;
   int64_t n_powder_cells = 0;
+  std::vector<FaceRef> faces;
 
   std::vector<BeamData> beams;
   std::vector<int> old_segment;
@@ -79,6 +83,7 @@ struct Context {
   double *d_liquid = nullptr;
   uint8_t *d_liquid_dirty = nullptr;
   double *d_powder = nullptr, *d_face_powder = nullptr;
+  FaceRef *d_faces = nullptr;
   double *d_R = nullptr, *d_Y = nullptr, *d_acc = nullptr, *d_F = nullptr;
   double *d_inv_mass = nullptr;
   hipGraphExec_t graph_exec = nullptr;

@@ -35,6 +35,17 @@ constexpr uint32_t kRotated = 1u << 10;
 constexpr uint8_t kTableGeneral = 0;
 constexpr uint8_t kTableConstant = 1;
 
+// Merged piecewise-linear form of all state-property tables of a material:
+// the union of their breakpoints B (sorted, +inf padded) splits the
+// temperature axis into intervals j = #{B_k <= T}; inside interval j every
+// table is linear and stored as (value at the interval's left breakpoint,
+// slope), so one search per quadrature point serves every property and state.
+constexpr int kMergedMax = 32;
+struct MergedTable {
+  double B[kMergedMax];
+  double coef[ADM_N_STATE_PROPS][kMergedMax][ADM_MAX_STATES][2];
+};
+
 struct MatConst {
   int32_t n_materials, n_states, use_table, p_order, single_layout, pad_;
   double table[ADM_MAX_MATERIALS][ADM_N_STATE_PROPS][ADM_MAX_STATES]
@@ -47,6 +58,11 @@ struct MatConst {
   uint8_t kind[ADM_MAX_MATERIALS][ADM_N_STATE_PROPS][ADM_MAX_STATES];
   // thermal_conductivity_x/y/z identical for this material
   uint8_t isotropic[ADM_MAX_MATERIALS];
+  // 1 if every table of the material has the regular form (strictly
+  // increasing temperatures, then repeats of the last pair) and the merged
+  // breakpoints fit kMergedMax - 1
+  uint8_t merged_ok;
+  MergedTable merged[ADM_MAX_MATERIALS];
 };
 
 struct KConst {
@@ -79,6 +95,21 @@ struct CellArgs {
   int32_t n_beams;
   int32_t sources_on;
   adm_beam_state beams[ADM_MAX_BEAMS];
+};
+
+// One boundary face carrying a convective/radiative term.
+struct FaceRef {
+  int32_t cell;
+  int32_t info;   // face (3 bits) | boundary type << 3 | material << 8
+};
+
+struct FaceArgs {
+  const FaceRef *faces;
+  double *face_powder;  // [n_faces][n_face_q] (SolidLiquidPowder)
+  const KConst *kc;
+  int32_t nc[3];
+  int32_t nn[3];
+  int64_t n_faces;
 };
 
 // ---------------------------------------------------------------------------
@@ -151,6 +182,64 @@ __device__ __forceinline__ void temperature_powers(const MatConst &m, double T,
   for (int i = 1; i < ADM_MAX_POLY; ++i) tp[i] = i <= m.p_order ? tp[i - 1] * T : 0.0;
 }
 
+// Property evaluators: polynomial, general table, merged table.
+struct PolyEval {
+  const MatConst &m;
+  int mid;
+  double tp[ADM_MAX_POLY];
+  __device__ __forceinline__ PolyEval(const MatConst &m_, int mid_, double T)
+      : m(m_), mid(mid_) {
+    temperature_powers(m, T, tp);
+  }
+  template <int NS>
+  __device__ __forceinline__ double operator()(int prop,
+                                               const double (&r)[NS]) const {
+    return property<NS, false>(m, prop, mid, r, 0.0, tp);
+  }
+};
+
+struct TableEval {
+  const MatConst &m;
+  int mid;
+  double T;
+  __device__ __forceinline__ TableEval(const MatConst &m_, int mid_, double T_)
+      : m(m_), mid(mid_), T(T_) {}
+  template <int NS>
+  __device__ __forceinline__ double operator()(int prop,
+                                               const double (&r)[NS]) const {
+    const double tp[ADM_MAX_POLY] = {1.0, 0.0, 0.0, 0.0, 0.0};
+    return property<NS, true>(m, prop, mid, r, T, tp);
+  }
+};
+
+// B and coef may live in LDS (cell kernel) or global memory (face kernel).
+struct MergedEval {
+  const double *coef;  // &merged.coef[0][0][0][0]
+  int j;
+  double d;
+  __device__ __forceinline__ MergedEval(const double *B, const double *coef_,
+                                        double T)
+      : coef(coef_) {
+    int k = 0;
+#pragma unroll
+    for (int step = kMergedMax / 2; step >= 1; step >>= 1)
+      if (B[k + step - 1] <= T) k += step;
+    j = k;
+    d = T - B[k > 0 ? k - 1 : 0];
+  }
+  template <int NS>
+  __device__ __forceinline__ double operator()(int prop,
+                                               const double (&r)[NS]) const {
+    double value = 0.0;
+    const double *c = coef + ((prop * kMergedMax + j) * ADM_MAX_STATES) * 2;
+    // every state is added, as in the reference (a zero ratio adds +-0)
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      value += r[s] * __builtin_fma(d, c[2 * s + 1], c[2 * s]);
+    return value;
+  }
+};
+
 // update_state_ratios for one quadrature point. Returns true if the powder
 // ratio changed.
 template <int NS>
@@ -190,13 +279,12 @@ __device__ __forceinline__ bool state_ratios(const MatConst &m, int mid,
   }
 }
 
-template <int NS, bool TABLE>
+template <int NS, class Eval>
 __device__ __forceinline__ double inv_rho_cp(const MatConst &m, int mid,
-                                             const double (&r)[NS], double T,
-                                             const double (&tp)[ADM_MAX_POLY]) {
-  const double density = property<NS, TABLE>(m, ADM_DENSITY, mid, r, T, tp);
-  double specific_heat =
-      property<NS, TABLE>(m, ADM_SPECIFIC_HEAT, mid, r, T, tp);
+                                             const double (&r)[NS],
+                                             const Eval &ev) {
+  const double density = ev(ADM_DENSITY, r);
+  double specific_heat = ev(ADM_SPECIFIC_HEAT, r);
   if constexpr (NS > 1) {
     const double is_mushy = r[1] * r[0];
     if (is_mushy > 0.0) specific_heat += m.latent_factor[mid];
@@ -226,21 +314,48 @@ __device__ __forceinline__ double beam_value(const adm_beam_state &s, double x,
   xpy_squared += dy * dy;
   const double xpy_area = 5. * s.radius_squared - xpy_squared;
   if (xpy_area < 0.) return 0.0;
+  double exponent, factor = 1.0;
   if (s.type == ADM_SOURCE_GOLDAK) {
-    double exponent = z * z * s.inv_depth_squared;
+    // alpha * exp(-3 (z^2/d^2 + r^2/R^2))
+    exponent = z * z * s.inv_depth_squared;
     exponent += xpy_squared * s.inv_radius_squared;
     exponent *= -3.0;
-    return 1.0 * s.alpha * exp(exponent);
+  } else {
+    // alpha * exp(ln(0.1) r^2/R^2) * (-3 zr^2 - 2 zr + 1)
+    const double z_rel = z * s.inv_depth;
+    double dist = z_rel * z_rel;
+    dist *= -3.0;
+    dist -= z_rel;
+    dist -= z_rel;
+    dist += 1.0;
+    factor = dist;
+    exponent = log_01 * xpy_squared;
+    exponent *= s.inv_radius_squared;
   }
-  const double z_rel = z * s.inv_depth;
-  double dist = z_rel * z_rel;
-  dist *= -3.0;
-  dist -= z_rel;
-  dist -= z_rel;
-  dist += 1.0;
-  double exponent = log_01 * xpy_squared;
-  exponent *= s.inv_radius_squared;
-  return 1.0 * s.alpha * exp(exponent) * dist;
+  // factor == 1 for Goldak: (alpha * e) * 1 is alpha * e exactly
+  return 1.0 * s.alpha * exp(exponent) * factor;
+}
+
+// True if beam s can be non-zero somewhere in the box [lo, hi] (the cut-offs
+// z + depth < 0 and |xy|^2 > 5 R^2 of the reference, or the cube's box).
+__host__ __device__ __forceinline__ bool beam_may_touch(
+    const adm_beam_state &s, const double (&lo)[3], const double (&hi)[3]) {
+  if (s.type == ADM_SOURCE_CUBE) {
+    if (!s.on) return false;
+    for (int d = 0; d < 3; ++d)
+      if (hi[d] < s.cube_min[d] || lo[d] > s.cube_max[d]) return false;
+    return true;
+  }
+  if (hi[2] - s.center[2] + s.depth < 0.0) return false;
+  double d2 = 0.0;
+  for (int d = 0; d < 2; ++d) {
+    const double e = s.center[d] < lo[d]
+                         ? lo[d] - s.center[d]
+                         : (s.center[d] > hi[d] ? s.center[d] - hi[d] : 0.0);
+    d2 += e * e;
+  }
+  // generous margin: a false positive only costs the evaluation
+  return !(d2 > 5.0 * s.radius_squared * (1.0 + 1e-6));
 }
 
 }  // namespace adm

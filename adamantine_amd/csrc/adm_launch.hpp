@@ -8,8 +8,13 @@
 
 namespace adm {
 
-template <int P, int NS, bool TABLE>
+// PK: 0 polynomial properties, 1 general tables, 2 merged tables
+template <int P, int NS, int PK>
 void launch_cell(const CellArgs &a, const double *src, double *dst,
-                 int64_t n_cells, hipStream_t s);
+                 int64_t n_cells, size_t lds_bytes, hipStream_t s);
+
+template <int P, int NS, int PK>
+void launch_face(const FaceArgs &a, const double *src, double *dst,
+                 hipStream_t s);
 
 }  // namespace adm

@@ -262,6 +262,11 @@ int adm_get_cell_state(void *ctx, double *cell_state);
 int adm_get_qp_state(void *ctx, double *liquid, double *powder);
 int adm_set_qp_state(void *ctx, const double *liquid, const double *powder);
 
+/* Per-face-quadrature-point powder ratio of the boundary faces
+ * (_face_powder_ratio of source/ThermalOperator.hh:225), host out
+ * [n_cells][6][n_face_q]; faces without a boundary term read as zero. */
+int adm_get_face_state(void *ctx, double *face_powder);
+
 /* set_material_deposition_orientation (source/ThermalOperator.templates.hh:
  * 897-940): per-cell cos/sin (host, n_cells each). */
 int adm_set_deposition(void *ctx, const double *cos_, const double *sin_);
