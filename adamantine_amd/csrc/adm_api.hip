@@ -318,6 +318,13 @@ Context::Context(const adm_problem_desc &d) {
       kc.S[q][i] = lagrange(n1, xs, i, kc.xq[q]);
       kc.Dc[q][i] = lagrange_deriv(n1, kc.xq, i, kc.xq[q]);
     }
+  for (int q = 0; q < n1; ++q)
+    for (int i = 0; i < n1; ++i) {
+      long double acc = 0.0L;
+      for (int k = 0; k < n1; ++k)
+        acc += (long double)kc.Dc[q][k] * (long double)kc.S[k][i];
+      kc.DcS[q][i] = (double)acc;
+    }
   det = 1.0;
   for (int k = 0; k < 3; ++k) {
     kc.h[k] = m.extent[k] / nc[k];
@@ -701,6 +708,7 @@ CellArgs Context::cell_args() const {
   }
   a.n_beams = (int)beams.size();
   a.sources_on = sources_on ? 1 : 0;
+  a.n_cells = n_cells;
   for (size_t b = 0; b < beams.size(); ++b) a.beams[b] = beam_states[b];
   return a;
 }
@@ -719,8 +727,9 @@ FaceArgs Context::face_args() const {
   return a;
 }
 
-void Context::apply(const double *src, double *dst, hipStream_t s) {
-  const CellArgs a = cell_args();
+void Context::apply(const double *src, double *dst, int add, hipStream_t s) {
+  CellArgs a = cell_args();
+  a.add = add;
   const int pk = property_kind();
   const size_t lds = pk == 2 ? sizeof(double) * kMergedLds * kc.mat.n_materials
                              : 0;
@@ -791,7 +800,7 @@ void Context::step(int method, double t, double dt, double *T, hipStream_t s) {
   const unsigned grid = (unsigned)((n_dofs + block - 1) / block);
   if (method == ADM_FORWARD_EULER) {
     set_time(t);
-    apply(T, d_R, s);
+    apply(T, d_R, 0, s);
     hipLaunchKernelGGL(fe_update_kernel, dim3(grid), dim3(block), 0, s, T, d_R,
                        d_inv_mass, dt, n_dofs);
   } else if (method == ADM_RK_FOURTH_ORDER) {
@@ -800,7 +809,7 @@ void Context::step(int method, double t, double dt, double *T, hipStream_t s) {
     static const double anext[4] = {0.5, 0.5, 1.0, 0.0};
     for (int i = 0; i < 4; ++i) {
       set_time(t + c[i] * dt);
-      apply(i == 0 ? T : d_Y, d_R, s);
+      apply(i == 0 ? T : d_Y, d_R, 0, s);
       hipLaunchKernelGGL(rk4_stage_kernel, dim3(grid), dim3(block), 0, s, T,
                          d_R, d_inv_mass, d_acc, d_Y, dt * b[i],
                          dt * anext[i], i == 0 ? 1 : 0, i == 3 ? 1 : 0,
@@ -812,7 +821,7 @@ void Context::step(int method, double t, double dt, double *T, hipStream_t s) {
     static const double c[3] = {0.0, 0.5, 1.0};
     for (int i = 0; i < 3; ++i) {
       set_time(t + c[i] * dt);
-      apply(i == 0 ? T : d_Y, d_R, s);
+      apply(i == 0 ? T : d_Y, d_R, 0, s);
       const double *an = i + 1 < 3 ? a[i + 1] : a[0];
       hipLaunchKernelGGL(rk_stage_kernel, dim3(grid), dim3(block), 0, s, T,
                          d_R, d_inv_mass, d_F, i, 3, dt * an[0], dt * an[1],
@@ -898,8 +907,9 @@ int adm_vmult(void *ptr, const double *src, double *dst, int add,
   ADM_CTX(ptr)
   if (!src || !dst) return adm::set_error(ADM_ERR_INVALID, "null vector");
   hipStream_t s = static_cast<hipStream_t>(stream);
+  // nodes on brick faces are accumulated atomically: start from zero
   if (!add) ctx.fill(dst, 0.0, ctx.n_dofs, s);
-  ctx.apply(src, dst, s);
+  ctx.apply(src, dst, add, s);
   ADM_API_END
 }
 

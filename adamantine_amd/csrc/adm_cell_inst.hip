@@ -8,10 +8,11 @@ namespace adm {
 template <int P, int NS, int PK>
 void launch_cell(const CellArgs &a, const double *src, double *dst,
                  int64_t n_cells, size_t lds_bytes, hipStream_t s) {
-  const int block = 256;
-  const int64_t grid = (n_cells + block - 1) / block;
-  hipLaunchKernelGGL((cell_kernel<P, NS, PK>), dim3((unsigned)grid),
-                     dim3(block), lds_bytes, s, a, src, dst, n_cells);
+  constexpr int cpb = CellLanes<P>::cells_per_block;
+  const unsigned grid = (unsigned)((n_cells + cpb - 1) / cpb);
+  const size_t xbytes = sizeof(double) * cpb * (P + 1) * (P + 1) * (P + 1);
+  hipLaunchKernelGGL((cell_kernel<P, NS, PK>), dim3(grid), dim3(256),
+                     lds_bytes + xbytes, s, a, src, dst);
 }
 
 template <int P, int NS, int PK>

@@ -47,7 +47,7 @@ struct Context {
   FaceArgs face_args() const;
   void build_merged_tables();
   int property_kind() const;
-  void apply(const double *src, double *dst, hipStream_t s);
+  void apply(const double *src, double *dst, int add, hipStream_t s);
   void fill(double *x, double v, int64_t n, hipStream_t s);
   void compute_inverse_mass(double *out, hipStream_t s);
   void ensure_step_buffers(int method);

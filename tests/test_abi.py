@@ -1,0 +1,79 @@
+"""The C-ABI library loads, exports every entry point of include/adm.h, and
+reports argument errors without a GPU (no compute calls here)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(adm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from adamantine_amd import thermal
+    lib = thermal.load_library()
+    names = declared("adm.h")
+    assert "adm_vmult" in names and "adm_evolve" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_struct_layout_matches_header():
+    """ctypes mirrors of the descriptor structs have the C sizes."""
+    from adamantine_amd.problem import (BeamDesc, BeamState, MaterialDesc,
+                                        MeshDesc, ProblemDesc, Segment)
+    src = r'''
+#include <stdio.h>
+#include "adm.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(adm_material_desc),
+         sizeof(adm_mesh_desc), sizeof(adm_segment), sizeof(adm_beam_desc),
+         sizeof(adm_beam_state), sizeof(adm_problem_desc));
+  return 0;
+}
+'''
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "s.c"), "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o",
+                        os.path.join(d, "s"), os.path.join(d, "s.c")],
+                       check=True)
+        out = subprocess.run([os.path.join(d, "s")], capture_output=True,
+                             text=True, check=True).stdout.split()
+    sizes = [C.sizeof(t) for t in (MaterialDesc, MeshDesc, Segment, BeamDesc,
+                                   BeamState, ProblemDesc)]
+    assert [int(x) for x in out] == sizes
+
+
+def test_invalid_descriptor_is_an_error_not_a_crash():
+    """adm_create validates before touching the device (ASSERT_THROW
+    semantics: an error code and a message)."""
+    from adamantine_amd import thermal
+    from adamantine_amd.problem import Problem
+    lib = thermal.load_library()
+    for bad in (Problem(dim=2, n_cells=(4, 1, 5)),
+                Problem(fe_degree=3),
+                Problem(n_cells=(0, 4, 4))):
+        d, keep = bad.to_c()
+        h = C.c_void_p()
+        rc = lib.adm_create(C.byref(d), C.byref(h))
+        assert rc != 0
+        assert lib.adm_last_error()
+    assert lib.adm_vmult(None, None, None, 0, None) != 0
+    assert b"context" in lib.adm_last_error()
+
+
+def test_oracle_is_not_linked_by_the_library():
+    """The product library must not depend on the oracle."""
+    import subprocess
+    from adamantine_amd import thermal
+    out = subprocess.run(["ldd", thermal.LIB_PATH], capture_output=True,
+                         text=True).stdout
+    assert "oracle" not in out
