@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -137,8 +138,10 @@ static int scan_segment(const BeamData &b, double t, int &old_segment,
 }
 
 void beam_update(const BeamData &b, double t, int &old_segment,
-                 adm_beam_state &s) {
+                 adm_beam_state &s, double *pmod_out, double *den_out) {
   std::memset(&s, 0, sizeof(s));
+  if (pmod_out) *pmod_out = 0.0;
+  if (den_out) *den_out = 1.0;
   s.type = b.desc.type;
   if (b.desc.type == ADM_SOURCE_CUBE) {
     s.on = (t > b.desc.start_time) && (t < b.desc.end_time);
@@ -168,18 +171,21 @@ void beam_update(const BeamData &b, double t, int &old_segment,
   const double dd = b.desc.depth;
   s.depth = dd;
   s.radius_squared = r2;
+  if (pmod_out) *pmod_out = pmod;
   if (b.desc.type == ADM_SOURCE_GOLDAK) {
     const double pi_over_3_to_1p5 = std::pow(M_PI / 3.0, 1.5);
     s.alpha = 2.0 * b.desc.absorption_efficiency * b.desc.max_power * pmod /
               (r2 * dd * pi_over_3_to_1p5);
     s.inv_radius_squared = 1.0 / r2;
     s.inv_depth_squared = 1.0 / (dd * dd);
+    if (den_out) *den_out = r2 * dd * pi_over_3_to_1p5;
   } else {
     const double log_01 = std::log(0.1);
     s.alpha = -b.desc.absorption_efficiency * b.desc.max_power * pmod *
               log_01 / (M_PI * r2 * dd);
     s.inv_depth = 1.0 / dd;
     s.inv_radius_squared = 1.0 / r2;
+    if (den_out) *den_out = M_PI * r2 * dd;
   }
 }
 
@@ -479,12 +485,6 @@ Context::~Context() {
   ADM_FREE(d_powder);
   ADM_FREE(d_face_powder);
   ADM_FREE(d_faces);
-  ADM_FREE(d_R);
-  ADM_FREE(d_Y);
-  ADM_FREE(d_acc);
-  ADM_FREE(d_F);
-  ADM_FREE(d_inv_mass);
-  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
 }
 
 void Context::upload_constants() {
@@ -730,21 +730,28 @@ FaceArgs Context::face_args() const {
 void Context::apply(const double *src, double *dst, int add, hipStream_t s) {
   CellArgs a = cell_args();
   a.add = add;
+  launch(a, face_args(), src, dst, 1, s);
+}
+
+// Cell and face kernels of the context's configuration; with n_members > 1
+// the arguments carry the member table and the grid's y index is the member.
+void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
+                     double *dst, unsigned n_members, hipStream_t s) const {
   const int pk = property_kind();
   const size_t lds = pk == 2 ? sizeof(double) * kMergedLds * kc.mat.n_materials
                              : 0;
-  const FaceArgs fa = face_args();
+  const unsigned M = n_members;
 #define ADM_DISPATCH(P_, NS_)                                           \
   if (p == P_ && n_states == NS_) {                                      \
     if (pk == 0) {                                                       \
-      launch_cell<P_, NS_, 0>(a, src, dst, n_cells, lds, s);             \
-      if (fa.n_faces) launch_face<P_, NS_, 0>(fa, src, dst, s);          \
+      launch_cell<P_, NS_, 0>(a, src, dst, n_cells, lds, M, s);          \
+      if (fa.n_faces) launch_face<P_, NS_, 0>(fa, src, dst, M, s);       \
     } else if (pk == 1) {                                                \
-      launch_cell<P_, NS_, 1>(a, src, dst, n_cells, lds, s);             \
-      if (fa.n_faces) launch_face<P_, NS_, 1>(fa, src, dst, s);          \
+      launch_cell<P_, NS_, 1>(a, src, dst, n_cells, lds, M, s);          \
+      if (fa.n_faces) launch_face<P_, NS_, 1>(fa, src, dst, M, s);       \
     } else {                                                             \
-      launch_cell<P_, NS_, 2>(a, src, dst, n_cells, lds, s);             \
-      if (fa.n_faces) launch_face<P_, NS_, 2>(fa, src, dst, s);          \
+      launch_cell<P_, NS_, 2>(a, src, dst, n_cells, lds, M, s);          \
+      if (fa.n_faces) launch_face<P_, NS_, 2>(fa, src, dst, M, s);       \
     }                                                                    \
   }
   ADM_DISPATCH(1, 1)
@@ -776,60 +783,208 @@ void Context::compute_inverse_mass(double *out, hipStream_t s) {
   ADM_HIP(hipGetLastError());
 }
 
-void Context::ensure_step_buffers(int method) {
-  if (!d_R) {
-    ADM_HIP(hipMalloc(&d_R, sizeof(double) * n_dofs));
-    ADM_HIP(hipMemset(d_R, 0, sizeof(double) * n_dofs));
+void StepBuffers::ensure(int method, int64_t n_) {
+  n = n_;
+  if (!R) {
+    ADM_HIP(hipMalloc(&R, sizeof(double) * n));
+    ADM_HIP(hipMemset(R, 0, sizeof(double) * n));
   }
-  if (!d_inv_mass) {
-    ADM_HIP(hipMalloc(&d_inv_mass, sizeof(double) * n_dofs));
-    compute_inverse_mass(d_inv_mass, nullptr);
-    ADM_HIP(hipDeviceSynchronize());
+  if (method != ADM_FORWARD_EULER && !Y) {
+    ADM_HIP(hipMalloc(&Y, sizeof(double) * n));
+    ADM_HIP(hipMalloc(&acc, sizeof(double) * n));
   }
-  if (method != ADM_FORWARD_EULER && !d_Y) {
-    ADM_HIP(hipMalloc(&d_Y, sizeof(double) * n_dofs));
-    ADM_HIP(hipMalloc(&d_acc, sizeof(double) * n_dofs));
-  }
-  if (method == ADM_RK_THIRD_ORDER && !d_F)
-    ADM_HIP(hipMalloc(&d_F, sizeof(double) * n_dofs * 3));
+  if (method == ADM_RK_THIRD_ORDER && !F)
+    ADM_HIP(hipMalloc(&F, sizeof(double) * n * 3));
 }
 
-// One evolve_one_time_step on the device, enqueued on s.
-void Context::step(int method, double t, double dt, double *T, hipStream_t s) {
+StepBuffers::~StepBuffers() {
+  for (double *q : {R, Y, acc, F, inv_mass})
+    if (q) (void)hipFree(q);
+}
+
+// One evolve_one_time_step (source/ThermalPhysics.templates.hh:854-888) on
+// the device, enqueued on s. rhs(t_stage, x, r) sets the time and adds A(x)
+// into the cleared residual r; the vector kernels then apply the inverse
+// mass and deal.II's stage combinations (forward Euler as sadd; classic RK4
+// and RK3 as TimeStepping::ExplicitRungeKutta orders them).
+template <class Rhs>
+static void explicit_step(int method, double t, double dt, double *T,
+                          StepBuffers &b, Rhs &&rhs, hipStream_t s) {
   const int block = 256;
-  const unsigned grid = (unsigned)((n_dofs + block - 1) / block);
+  const int64_t n = b.n;
+  const unsigned grid = (unsigned)((n + block - 1) / block);
   if (method == ADM_FORWARD_EULER) {
-    set_time(t);
-    apply(T, d_R, 0, s);
-    hipLaunchKernelGGL(fe_update_kernel, dim3(grid), dim3(block), 0, s, T, d_R,
-                       d_inv_mass, dt, n_dofs);
+    rhs(t, T, b.R);
+    hipLaunchKernelGGL(fe_update_kernel, dim3(grid), dim3(block), 0, s, T, b.R,
+                       b.inv_mass, dt, n);
   } else if (method == ADM_RK_FOURTH_ORDER) {
-    static const double b[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+    static const double bw[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
     static const double c[4] = {0.0, 0.5, 0.5, 1.0};
     static const double anext[4] = {0.5, 0.5, 1.0, 0.0};
     for (int i = 0; i < 4; ++i) {
-      set_time(t + c[i] * dt);
-      apply(i == 0 ? T : d_Y, d_R, 0, s);
+      rhs(t + c[i] * dt, i == 0 ? T : b.Y, b.R);
       hipLaunchKernelGGL(rk4_stage_kernel, dim3(grid), dim3(block), 0, s, T,
-                         d_R, d_inv_mass, d_acc, d_Y, dt * b[i],
-                         dt * anext[i], i == 0 ? 1 : 0, i == 3 ? 1 : 0,
-                         n_dofs);
+                         b.R, b.inv_mass, b.acc, b.Y, dt * bw[i],
+                         dt * anext[i], i == 0 ? 1 : 0, i == 3 ? 1 : 0, n);
     }
   } else {
     static const double a[3][3] = {{0, 0, 0}, {0.5, 0, 0}, {-1.0, 2.0, 0}};
-    static const double b[3] = {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0};
+    static const double bw[3] = {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0};
     static const double c[3] = {0.0, 0.5, 1.0};
     for (int i = 0; i < 3; ++i) {
-      set_time(t + c[i] * dt);
-      apply(i == 0 ? T : d_Y, d_R, 0, s);
+      rhs(t + c[i] * dt, i == 0 ? T : b.Y, b.R);
       const double *an = i + 1 < 3 ? a[i + 1] : a[0];
       hipLaunchKernelGGL(rk_stage_kernel, dim3(grid), dim3(block), 0, s, T,
-                         d_R, d_inv_mass, d_F, i, 3, dt * an[0], dt * an[1],
-                         dt * an[2], d_Y, dt * b[0], dt * b[1], dt * b[2],
-                         n_dofs);
+                         b.R, b.inv_mass, b.F, i, 3, dt * an[0], dt * an[1],
+                         dt * an[2], b.Y, dt * bw[0], dt * bw[1], dt * bw[2],
+                         n);
     }
   }
   ADM_HIP(hipGetLastError());
+}
+
+void Context::ensure_step_buffers(int method) {
+  sb.ensure(method, n_dofs);
+  if (!sb.inv_mass) {
+    ADM_HIP(hipMalloc(&sb.inv_mass, sizeof(double) * n_dofs));
+    compute_inverse_mass(sb.inv_mass, nullptr);
+    ADM_HIP(hipDeviceSynchronize());
+  }
+}
+
+void Context::step(int method, double t, double dt, double *T, hipStream_t s) {
+  explicit_step(
+      method, t, dt, T, sb,
+      [&](double ts, const double *x, double *r) {
+        set_time(ts);
+        apply(x, r, 0, s);
+      },
+      s);
+}
+
+// ---------------------------------------------------------------------------
+// Ensemble
+// ---------------------------------------------------------------------------
+
+Ensemble::Ensemble(const adm_problem_desc &base, int n,
+                   const adm_member_desc *md) {
+  if (n < 1) throw AdmError(ADM_ERR_INVALID, "n_members < 1");
+  if (!md) throw AdmError(ADM_ERR_INVALID, "null member descriptors");
+  n_members = n;
+  for (int j = 0; j < n; ++j) {
+    adm_problem_desc d = base;
+    const adm_member_desc &m = md[j];
+    for (int b = 0; b < base.n_beams && b < ADM_MAX_BEAMS; ++b) {
+      d.beams[b].absorption_efficiency = m.absorption_efficiency[b];
+      d.beams[b].max_power = m.max_power[b];
+    }
+    if (!(m.conductivity_scale > 0.0))
+      throw AdmError(ADM_ERR_INVALID, "conductivity_scale must be > 0");
+    if (m.conductivity_scale != 1.0) {
+      adm_material_desc &mat = d.material;
+      for (int mm = 0; mm < ADM_MAX_MATERIALS; ++mm)
+        for (int pr = ADM_THERMAL_CONDUCTIVITY_X;
+             pr <= ADM_THERMAL_CONDUCTIVITY_Z; ++pr)
+          for (int st = 0; st < ADM_MAX_STATES; ++st) {
+            for (int i = 0; i < ADM_TABLE_SIZE; ++i)
+              mat.table[mm][pr][st][i][1] *= m.conductivity_scale;
+            for (int i = 0; i < ADM_MAX_POLY; ++i)
+              mat.poly[mm][pr][st][i] *= m.conductivity_scale;
+          }
+    }
+    if (m.cell_active) d.cell_active = m.cell_active;
+    if (m.cell_state) d.cell_state = m.cell_state;
+    members.emplace_back(new Context(d));
+  }
+  const Context &c0 = *members[0];
+  n_dofs = c0.n_dofs;
+  std::vector<MemberArgs> h(n);
+  for (int j = 0; j < n; ++j) {
+    const Context &c = *members[j];
+    if (c.property_kind() != c0.property_kind())
+      throw AdmError(ADM_ERR_INVALID,
+                     "members differ in the form of their material tables");
+    const CellArgs a = c.cell_args();
+    MemberArgs &m = h[j];
+    std::memset(&m, 0, sizeof(m));
+    m.meta = a.meta;
+    m.powder_slot = a.powder_slot;
+    m.powder = a.powder;
+    m.face_powder = a.face_powder;
+    m.liquid = a.liquid;
+    m.liquid_dirty = a.liquid_dirty;
+    m.dep_cos = a.dep_cos;
+    m.dep_sin = a.dep_sin;
+    m.kc = a.kc;
+    m.faces = c.d_faces;
+    m.n_faces = (int64_t)c.faces.size();
+    max_faces = std::max(max_faces, m.n_faces);
+    for (size_t b = 0; b < c.beams.size(); ++b)
+      m.beam_gain[b] =
+          c.beams[b].desc.absorption_efficiency * c.beams[b].desc.max_power;
+  }
+  ADM_HIP(hipMalloc(&d_members, sizeof(MemberArgs) * n));
+  ADM_HIP(hipMemcpy(d_members, h.data(), sizeof(MemberArgs) * n,
+                    hipMemcpyHostToDevice));
+}
+
+Ensemble::~Ensemble() {
+  if (d_members) (void)hipFree(d_members);
+}
+
+void Ensemble::set_time(double t) {
+  // The scan paths, radii and depths are shared, so the beam positions and
+  // power modifiers are member 0's; the kernels rebuild each member's alpha.
+  Context &c0 = *members[0];
+  c0.sources_on = false;
+  for (size_t b = 0; b < c0.beams.size(); ++b) {
+    beam_update(c0.beams[b], t, c0.old_segment[b], c0.beam_states[b],
+                &pmod[b], &den[b]);
+    if (c0.beam_states[b].on) c0.sources_on = true;
+  }
+}
+
+void Ensemble::apply(const double *src, double *dst, int add, hipStream_t s) {
+  Context &c0 = *members[0];
+  CellArgs a = c0.cell_args();
+  a.add = add;
+  a.members = d_members;
+  a.member_stride = n_dofs;
+  for (int b = 0; b < ADM_MAX_BEAMS; ++b) {
+    a.beam_pmod[b] = pmod[b];
+    a.beam_den[b] = den[b];
+  }
+  FaceArgs fa = c0.face_args();
+  fa.members = d_members;
+  fa.member_stride = n_dofs;
+  fa.n_faces = max_faces;
+  c0.launch(a, fa, src, dst, (unsigned)n_members, s);
+  ADM_HIP(hipGetLastError());
+}
+
+void Ensemble::compute_inverse_mass(double *out, hipStream_t s) {
+  for (int j = 0; j < n_members; ++j)
+    members[j]->compute_inverse_mass(out + (int64_t)j * n_dofs, s);
+}
+
+void Ensemble::ensure_step_buffers(int method) {
+  sb.ensure(method, n_dofs * n_members);
+  if (!sb.inv_mass) {
+    ADM_HIP(hipMalloc(&sb.inv_mass, sizeof(double) * n_dofs * n_members));
+    compute_inverse_mass(sb.inv_mass, nullptr);
+    ADM_HIP(hipDeviceSynchronize());
+  }
+}
+
+void Ensemble::step(int method, double t, double dt, double *T,
+                    hipStream_t s) {
+  explicit_step(
+      method, t, dt, T, sb,
+      [&](double ts, const double *x, double *r) {
+        set_time(ts);
+        apply(x, r, 0, s);
+      },
+      s);
 }
 
 }  // namespace adm
@@ -1055,6 +1210,107 @@ int adm_beam_state_at(void *ptr, int beam, double t, adm_beam_state *out) {
   if (beam < 0 || beam >= (int)ctx.beams.size() || !out)
     return adm::set_error(ADM_ERR_INVALID, "invalid beam query");
   adm::beam_update(ctx.beams[beam], t, ctx.old_segment[beam], *out);
+  ADM_API_END
+}
+
+int adm_normal_random_vector(uint32_t length, uint32_t n_rejected_draws,
+                             double mean, double stddev, double *out) {
+  ADM_API_BEGIN
+  if (!out && length) return adm::set_error(ADM_ERR_INVALID, "null output");
+  if (!(stddev >= 0.0))
+    return adm::set_error(ADM_ERR_INVALID, "stddev must be >= 0");
+  // the reference's own generator and distribution (default seed), so the
+  // draws are those of the standard library both are built against
+  std::mt19937 prng;
+  std::normal_distribution<> normal(mean, stddev);
+  for (uint32_t i = 0; i < n_rejected_draws; ++i) (void)normal(prng);
+  for (uint32_t i = 0; i < length; ++i) {
+    do {
+      out[i] = normal(prng);
+    } while (out[i] < 0.);
+  }
+  ADM_API_END
+}
+
+#define ADM_ENS(ptr)                                                \
+  if (!(ptr)) return adm::set_error(ADM_ERR_STATE, "null ensemble"); \
+  adm::Ensemble &ens = *static_cast<adm::Ensemble *>(ptr);
+
+int adm_ensemble_create(const adm_problem_desc *base, int32_t n_members,
+                        const adm_member_desc *members, void **out) {
+  ADM_API_BEGIN
+  if (!base || !members || !out)
+    return adm::set_error(ADM_ERR_INVALID, "null argument");
+  *out = new adm::Ensemble(*base, n_members, members);
+  ADM_API_END
+}
+
+int adm_ensemble_destroy(void *ptr) {
+  ADM_API_BEGIN
+  delete static_cast<adm::Ensemble *>(ptr);
+  ADM_API_END
+}
+
+int adm_ensemble_n_dofs(const void *ptr, int64_t *n) {
+  ADM_API_BEGIN
+  if (!ptr || !n) return adm::set_error(ADM_ERR_INVALID, "null argument");
+  *n = static_cast<const adm::Ensemble *>(ptr)->n_dofs;
+  ADM_API_END
+}
+
+int adm_ensemble_compute_inverse_mass(void *ptr, double *inv_mass,
+                                      void *stream) {
+  ADM_API_BEGIN
+  ADM_ENS(ptr)
+  if (!inv_mass) return adm::set_error(ADM_ERR_INVALID, "null output");
+  ens.compute_inverse_mass(inv_mass, static_cast<hipStream_t>(stream));
+  ADM_API_END
+}
+
+int adm_ensemble_set_time(void *ptr, double t) {
+  ADM_API_BEGIN
+  ADM_ENS(ptr)
+  ens.set_time(t);
+  ADM_API_END
+}
+
+int adm_ensemble_vmult(void *ptr, const double *src, double *dst, int add,
+                       void *stream) {
+  ADM_API_BEGIN
+  ADM_ENS(ptr)
+  if (!src || !dst) return adm::set_error(ADM_ERR_INVALID, "null vector");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!add) ens.members[0]->fill(dst, 0.0, ens.n_dofs * ens.n_members, s);
+  ens.apply(src, dst, add, s);
+  ADM_API_END
+}
+
+int adm_ensemble_evolve(void *ptr, int method, double t, double dt,
+                        int n_steps, double *T, double *t_end, void *stream) {
+  ADM_API_BEGIN
+  ADM_ENS(ptr)
+  if (!T) return adm::set_error(ADM_ERR_INVALID, "null solution vector");
+  if (method < ADM_FORWARD_EULER || method > ADM_RK_FOURTH_ORDER)
+    return adm::set_error(ADM_ERR_INVALID, "unknown time stepping method");
+  if (n_steps < 0) return adm::set_error(ADM_ERR_INVALID, "n_steps < 0");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  ens.ensure_step_buffers(method);
+  for (int k = 0; k < n_steps; ++k) {
+    ens.step(method, t, dt, T, s);
+    t = t + dt;
+  }
+  if (t_end) *t_end = t;
+  ADM_API_END
+}
+
+int adm_ensemble_get_qp_state(void *ptr, int32_t member, double *liquid,
+                              double *powder) {
+  ADM_API_BEGIN
+  ADM_ENS(ptr)
+  if (member < 0 || member >= ens.n_members)
+    return adm::set_error(ADM_ERR_INVALID, "member index out of range");
+  ADM_HIP(hipDeviceSynchronize());
+  ens.members[member]->get_qp_state(liquid, powder);
   ADM_API_END
 }
 

@@ -7,28 +7,30 @@ namespace adm {
 
 template <int P, int NS, int PK>
 void launch_cell(const CellArgs &a, const double *src, double *dst,
-                 int64_t n_cells, size_t lds_bytes, hipStream_t s) {
+                 int64_t n_cells, size_t lds_bytes, unsigned n_members,
+                 hipStream_t s) {
   constexpr int cpb = CellLanes<P>::cells_per_block;
   const unsigned grid = (unsigned)((n_cells + cpb - 1) / cpb);
   const size_t xbytes = sizeof(double) * cpb * (P + 1) * (P + 1) * (P + 1);
-  hipLaunchKernelGGL((cell_kernel<P, NS, PK>), dim3(grid), dim3(256),
+  hipLaunchKernelGGL((cell_kernel<P, NS, PK>), dim3(grid, n_members), dim3(256),
                      lds_bytes + xbytes, s, a, src, dst);
 }
 
 template <int P, int NS, int PK>
 void launch_face(const FaceArgs &a, const double *src, double *dst,
-                 hipStream_t s) {
+                 unsigned n_members, hipStream_t s) {
   const int block = 256;
   const int64_t grid = (a.n_faces + block - 1) / block;
-  hipLaunchKernelGGL((face_kernel<P, NS, PK>), dim3((unsigned)grid),
+  hipLaunchKernelGGL((face_kernel<P, NS, PK>), dim3((unsigned)grid, n_members),
                      dim3(block), 0, s, a, src, dst);
 }
 
 template void launch_cell<ADM_P, ADM_NS, ADM_PK>(const CellArgs &,
                                                  const double *, double *,
-                                                 int64_t, size_t, hipStream_t);
+                                                 int64_t, size_t, unsigned,
+                                                 hipStream_t);
 template void launch_face<ADM_P, ADM_NS, ADM_PK>(const FaceArgs &,
                                                  const double *, double *,
-                                                 hipStream_t);
+                                                 unsigned, hipStream_t);
 
 }  // namespace adm

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -25,10 +26,27 @@ struct BeamData {
   std::vector<adm_segment> segments;
 };
 
+// HeatSource::update_time on the host. pmod_out / den_out (optional) receive
+// the power modifier and the parenthesised denominator of alpha.
 void beam_update(const BeamData &b, double t, int &old_segment,
-                 adm_beam_state &s);
+                 adm_beam_state &s, double *pmod_out = nullptr,
+                 double *den_out = nullptr);
 void gauss_points(int n, double *x, double *w);
 void gauss_lobatto_points(int n, double *x, double *w);
+
+// Device vectors of the explicit time steppers: the residual R (kept zero
+// between stages), stage input Y, RK4 accumulator, RK3 stage values F and the
+// inverse lumped mass, all of length n.
+struct StepBuffers {
+  int64_t n = 0;
+  double *R = nullptr, *Y = nullptr, *acc = nullptr, *F = nullptr;
+  double *inv_mass = nullptr;
+  StepBuffers() = default;
+  StepBuffers(const StepBuffers &) = delete;
+  StepBuffers &operator=(const StepBuffers &) = delete;
+  ~StepBuffers();
+  void ensure(int method, int64_t n_);
+};
 
 struct Context {
   explicit Context(const adm_problem_desc &d);
@@ -48,6 +66,8 @@ struct Context {
   void build_merged_tables();
   int property_kind() const;
   void apply(const double *src, double *dst, int add, hipStream_t s);
+  void launch(const CellArgs &a, const FaceArgs &fa, const double *src,
+              double *dst, unsigned n_members, hipStream_t s) const;
   void fill(double *x, double v, int64_t n, hipStream_t s);
   void compute_inverse_mass(double *out, hipStream_t s);
   void ensure_step_buffers(int method);
@@ -84,9 +104,31 @@ struct Context {
   uint8_t *d_liquid_dirty = nullptr;
   double *d_powder = nullptr, *d_face_powder = nullptr;
   FaceRef *d_faces = nullptr;
-  double *d_R = nullptr, *d_Y = nullptr, *d_acc = nullptr, *d_F = nullptr;
-  double *d_inv_mass = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
+  StepBuffers sb;
+};
+
+// Members of an ensemble: one Context each (own state, tables, face lists),
+// stepped together by single launches over (cells, members).
+struct Ensemble {
+  Ensemble(const adm_problem_desc &base, int n, const adm_member_desc *md);
+  ~Ensemble();
+  Ensemble(const Ensemble &) = delete;
+  Ensemble &operator=(const Ensemble &) = delete;
+
+  void set_time(double t);
+  void apply(const double *src, double *dst, int add, hipStream_t s);
+  void compute_inverse_mass(double *out, hipStream_t s);
+  void ensure_step_buffers(int method);
+  void step(int method, double t, double dt, double *T, hipStream_t s);
+
+  int n_members = 0;
+  int64_t n_dofs = 0;
+  int64_t max_faces = 0;
+  std::vector<std::unique_ptr<Context>> members;
+  MemberArgs *d_members = nullptr;
+  double pmod[ADM_MAX_BEAMS] = {0, 0, 0, 0};
+  double den[ADM_MAX_BEAMS] = {1, 1, 1, 1};
+  StepBuffers sb;
 };
 
 }  // namespace adm

@@ -80,6 +80,30 @@ struct KConst {
   MatConst mat;
 };
 
+// One boundary face carrying a convective/radiative term.
+struct FaceRef {
+  int32_t cell;
+  int32_t info;   // face (3 bits) | boundary type << 3 | material << 8
+};
+
+// Per-member data of an ensemble (device array, indexed by blockIdx.y). The
+// pointers replace the CellArgs / FaceArgs ones of the same name; beam_gain
+// is absorption_efficiency * max_power of each beam.
+struct MemberArgs {
+  const uint32_t *meta;
+  const int32_t *powder_slot;
+  double *powder;
+  double *face_powder;
+  double *liquid;
+  uint8_t *liquid_dirty;
+  const double *dep_cos;
+  const double *dep_sin;
+  const KConst *kc;
+  const FaceRef *faces;
+  int64_t n_faces;
+  double beam_gain[ADM_MAX_BEAMS];
+};
+
 // Arguments of the cell kernel (by value).
 struct CellArgs {
   const uint32_t *meta;       // [n_cells]
@@ -98,12 +122,14 @@ struct CellArgs {
   int32_t add;          // unused by the atomic-accumulation kernel
   int64_t n_cells;
   adm_beam_state beams[ADM_MAX_BEAMS];
-};
-
-// One boundary face carrying a convective/radiative term.
-struct FaceRef {
-  int32_t cell;
-  int32_t info;   // face (3 bits) | boundary type << 3 | material << 8
+  // ensemble launch: members[blockIdx.y] replaces the per-cell pointers and
+  // kc, vectors are offset by blockIdx.y * member_stride, and each beam's
+  // alpha is rebuilt from the member's gain with the shared power modifier
+  // and denominator of GoldakHeatSource / ElectronBeamHeatSource::update_time
+  const MemberArgs *members;  // NULL: one operator
+  int64_t member_stride;
+  double beam_pmod[ADM_MAX_BEAMS];
+  double beam_den[ADM_MAX_BEAMS];
 };
 
 struct FaceArgs {
@@ -112,7 +138,9 @@ struct FaceArgs {
   const KConst *kc;
   int32_t nc[3];
   int32_t nn[3];
-  int64_t n_faces;
+  int64_t n_faces;      // ensemble: the largest member face count
+  const MemberArgs *members;  // NULL: one operator
+  int64_t member_stride;
 };
 
 // ---------------------------------------------------------------------------
@@ -329,6 +357,21 @@ __device__ __forceinline__ double inv_rho_cp(const MatConst &m, int mid,
 // ---------------------------------------------------------------------------
 // heat sources (one lane per point)
 // ---------------------------------------------------------------------------
+
+// alpha of an ensemble member's beam from g = absorption_efficiency *
+// max_power, with the operation order of GoldakHeatSource::update_time
+// (source/GoldakHeatSource.cc:24-44: 2 eta P pmod / (r^2 d (pi/3)^1.5)) and
+// ElectronBeamHeatSource::update_time (source/ElectronBeamHeatSource.cc:
+// 24-45: -eta P pmod ln(0.1) / (pi r^2 d)); den is the parenthesised
+// denominator. Scaling by 2 and negation are exact, so the result equals the
+// single-operator host value bit for bit.
+__device__ __forceinline__ double member_alpha(const adm_beam_state &s,
+                                               double g, double pmod,
+                                               double den, double log_01) {
+  if (s.type == ADM_SOURCE_GOLDAK) return (2.0 * g) * pmod / den;
+  if (s.type == ADM_SOURCE_ELECTRON_BEAM) return (-g) * pmod * log_01 / den;
+  return s.alpha;
+}
 
 __device__ __forceinline__ double beam_value(const adm_beam_state &s, double x,
                                              double y, double z_,

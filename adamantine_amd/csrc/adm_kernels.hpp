@@ -92,10 +92,16 @@ __global__ void __launch_bounds__(256)
                 double *__restrict__ dst) {
   constexpr int N = P + 1, NF = N * N;
   const int64_t fi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (fi >= a.n_faces) return;
-  const KConst &kc = *a.kc;
+  const MemberArgs *mb = a.members ? a.members + blockIdx.y : nullptr;
+  if (fi >= (mb ? mb->n_faces : a.n_faces)) return;
+  const KConst &kc = *(mb ? mb->kc : a.kc);
   const MatConst &m = kc.mat;
-  const FaceRef fr = a.faces[fi];
+  const FaceRef fr = (mb ? mb->faces : a.faces)[fi];
+  double *face_powder = mb ? mb->face_powder : a.face_powder;
+  if (mb) {
+    src += (int64_t)blockIdx.y * a.member_stride;
+    dst += (int64_t)blockIdx.y * a.member_stride;
+  }
   const int64_t c = fr.cell;
   const int f = fr.info & 7, bt = (fr.info >> 3) & 31, mid = (fr.info >> 8) & 3;
   const int d = f / 2, side = f % 2;
@@ -115,7 +121,7 @@ __global__ void __launch_bounds__(256)
   }
   sweep<N, 0, false, false, 2>(kc.S, fu, ft);
   sweep<N, 1, false, false, 2>(kc.S, ft, fv);
-  double *fp = NS == 3 ? a.face_powder + fi * NF : nullptr;
+  double *fp = NS == 3 ? face_powder + fi * NF : nullptr;
 #pragma unroll
   for (int q = 0; q < NF; ++q) {
     const double T = fv[q];
@@ -190,8 +196,21 @@ __global__ void __launch_bounds__(256)
                 double *__restrict__ dst) {
   constexpr int N = P + 1, NP = N * N;
   constexpr int CPW = CellLanes<P>::cells_per_wave;
-  const KConst &kc = *a.kc;
+  // ensemble member of this block (uniform), else the single operator
+  const MemberArgs *mb = a.members ? a.members + blockIdx.y : nullptr;
+  const KConst &kc = *(mb ? mb->kc : a.kc);
   const MatConst &m = kc.mat;
+  const uint32_t *meta_p = mb ? mb->meta : a.meta;
+  const int32_t *slot_p = mb ? mb->powder_slot : a.powder_slot;
+  double *powder_p = mb ? mb->powder : a.powder;
+  double *liquid_p = mb ? mb->liquid : a.liquid;
+  uint8_t *dirty_p = mb ? mb->liquid_dirty : a.liquid_dirty;
+  const double *cos_p = mb ? mb->dep_cos : a.dep_cos;
+  const double *sin_p = mb ? mb->dep_sin : a.dep_sin;
+  if (mb) {
+    src += (int64_t)blockIdx.y * a.member_stride;
+    dst += (int64_t)blockIdx.y * a.member_stride;
+  }
   extern __shared__ double lds[];
   __shared__ adm_beam_state sbeams[ADM_MAX_BEAMS];
   // one lane copies the beam states with uniform (scalar) loads
@@ -199,6 +218,10 @@ __global__ void __launch_bounds__(256)
     static_for<0, ADM_MAX_BEAMS>([&](auto bc) {
       constexpr int bb = decltype(bc)::value;
       sbeams[bb] = a.beams[bb];
+      if (mb)
+        sbeams[bb].alpha = member_alpha(a.beams[bb], mb->beam_gain[bb],
+                                        a.beam_pmod[bb], a.beam_den[bb],
+                                        kc.log_01);
     });
   }
   if constexpr (PK == 2) {
@@ -213,7 +236,7 @@ __global__ void __launch_bounds__(256)
   const int j = lane / N, kz = lane % N;
   const int64_t c = ((int64_t)blockIdx.x * 4 + wave) * CPW + j;
   const bool inside = j < CPW && c < a.n_cells;
-  const uint32_t meta = inside ? a.meta[c] : 0u;
+  const uint32_t meta = inside ? meta_p[c] : 0u;
   if (!(meta & kActive)) return;  // whole cells (all N lanes) leave together
   // exchange buffer of this cell: [N planes][NP]
   double *xb = lds + (PK == 2 ? m.n_materials * kMergedLds : 0) +
@@ -263,21 +286,21 @@ __global__ void __launch_bounds__(256)
     }
   }
 
-  const int64_t pslot = (meta & kHasPowder) ? (int64_t)a.powder_slot[c] : -1;
-  double *pw = pslot >= 0 ? a.powder + (pslot * N + kz) * NP : nullptr;
+  const int64_t pslot = (meta & kHasPowder) ? (int64_t)slot_p[c] : -1;
+  double *pw = pslot >= 0 ? powder_p + (pslot * N + kz) * NP : nullptr;
   double *lq = nullptr;
   bool dirty = false;
   if constexpr (NS > 1) {
-    if (a.liquid) {
-      dirty = a.liquid_dirty[c] != 0;
-      lq = a.liquid + (c * N + kz) * NP;
+    if (liquid_p) {
+      dirty = dirty_p[c] != 0;
+      lq = liquid_p + (c * N + kz) * NP;
     }
   }
   const bool rotated = (meta & kRotated) != 0;
   double cs = 1.0, sn = 0.0;
   if (rotated) {
-    cs = a.dep_cos[c];
-    sn = a.dep_sin[c];
+    cs = cos_p[c];
+    sn = sin_p[c];
   }
   const bool iso = m.isotropic[mid] != 0;
   const double *mB = lds + mid * kMergedLds;
@@ -451,7 +474,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int k = 0; k < N; ++k) any_liquid |= xb[k * NP] != 0.0;
     if (lq && kz == 0 && any_liquid != dirty)
-      a.liquid_dirty[c] = any_liquid ? 1 : 0;
+      dirty_p[c] = any_liquid ? 1 : 0;
   }
   // node plane kz: S^T along z of the x/y-integrated planes, plus the
   // transposed z-derivative of the z-fluxes (Dc*S)^T

@@ -8,13 +8,15 @@
 
 namespace adm {
 
-// PK: 0 polynomial properties, 1 general tables, 2 merged tables
+// PK: 0 polynomial properties, 1 general tables, 2 merged tables.
+// n_members > 1 only with CellArgs/FaceArgs::members set (grid y = member).
 template <int P, int NS, int PK>
 void launch_cell(const CellArgs &a, const double *src, double *dst,
-                 int64_t n_cells, size_t lds_bytes, hipStream_t s);
+                 int64_t n_cells, size_t lds_bytes, unsigned n_members,
+                 hipStream_t s);
 
 template <int P, int NS, int PK>
 void launch_face(const FaceArgs &a, const double *src, double *dst,
-                 hipStream_t s);
+                 unsigned n_members, hipStream_t s);
 
 }  // namespace adm

@@ -1,0 +1,250 @@
+"""Ensemble of thermal simulations stepped together on one GPU.
+
+The reference runs an ensemble as one ``ThermalPhysics`` per member, built
+from a per-member copy of the input database whose ``ensemble.<key>_stddev``
+keys were drawn from a normal distribution
+(source/ensemble_management.cc:166-265), and steps the members one after the
+other (application/adamantine.hh:2137-2142). Here the members share one
+mesh and FE space and are stepped by single kernel launches over
+(cells, members): member j's nodes are entries [j*n_dofs, (j+1)*n_dofs) of
+every ensemble vector.
+
+What a member may change (``Member``): each beam's absorption efficiency
+and max power, a factor on every thermal conductivity, the active cells and
+the initial cell state.
+"""
+from __future__ import annotations
+
+import copy
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from .problem import (MAX_BEAMS, STATE_PROPERTY_NAMES, Problem, ProblemDesc)
+from .thermal import (_dev_ptr, _stream_ptr, _vp, check, load_library)
+
+_dp = C.POINTER(C.c_double)
+
+
+class MemberDesc(C.Structure):
+    _fields_ = [
+        ("absorption_efficiency", C.c_double * MAX_BEAMS),
+        ("max_power", C.c_double * MAX_BEAMS),
+        ("conductivity_scale", C.c_double),
+        ("cell_active", C.POINTER(C.c_uint8)),
+        ("cell_state", C.POINTER(C.c_double)),
+    ]
+
+
+@dataclass
+class Member:
+    """One member's departures from the base problem (None: unchanged)."""
+    absorption_efficiency: Optional[Sequence[float]] = None
+    max_power: Optional[Sequence[float]] = None
+    conductivity_scale: float = 1.0
+    cell_active: Optional[np.ndarray] = None
+    cell_state: Optional[np.ndarray] = None
+
+
+def _bind(L):
+    if getattr(L, "_ens_bound", False):
+        return L
+    L.adm_normal_random_vector.argtypes = [C.c_uint32, C.c_uint32, C.c_double,
+                                           C.c_double, _dp]
+    L.adm_ensemble_create.argtypes = [C.POINTER(ProblemDesc), C.c_int32,
+                                      C.POINTER(MemberDesc), C.POINTER(_vp)]
+    L.adm_ensemble_destroy.argtypes = [_vp]
+    L.adm_ensemble_n_dofs.argtypes = [_vp, C.POINTER(C.c_int64)]
+    L.adm_ensemble_compute_inverse_mass.argtypes = [_vp, _vp, _vp]
+    L.adm_ensemble_set_time.argtypes = [_vp, C.c_double]
+    L.adm_ensemble_vmult.argtypes = [_vp, _vp, _vp, C.c_int, _vp]
+    L.adm_ensemble_evolve.argtypes = [_vp, C.c_int, C.c_double, C.c_double,
+                                      C.c_int, _vp, _dp, _vp]
+    L.adm_ensemble_get_qp_state.argtypes = [_vp, C.c_int32, _dp, _dp]
+    L._ens_bound = True
+    return L
+
+
+def _lib():
+    return _bind(load_library())
+
+
+# ---------------------------------------------------------------------------
+# draws (source/ensemble_management.cc)
+# ---------------------------------------------------------------------------
+
+def normal_random_vector(length: int, n_rejected_draws: int, mean: float,
+                         stddev: float) -> np.ndarray:
+    """get_normal_random_vector (source/ensemble_management.cc:18-52):
+    default-seeded std::mt19937, n_rejected_draws draws discarded, negative
+    draws redrawn. Runs on the host (no GPU needed)."""
+    out = np.zeros(int(length))
+    check(_lib().adm_normal_random_vector(
+        int(length), int(n_rejected_draws), float(mean), float(stddev),
+        out.ctypes.data_as(_dp)))
+    return out
+
+
+def draw_member_values(keys: Sequence[Tuple[str, float, float]],
+                       global_ensemble_size: int, first_local_member: int = 0,
+                       local_ensemble_size: Optional[int] = None
+                       ) -> Dict[str, np.ndarray]:
+    """create_database_ensemble (source/ensemble_management.cc:166-265) for
+    the (key, mean, stddev) triples in input order: the k-th key discards
+    first_local_member + k * global_ensemble_size draws."""
+    n_local = (global_ensemble_size if local_ensemble_size is None
+               else local_ensemble_size)
+    out = {}
+    n_rejected = first_local_member
+    for key, mean, stddev in keys:
+        out[key] = normal_random_vector(n_local, n_rejected, mean, stddev)
+        n_rejected += global_ensemble_size
+    return out
+
+
+# ---------------------------------------------------------------------------
+# one member as a stand-alone problem
+# ---------------------------------------------------------------------------
+
+def member_problem(base: Problem, m: Member) -> Problem:
+    """The stand-alone Problem of one member: what the reference builds from
+    the member's database. Used to run a member alone (the serial member loop
+    of run_ensemble) and by the tests."""
+    p = copy.deepcopy(base)
+    for b, beam in enumerate(p.beams):
+        if m.absorption_efficiency is not None:
+            beam.absorption_efficiency = float(m.absorption_efficiency[b])
+        if m.max_power is not None:
+            beam.max_power = float(m.max_power[b])
+    if m.conductivity_scale != 1.0:
+        s = float(m.conductivity_scale)
+        for name in ("thermal_conductivity_x", "thermal_conductivity_y",
+                     "thermal_conductivity_z"):
+            k = STATE_PROPERTY_NAMES.index(name)
+            p.material.table[:, k, :, :, 1] *= s
+            p.material.poly[:, k, :, :] *= s
+    if m.cell_active is not None:
+        p.cell_active = np.asarray(m.cell_active, dtype=np.uint8).copy()
+    if m.cell_state is not None:
+        p.cell_state = np.asarray(m.cell_state, dtype=np.float64).copy()
+    return p
+
+
+# ---------------------------------------------------------------------------
+# device ensemble
+# ---------------------------------------------------------------------------
+
+class EnsembleDevice:
+    """All members' ThermalOperator + ThermalPhysics (explicit steppers) in
+    one object; every call acts on every member."""
+
+    def __init__(self, base: Problem, members: Sequence[Member],
+                 method: Optional[int] = None):
+        L = _lib()
+        self.base = base
+        self.members = list(members)
+        self.method = base.method if method is None else method
+        M = len(self.members)
+        self._desc, keep = base.to_c()
+        md = (MemberDesc * max(1, M))()
+        nc = base.n_cells_total
+        ns = base.material.n_states
+        for j, m in enumerate(self.members):
+            d = md[j]
+            for b in range(MAX_BEAMS):
+                if b < len(base.beams):
+                    eta = (base.beams[b].absorption_efficiency
+                           if m.absorption_efficiency is None
+                           else m.absorption_efficiency[b])
+                    pw = (base.beams[b].max_power if m.max_power is None
+                          else m.max_power[b])
+                    d.absorption_efficiency[b] = float(eta)
+                    d.max_power[b] = float(pw)
+            d.conductivity_scale = float(m.conductivity_scale)
+            if m.cell_active is not None:
+                a = np.ascontiguousarray(m.cell_active, dtype=np.uint8)
+                if a.size != nc:
+                    raise ValueError("cell_active has the wrong size")
+                keep.append(a)
+                d.cell_active = a.ctypes.data_as(C.POINTER(C.c_uint8))
+            if m.cell_state is not None:
+                a = np.ascontiguousarray(m.cell_state, dtype=np.float64)
+                if a.size != nc * ns:
+                    raise ValueError("cell_state has the wrong size")
+                keep.append(a)
+                d.cell_state = a.ctypes.data_as(_dp)
+        h = _vp()
+        check(L.adm_ensemble_create(C.byref(self._desc), M, md, C.byref(h)))
+        self._h = h
+        self._keep = keep
+        n = C.c_int64()
+        check(L.adm_ensemble_n_dofs(h, C.byref(n)))
+        self.n_members = M
+        self.n_dofs = n.value
+        self.n_cells = nc
+        self.n_q = (base.fe_degree + 1) ** base.dim
+        self._inverse_mass = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib().adm_ensemble_destroy(h)
+            self._h = None
+
+    def initialize_dof_vector(self, value: float = 0.0, device=None):
+        """[n_members, n_dofs] float64 tensor (member-major storage)."""
+        import torch
+        return torch.full((self.n_members, self.n_dofs), float(value),
+                          dtype=torch.float64, device=device or "cuda")
+
+    def compute_inverse_mass_matrix(self, stream=None):
+        self._inverse_mass = self.initialize_dof_vector()
+        check(_lib().adm_ensemble_compute_inverse_mass(
+            self._h, _dev_ptr(self._inverse_mass), _stream_ptr(stream)))
+
+    def get_inverse_mass_matrix(self):
+        if self._inverse_mass is None:
+            self.compute_inverse_mass_matrix()
+        return self._inverse_mass
+
+    def set_time(self, t: float):
+        check(_lib().adm_ensemble_set_time(self._h, float(t)))
+
+    def _check_vec(self, v):
+        if tuple(v.shape) != (self.n_members, self.n_dofs) and \
+                v.numel() != self.n_members * self.n_dofs:
+            raise ValueError("ensemble vectors hold n_members * n_dofs values")
+
+    def vmult(self, dst, src, stream=None):
+        self._check_vec(dst)
+        self._check_vec(src)
+        check(_lib().adm_ensemble_vmult(self._h, _dev_ptr(src), _dev_ptr(dst),
+                                        0, _stream_ptr(stream)))
+
+    def vmult_add(self, dst, src, stream=None):
+        self._check_vec(dst)
+        self._check_vec(src)
+        check(_lib().adm_ensemble_vmult(self._h, _dev_ptr(src), _dev_ptr(dst),
+                                        1, _stream_ptr(stream)))
+
+    def evolve(self, t: float, delta_t: float, n_steps: int, solution,
+               stream=None) -> float:
+        self._check_vec(solution)
+        t_end = C.c_double()
+        check(_lib().adm_ensemble_evolve(
+            self._h, int(self.method), float(t), float(delta_t), int(n_steps),
+            _dev_ptr(solution), C.byref(t_end), _stream_ptr(stream)))
+        return t_end.value
+
+    def evolve_one_time_step(self, t, delta_t, solution, stream=None):
+        return self.evolve(t, delta_t, 1, solution, stream=stream)
+
+    def qp_state(self, member: int):
+        liq = np.zeros((self.n_cells, self.n_q))
+        pw = np.zeros((self.n_cells, self.n_q))
+        check(_lib().adm_ensemble_get_qp_state(
+            self._h, int(member), liq.ctypes.data_as(_dp),
+            pw.ctypes.data_as(_dp)))
+        return liq, pw

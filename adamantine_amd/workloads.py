@@ -131,3 +131,51 @@ def smooth_temperature(problem: Problem) -> np.ndarray:
     L, W, H = problem.extent
     return (300.0 + 50.0 * np.sin(np.pi * X[:, 0] / L) *
             np.cos(np.pi * X[:, 1] / W) + 10.0 * X[:, 2] / H)
+
+
+def ensemble_workload(n_members: int = 64, n_cells=(64, 32, 16),
+                      extent=(4 * MM, 2 * MM, 1 * MM), powder_layers: int = 4,
+                      method=RK_FOURTH_ORDER):
+    """The ensemble workload: ``n_members`` members over one 64x32x16 Q2 box
+    with the material model of ``plain_large``.
+
+    Member j draws beam_0.absorption_efficiency ~ N(0.3, 0.05) and a
+    thermal-conductivity factor ~ N(1, 0.1), in that key order, with the
+    reference's scheme (default-seeded mt19937, the k-th key discarding
+    k * n_members draws, negatives redrawn). Member j has the lowest
+    L_j = 12 + (j mod 5) cell layers active (at most n_z; 3/4 n_z + (j mod 5)
+    for other n_z), the top ``powder_layers`` of them powder. The serpentine
+    scan path runs on the top of the lowest active height (12 layers), so
+    deeper members see the source below their surface.
+
+    Returns (base problem, members)."""
+    from .ensemble import Member, draw_member_values
+    nz = n_cells[2]
+    hz = extent[2] / nz
+    base_layers = (3 * nz) // 4
+    p = plain_large(n_cells=n_cells, extent=extent,
+                    powder_layers=powder_layers, method=method)
+    top = base_layers * hz
+    p.beams[0].segments = serpentine(z=top)
+    p.set_material_height(top)
+    p.set_powder_layer(top, powder_layers * hz)
+    vals = draw_member_values(
+        [("beam_0.absorption_efficiency", 0.3, 0.05),
+         ("thermal_conductivity_scale", 1.0, 0.1)], n_members)
+    z = p.cell_centers()[:, -1]
+    members = []
+    for j in range(n_members):
+        layers = min(base_layers + j % 5, nz)
+        height = layers * hz
+        active = (z < height).astype(np.uint8)
+        st = np.zeros((p.n_cells_total, 3))
+        st[:, 0] = 1.0
+        powder = z >= height - powder_layers * hz
+        st[powder, 0] = 0.0
+        st[powder, 2] = 1.0
+        members.append(Member(
+            absorption_efficiency=[vals["beam_0.absorption_efficiency"][j]],
+            max_power=[p.beams[0].max_power],
+            conductivity_scale=float(vals["thermal_conductivity_scale"][j]),
+            cell_active=active, cell_state=st))
+    return p, members

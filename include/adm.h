@@ -237,9 +237,9 @@ int adm_vmult(void *ctx, const double *src, double *dst, int add,
 
 /* Repeated evolve_one_time_step (source/ThermalPhysics.templates.hh:854-888):
  * n_steps steps of `method` from time t with step dt on the device solution
- * vector T (n_dofs doubles, updated in place). With use_graph != 0 the step
- * loop is captured once in a hipGraph and replayed. t_end (host, may be NULL)
- * receives the final time. */
+ * vector T (n_dofs doubles, updated in place). use_graph is reserved (the
+ * steps are enqueued as direct launches whatever its value). t_end (host,
+ * may be NULL) receives the final time. */
 int adm_evolve(void *ctx, int method, double t, double dt, int n_steps,
                double *T, int use_graph, double *t_end, void *stream);
 
@@ -280,6 +280,71 @@ int adm_set_beam_parameters(void *ctx, int beam, double absorption_efficiency,
 /* Evaluate beam `beam` of the context at time t on the host, with the
  * forward-only segment search of ScanPath (source/ScanPath.cc:197-275). */
 int adm_beam_state_at(void *ctx, int beam, double t, adm_beam_state *out);
+
+/* ------------------------------------------------------------------------ */
+/* Ensemble                                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* get_normal_random_vector (source/ensemble_management.cc:18-52): `length`
+ * draws of N(mean, stddev) from a default-seeded std::mt19937 after
+ * discarding n_rejected_draws draws; negative draws are redrawn. out: host.
+ * create_database_ensemble (:213-265) calls it once per perturbed key, the
+ * k-th key (in input order) discarding first_local_member + k*n_members. */
+int adm_normal_random_vector(uint32_t length, uint32_t n_rejected_draws,
+                             double mean, double stddev, double *out);
+
+/* What one ensemble member changes in the base problem. The reference builds
+ * one database per member by drawing each `ensemble.<key>_stddev` key
+ * (source/ensemble_management.cc:166-265) and one ThermalPhysics per member
+ * (application/adamantine.hh:1961-2060); these are the keys that can vary
+ * here. Every member shares the mesh, FE space, material tables (up to the
+ * conductivity scale), boundary types, beam kinds, radii, depths and scan
+ * paths of the base problem. */
+typedef struct adm_member_desc {
+  double absorption_efficiency[ADM_MAX_BEAMS]; /* beam_<b>.absorption_efficiency */
+  double max_power[ADM_MAX_BEAMS];             /* beam_<b>.max_power */
+  /* multiplies every thermal conductivity table value / polynomial
+   * coefficient of every material and state (1 = unchanged) */
+  double conductivity_scale;
+  const uint8_t *cell_active;  /* host [n_cells]; NULL: the base problem's */
+  const double *cell_state;    /* host [n_cells][n_states]; NULL: base */
+} adm_member_desc;
+
+/* Creates an ensemble of n_members operators/physics over one mesh. Member
+ * vectors are stored member-major: member j's nodes are
+ * [j*n_dofs, (j+1)*n_dofs) of every ensemble vector. Replaces the
+ * per-member ThermalPhysics objects of run_ensemble
+ * (application/adamantine.hh:1961-2060). */
+int adm_ensemble_create(const adm_problem_desc *base, int32_t n_members,
+                        const adm_member_desc *members, void **ens);
+int adm_ensemble_destroy(void *ens);
+
+/* n_dofs per member (identical for all members). */
+int adm_ensemble_n_dofs(const void *ens, int64_t *n_dofs);
+
+/* Every member's compute_inverse_mass_matrix (see adm_compute_inverse_mass);
+ * inv_mass: device, n_members*n_dofs doubles. */
+int adm_ensemble_compute_inverse_mass(void *ens, double *inv_mass,
+                                      void *stream);
+
+/* set_time(t) for every member. */
+int adm_ensemble_set_time(void *ens, double t);
+
+/* vmult / vmult_add of every member in one launch per kernel:
+ * dst_j = A_j(src_j) (+ dst_j). src, dst: device, n_members*n_dofs. */
+int adm_ensemble_vmult(void *ens, const double *src, double *dst, int add,
+                       void *stream);
+
+/* evolve_one_time_step of every member, n_steps times, batched. This
+ * replaces the serial member loop of run_ensemble
+ * (application/adamantine.hh:2137-2142). T: device, n_members*n_dofs. */
+int adm_ensemble_evolve(void *ens, int method, double t, double dt,
+                        int n_steps, double *T, double *t_end, void *stream);
+
+/* Member `member`'s per-quadrature-point liquid and powder ratios (host,
+ * [n_cells][n_q]; either pointer may be NULL). */
+int adm_ensemble_get_qp_state(void *ens, int32_t member, double *liquid,
+                              double *powder);
 
 #ifdef __cplusplus
 }

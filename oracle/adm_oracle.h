@@ -70,6 +70,12 @@ void oracle_basis_1d(const oracle_t *o, double *support, double *qpoints,
                      double *qweights, double *shape, double *coll_deriv,
                      double *gl_weights);
 
+/* Ensemble draws (adm_oracle_da.c): get_normal_random_vector of
+ * source/ensemble_management.cc:18-52 and the raw default-seeded MT19937. */
+void oracle_normal_random_vector(uint32_t length, uint32_t n_rejected_draws,
+                                 double mean, double stddev, double *out);
+void oracle_mt19937(uint32_t n, uint32_t *out);
+
 #ifdef __cplusplus
 }
 #endif
