@@ -63,6 +63,16 @@ def _bind(L):
     L.adm_ensemble_evolve.argtypes = [_vp, C.c_int, C.c_double, C.c_double,
                                       C.c_int, _vp, _dp, _vp]
     L.adm_ensemble_get_qp_state.argtypes = [_vp, C.c_int32, _dp, _dp]
+    _ip = C.POINTER(C.c_int64)
+    L.adm_da_create.argtypes = [C.POINTER(_vp)]
+    L.adm_da_destroy.argtypes = [_vp]
+    L.adm_da_fill_noise.argtypes = [_vp, C.c_int32, C.c_int32, _dp, _dp]
+    L.adm_ensemble_nearest_dofs.argtypes = [_vp, C.c_int32, C.c_int64, _dp,
+                                            _ip]
+    L.adm_ensemble_analysis.argtypes = [_vp, _vp, C.c_int32, _dp, C.c_int32,
+                                        _ip, _dp, _dp, _dp, C.c_int32,
+                                        C.c_double, _vp]
+    L.adm_ensemble_transform.argtypes = [_vp, _vp, _dp, _vp]
     L._ens_bound = True
     return L
 
@@ -248,3 +258,108 @@ class EnsembleDevice:
             self._h, int(member), liq.ctypes.data_as(_dp),
             pw.ctypes.data_as(_dp)))
         return liq, pw
+
+
+# ---------------------------------------------------------------------------
+# data assimilation (source/DataAssimilator.cc)
+# ---------------------------------------------------------------------------
+
+LOCALIZATION = {"none": 0, "step_function": 1, "gaspari_cohn": 2}
+
+
+class DataAssimilator:
+    """Counterpart of adamantine's DataAssimilator (source/DataAssimilator.hh)
+    for a device ensemble: the observation-noise stream, the observation
+    operator and update_ensemble. ``localization_cutoff_function`` and
+    ``localization_cutoff_distance`` are the reference's input keys."""
+
+    def __init__(self, localization_cutoff_function: str = "none",
+                 localization_cutoff_distance: float = np.finfo(float).max):
+        key = localization_cutoff_function.lower()
+        if key not in LOCALIZATION:
+            raise ValueError("Unknown localization cutoff function. Valid "
+                             "options are 'gaspari_cohn', 'step_function', "
+                             "and 'none'.")
+        self.localization = LOCALIZATION[key]
+        self.cutoff = float(localization_cutoff_distance)
+        h = _vp()
+        check(_lib().adm_da_create(C.byref(h)))
+        self._h = h
+        self.expt_to_dof = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib().adm_da_destroy(h)
+            self._h = None
+
+    def fill_noise_vectors(self, count: int, R) -> np.ndarray:
+        """fill_noise_vector (:630-670) for `count` members in order."""
+        R = np.ascontiguousarray(R, dtype=np.float64)
+        m = R.shape[0]
+        out = np.zeros((int(count), m))
+        check(_lib().adm_da_fill_noise(self._h, int(count), m,
+                                       R.ctypes.data_as(_dp),
+                                       out.ctypes.data_as(_dp)))
+        return out
+
+    def update_dof_mapping(self, ensemble: "EnsembleDevice", points,
+                           member: int = 0) -> np.ndarray:
+        """get_expt_to_dof_mapping (source/experimental_data_utils.cc:
+        70-110) + update_dof_mapping: nearest active DoF of every point."""
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(pts.shape[0], dtype=np.int64)
+        check(_lib().adm_ensemble_nearest_dofs(
+            ensemble._h, int(member), pts.shape[0], pts.ctypes.data_as(_dp),
+            out.ctypes.data_as(C.POINTER(C.c_int64))))
+        self.expt_to_dof = out
+        return out
+
+    def update_ensemble(self, ensemble: "EnsembleDevice", solution, expt_data,
+                        R, params=None, noise=True, stream=None):
+        """update_ensemble (:95-189) on the device ensemble `solution`
+        ([n_members, n_dofs] tensor, in place) and the host parameter block
+        `params` ([n_members, n_params] float64 array, in place). noise:
+        True draws the perturbations from this object's stream, an array
+        gives them, False/None uses none."""
+        if self.expt_to_dof is None:
+            raise RuntimeError("update_dof_mapping first")
+        obs = self.expt_to_dof
+        y = np.ascontiguousarray(expt_data, dtype=np.float64)
+        R = np.ascontiguousarray(R, dtype=np.float64)
+        m = obs.size
+        if y.size != m or R.shape != (m, m):
+            raise ValueError("Unexpected experiment vector size.")
+        M = ensemble.n_members
+        if noise is True:
+            nz = self.fill_noise_vectors(M, R)
+        elif noise is None or noise is False:
+            nz = None
+        else:
+            nz = np.ascontiguousarray(noise, dtype=np.float64)
+        npar = 0
+        pp = None
+        if params is not None:
+            if not (isinstance(params, np.ndarray) and params.dtype == np.float64
+                    and params.flags.c_contiguous and params.shape[0] == M):
+                raise ValueError("params: C-contiguous float64 [n_members, k]")
+            npar = params.shape[1]
+            pp = params.ctypes.data_as(_dp)
+        ensemble._check_vec(solution)
+        check(_lib().adm_ensemble_analysis(
+            ensemble._h, _dev_ptr(solution), npar, pp, m,
+            obs.ctypes.data_as(C.POINTER(C.c_int64)), y.ctypes.data_as(_dp),
+            R.ctypes.data_as(_dp), None if nz is None else nz.ctypes.data_as(_dp),
+            self.localization, self.cutoff, _stream_ptr(stream)))
+
+
+def ensemble_transform(ensemble: EnsembleDevice, solution, Tm, stream=None):
+    """solution <- solution^T-combination: member j becomes
+    sum_l Tm[l, j] * member l (FP64 MFMA kernel)."""
+    Tm = np.ascontiguousarray(Tm, dtype=np.float64)
+    if Tm.shape != (ensemble.n_members, ensemble.n_members):
+        raise ValueError("Tm must be n_members x n_members")
+    ensemble._check_vec(solution)
+    check(_lib().adm_ensemble_transform(ensemble._h, _dev_ptr(solution),
+                                        Tm.ctypes.data_as(_dp),
+                                        _stream_ptr(stream)))

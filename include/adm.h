@@ -346,6 +346,62 @@ int adm_ensemble_evolve(void *ens, int method, double t, double dt,
 int adm_ensemble_get_qp_state(void *ens, int32_t member, double *liquid,
                               double *powder);
 
+/* ------------------------------------------------------------------------ */
+/* Data assimilation (EnKF)                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* DataAssimilator localization_cutoff_function (source/DataAssimilator.cc:
+ * 70-90) */
+enum adm_localization {
+  ADM_LOCALIZATION_NONE = 0,
+  ADM_LOCALIZATION_STEP_FUNCTION = 1,
+  ADM_LOCALIZATION_GASPARI_COHN = 2
+};
+
+/* The observation-noise stream of a DataAssimilator: its default-seeded
+ * std::mt19937 _prng and N(0, 1) _normal_dist_generator
+ * (source/DataAssimilator.hh:280-286), whose state persists across calls. */
+int adm_da_create(void **da);
+int adm_da_destroy(void *da);
+
+/* fill_noise_vector (source/DataAssimilator.cc:630-670) for `count`
+ * consecutive members: out[c][i] (host) = sqrt(R_ii) z_i for diagonal R,
+ * else (L z)_i with R = L L^T. R: host, n_obs x n_obs row-major. */
+int adm_da_fill_noise(void *da, int32_t count, int32_t n_obs, const double *R,
+                      double *out);
+
+/* get_expt_to_dof_mapping (source/experimental_data_utils.cc:70-110): the
+ * nearest DoF support point, among nodes of active cells of `member`, of
+ * each point (host points[n_points][3] -> host dofs[n_points]). Equidistant
+ * candidates resolve to the lowest node index. */
+int adm_ensemble_nearest_dofs(void *ens, int32_t member, int64_t n_points,
+                              const double *points, int64_t *dofs);
+
+/* DataAssimilator::update_ensemble (source/DataAssimilator.cc:95-189) on the
+ * augmented state [T; parameters]: with d_j = noise_j + (obs - H x_j) and P
+ * the localised sample covariance (calc_sample_covariance_sparse, :690-752),
+ * x_j += P H^T (H P H^T + R)^{-1} d_j (apply_kalman_gain, :403-476).
+ *   T       device, n_members*n_dofs (member-major), updated in place
+ *   params  host, n_members*n_params (member-major), updated in place
+ *   obs_dof host, n_obs DoFs observed (calc_H, :478-502)
+ *   obs     host, n_obs values;  R host, n_obs x n_obs row-major
+ *   noise   host, n_members*n_obs perturbations (adm_da_fill_noise) or NULL
+ *   localization adm_localization, cutoff its distance (DBL_MAX: none).
+ * Parameter rows are never localised, as in the reference. The inverse is
+ * a direct solve (the reference iterates GMRES to its tolerance). */
+int adm_ensemble_analysis(void *ens, double *T, int32_t n_params,
+                          double *params, int32_t n_obs,
+                          const int64_t *obs_dof, const double *obs,
+                          const double *R, const double *noise,
+                          int32_t localization, double cutoff, void *stream);
+
+/* The ensemble-space update kernel by itself: T <- T Tm, i.e. member j
+ * becomes sum_l Tm[l][j] T_l (Tm host, n_members x n_members row-major).
+ * No reference counterpart; adm_ensemble_analysis applies it with
+ * Tm = I + W. */
+int adm_ensemble_transform(void *ens, double *T, const double *Tm,
+                           void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,6 +11,8 @@
 #ifndef ADM_ORACLE_H
 #define ADM_ORACLE_H
 
+#include <stddef.h>
+
 #include "../include/adm.h"
 
 #ifdef __cplusplus
@@ -75,6 +77,28 @@ void oracle_basis_1d(const oracle_t *o, double *support, double *qpoints,
 void oracle_normal_random_vector(uint32_t length, uint32_t n_rejected_draws,
                                  double mean, double stddev, double *out);
 void oracle_mt19937(uint32_t n, uint32_t *out);
+
+/* EnKF analysis (adm_oracle_da.c), source/DataAssimilator.cc. fn: 0 none,
+ * 1 step_function, 2 gaspari_cohn. Ensembles are [N][n] member-major,
+ * rows >= n_sim are augmented parameters. */
+double oracle_gaspari_cohn(double r);
+double oracle_localization(int fn, double d, double cutoff);
+void oracle_sample_covariance(int N, int64_t n, int64_t n_sim,
+                              const double *ens, const double *coords, int dim,
+                              int fn, double cutoff, double *P);
+void oracle_apply_kalman_gain(int N, int64_t n, int64_t n_sim,
+                              const double *ens, const double *coords, int dim,
+                              int fn, double cutoff, int m,
+                              const int64_t *obs_dof, const double *R,
+                              const double *innov, double *out);
+void oracle_update_ensemble(int N, int64_t n, int64_t n_sim, double *ens,
+                            const double *coords, int dim, int fn,
+                            double cutoff, int m, const int64_t *obs_dof,
+                            const double *y, const double *R,
+                            const double *noise);
+size_t oracle_da_rng_size(void);
+void oracle_fill_noise(void *state, int count, int m, const double *R,
+                       double *out);
 
 #ifdef __cplusplus
 }
