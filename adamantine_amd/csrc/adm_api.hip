@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <random>
@@ -272,6 +273,8 @@ int Context::property_kind() const {
 }
 
 Context::Context(const adm_problem_desc &d) {
+  if (const char *e = std::getenv("ADM_CELL_ATOMICS"))
+    use_coloring = !(e[0] == '1');
   const adm_mesh_desc &m = d.mesh;
   if (m.dim != 3)
     throw AdmError(ADM_ERR_UNSUPPORTED, "the device path is three-dimensional");
@@ -338,6 +341,8 @@ Context::Context(const adm_problem_desc &d) {
     kc.origin[k] = m.origin[k];
     det *= kc.h[k];
   }
+  for (int q = 0; q < n1; ++q) kc.wq[q] = wq[q];
+  kc.det = det;
   for (int q = 0; q < nq; ++q) {
     double w = 1.0;
     int r = q;
@@ -741,26 +746,59 @@ void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
   const size_t lds = pk == 2 ? sizeof(double) * kMergedLds * kc.mat.n_materials
                              : 0;
   const unsigned M = n_members;
+  // the cell kernel runs once per parity color (cells of one color share no
+  // node: plain read-add-write of dst), or once with atomics
+  const int n_colors = use_coloring ? 8 : 1;
+  for (int col = 0; col < n_colors; ++col) {
+    CellArgs ac = a;
+    int64_t nlaunch = n_cells;
+    ac.color = -1;
+    if (use_coloring) {
+      ac.color = col;
+      nlaunch = 1;
+      for (int d = 0; d < 3; ++d) {
+        const int b = (col >> d) & 1;
+        ac.ncc[d] = nc[d] > b ? (nc[d] - b + 1) / 2 : 0;
+        nlaunch *= ac.ncc[d];
+      }
+      ac.n_color_cells = nlaunch;
+      if (nlaunch == 0) continue;
+    }
 #define ADM_DISPATCH(P_, NS_)                                           \
   if (p == P_ && n_states == NS_) {                                      \
-    if (pk == 0) {                                                       \
-      launch_cell<P_, NS_, 0>(a, src, dst, n_cells, lds, M, s);          \
-      if (fa.n_faces) launch_face<P_, NS_, 0>(fa, src, dst, M, s);       \
-    } else if (pk == 1) {                                                \
-      launch_cell<P_, NS_, 1>(a, src, dst, n_cells, lds, M, s);          \
-      if (fa.n_faces) launch_face<P_, NS_, 1>(fa, src, dst, M, s);       \
-    } else {                                                             \
-      launch_cell<P_, NS_, 2>(a, src, dst, n_cells, lds, M, s);          \
-      if (fa.n_faces) launch_face<P_, NS_, 2>(fa, src, dst, M, s);       \
-    }                                                                    \
+    if (pk == 0)                                                         \
+      launch_cell<P_, NS_, 0>(ac, src, dst, nlaunch, lds, M, s);         \
+    else if (pk == 1)                                                    \
+      launch_cell<P_, NS_, 1>(ac, src, dst, nlaunch, lds, M, s);         \
+    else                                                                 \
+      launch_cell<P_, NS_, 2>(ac, src, dst, nlaunch, lds, M, s);         \
   }
-  ADM_DISPATCH(1, 1)
-  ADM_DISPATCH(1, 2)
-  ADM_DISPATCH(1, 3)
-  ADM_DISPATCH(2, 1)
-  ADM_DISPATCH(2, 2)
-  ADM_DISPATCH(2, 3)
+    ADM_DISPATCH(1, 1)
+    ADM_DISPATCH(1, 2)
+    ADM_DISPATCH(1, 3)
+    ADM_DISPATCH(2, 1)
+    ADM_DISPATCH(2, 2)
+    ADM_DISPATCH(2, 3)
 #undef ADM_DISPATCH
+  }
+  if (fa.n_faces) {
+#define ADM_DISPATCH(P_, NS_)                                           \
+  if (p == P_ && n_states == NS_) {                                      \
+    if (pk == 0)                                                         \
+      launch_face<P_, NS_, 0>(fa, src, dst, M, s);                       \
+    else if (pk == 1)                                                    \
+      launch_face<P_, NS_, 1>(fa, src, dst, M, s);                       \
+    else                                                                 \
+      launch_face<P_, NS_, 2>(fa, src, dst, M, s);                       \
+  }
+    ADM_DISPATCH(1, 1)
+    ADM_DISPATCH(1, 2)
+    ADM_DISPATCH(1, 3)
+    ADM_DISPATCH(2, 1)
+    ADM_DISPATCH(2, 2)
+    ADM_DISPATCH(2, 3)
+#undef ADM_DISPATCH
+  }
   ADM_HIP(hipGetLastError());
 }
 

@@ -11,7 +11,8 @@ void launch_cell(const CellArgs &a, const double *src, double *dst,
                  hipStream_t s) {
   constexpr int cpb = CellLanes<P>::cells_per_block;
   const unsigned grid = (unsigned)((n_cells + cpb - 1) / cpb);
-  const size_t xbytes = sizeof(double) * cpb * (P + 1) * (P + 1) * (P + 1);
+  const size_t xbytes = sizeof(double) * cpb * CellLanes<P>::buffers *
+                        (P + 1) * (P + 1) * (P + 1);
   hipLaunchKernelGGL((cell_kernel<P, NS, PK>), dim3(grid, n_members), dim3(256),
                      lds_bytes + xbytes, s, a, src, dst);
 }

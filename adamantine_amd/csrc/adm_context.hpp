@@ -94,6 +94,9 @@ struct Context {
   std::vector<int> old_segment;
   std::vector<adm_beam_state> beam_states;
   bool sources_on = false;
+  // cell kernel in 8 parity colors without atomics (default), or one
+  // atomic launch (ADM_CELL_ATOMICS=1 in the environment at creation)
+  bool use_coloring = true;
 
   // device buffers
   KConst *d_kc = nullptr;

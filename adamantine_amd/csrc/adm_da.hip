@@ -133,10 +133,12 @@ static void lu_solve(int m, std::vector<double> &A, int nrhs,
   }
   for (int q = 0; q < nrhs; ++q) {
     double *b = B.data() + (size_t)q * m;
-    for (int c = 0; c < m; ++c) {
+    // P b first (the row swaps moved whole rows, stored multipliers
+    // included), then L y = P b
+    for (int c = 0; c < m; ++c)
       if (piv[c] != c) std::swap(b[c], b[piv[c]]);
+    for (int c = 0; c < m; ++c)
       for (int r = c + 1; r < m; ++r) b[r] -= A[(size_t)r * m + c] * b[c];
-    }
     for (int r = m - 1; r >= 0; --r) {
       double s = b[r];
       for (int k = r + 1; k < m; ++k) s -= A[(size_t)r * m + k] * b[k];
@@ -166,7 +168,7 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // NT output tiles of 16 nodes x 16 members on v_mfma_f64_16x16x4f64,
 // K = members in steps of 4. Operand layouts (16x16x4 f64): A lane l holds
 // A[row l%16][k l/16]; B lane l holds B[k l/16][col l%16]; D lane l
-// register r holds D[row 4(l/16)+r][col l%16]. Every X value of the
+// register r holds D[row 4r + l/16][col l%16]. Every X value of the
 // wavefront's nodes is read before any is written, and wavefronts own
 // disjoint nodes, so the update is in place.
 template <int NT>
@@ -201,7 +203,7 @@ __global__ void __launch_bounds__(256)
     if (j >= N) continue;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int64_t nd = i0 + 4 * kq + r;
+      const int64_t nd = i0 + 4 * r + kq;
       if (nd < n) X[(int64_t)j * n + nd] = acc[t][r];
     }
   }

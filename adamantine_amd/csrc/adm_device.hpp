@@ -70,7 +70,9 @@ struct KConst {
   double Dc[kMaxN][kMaxN];   // collocation derivative at the quadrature points
   double DcS[kMaxN][kMaxN];  // Dc * S: derivative at the points from nodes
   double xq[kMaxN];
-  double jxw[kMaxQ];
+  double wq[kMaxN];          // 1-D quadrature weights
+  double det;                // cell Jacobian determinant
+  double jxw[kMaxQ];         // det * ((1 * wq[x]) * wq[y]) * wq[z]
   double fjxw[3][kMaxFQ];    // face JxW per normal direction
   double h[3], inv_h[3], origin[3];
   int32_t n_boundary_ids;
@@ -130,6 +132,13 @@ struct CellArgs {
   int64_t member_stride;
   double beam_pmod[ADM_MAX_BEAMS];
   double beam_den[ADM_MAX_BEAMS];
+  // color >= 0: the launch covers the cells whose coordinate parities are
+  // (color & 1, color >> 1 & 1, color >> 2 & 1), ncc[d] of them per axis.
+  // Such cells share no node, so dst is updated by plain read-add-write;
+  // color < 0: every cell, atomic accumulation.
+  int32_t color;
+  int32_t ncc[3];
+  int64_t n_color_cells;
 };
 
 struct FaceArgs {

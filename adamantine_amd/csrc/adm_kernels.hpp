@@ -1,11 +1,11 @@
 // HIP kernels of the matrix-free heat operator (gfx950).
 //
-// Cell kernel: one lane per cell. The (p+1)^3 nodal values are gathered into
-// registers and the whole sum factorisation (values, collocation gradients,
-// transposed integration) runs in registers in FP64 with explicit FMA chains.
-// The per-quadrature-point physics (phase state, rho c_p with latent heat,
-// k(T), deposition rotation, heat sources) and the boundary-face terms of the
-// cell are fused in. Reference: source/ThermalOperator.templates.hh:479-751.
+// Cell kernel: (p+1)^2 lanes per cell, one x-line each; the sum
+// factorisation runs in FP64 with explicit FMA chains, exchanging lines
+// through LDS, and the per-quadrature-point physics (phase state, rho c_p
+// with latent heat, k(T), deposition rotation, heat sources) is fused in.
+// Boundary faces with a convective/radiative term have their own kernel.
+// Reference: source/ThermalOperator.templates.hh:479-751.
 #pragma once
 
 #include <type_traits>
@@ -42,6 +42,20 @@ __device__ __forceinline__ void sweep(const double (&M)[kMaxN][kMaxN],
                           acc);
     out[o] = ACC ? out[o] + acc : acc;
   }
+}
+
+// The pointers carried in the argument structs are generic; kernels
+// re-qualify them so that uniform constants become scalar loads and per-cell
+// data global (not flat) loads, which LDS waits do not have to wait for.
+#define ADM_GLOBAL __attribute__((address_space(1)))
+#define ADM_CONSTANT __attribute__((address_space(4)))
+template <class T>
+__device__ __forceinline__ ADM_GLOBAL T *as_global(T *p) {
+  return (ADM_GLOBAL T *)p;
+}
+template <class T>
+__device__ __forceinline__ const ADM_CONSTANT T *as_constant(const T *p) {
+  return (const ADM_CONSTANT T *)p;
 }
 
 // Property evaluator selected by the PK template parameter:
@@ -92,12 +106,17 @@ __global__ void __launch_bounds__(256)
                 double *__restrict__ dst) {
   constexpr int N = P + 1, NF = N * N;
   const int64_t fi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const MemberArgs *mb = a.members ? a.members + blockIdx.y : nullptr;
+  const ADM_CONSTANT MemberArgs *mb =
+      a.members ? as_constant(a.members) + blockIdx.y : nullptr;
   if (fi >= (mb ? mb->n_faces : a.n_faces)) return;
-  const KConst &kc = *(mb ? mb->kc : a.kc);
+  const KConst &kc = *(const KConst *)as_global(mb ? mb->kc : a.kc);
   const MatConst &m = kc.mat;
-  const FaceRef fr = (mb ? mb->faces : a.faces)[fi];
-  double *face_powder = mb ? mb->face_powder : a.face_powder;
+  const ADM_GLOBAL FaceRef *frp = as_global(mb ? mb->faces : a.faces) + fi;
+  FaceRef fr;
+  fr.cell = frp->cell;
+  fr.info = frp->info;
+  ADM_GLOBAL double *face_powder =
+      as_global(mb ? mb->face_powder : a.face_powder);
   if (mb) {
     src += (int64_t)blockIdx.y * a.member_stride;
     dst += (int64_t)blockIdx.y * a.member_stride;
@@ -121,7 +140,7 @@ __global__ void __launch_bounds__(256)
   }
   sweep<N, 0, false, false, 2>(kc.S, fu, ft);
   sweep<N, 1, false, false, 2>(kc.S, ft, fv);
-  double *fp = NS == 3 ? face_powder + fi * NF : nullptr;
+  ADM_GLOBAL double *fp = NS == 3 ? face_powder + fi * NF : nullptr;
 #pragma unroll
   for (int q = 0; q < NF; ++q) {
     const double T = fv[q];
@@ -163,17 +182,27 @@ constexpr int kMergedLds = kMergedMax + 5 * kMergedMax * ADM_MAX_STATES * 2;
 
 // Cell kernel (source/ThermalOperator.templates.hh:479-617).
 //
-// N = p+1 lanes per cell, lane k of a cell owning node plane z = k and
-// quadrature plane qz = k. The x and y contractions and the per-point
-// physics of the plane's N*N points run in registers; the z contractions
-// (values, z-gradient, transposed z-gradient, transposed values) exchange
-// planes through a wave-private LDS buffer. The N lanes of a cell sit in the
-// same wavefront, so no workgroup barrier is involved.
+// (p+1)^2 lanes per cell: a lane owns one x-line of (p+1) nodes or
+// quadrature points, indexed by its two other coordinates (a, b). x
+// contractions run in the lane's registers; y and z contractions exchange
+// lines through a wave-private LDS buffer (the lanes of a cell share a
+// wavefront, so no workgroup barrier is involved). The operation order of
+// every contraction is the oracle's: values by S along x, y, z; gradients by
+// the collocation derivative Dc; integration by Dc^T along x, y, z, the
+// source term, then S^T along z, y, x. A lane does the physics of its line's
+// p+1 quadrature points.
+//
+// Uniform constants are read into registers before the first global store
+// (the stores could otherwise alias them and force reloads); entries indexed
+// by the lane's (a, b) come from a block-shared LDS copy.
 template <int P>
 struct CellLanes {
   static constexpr int N = P + 1;
-  static constexpr int cells_per_wave = 64 / N;
+  static constexpr int lanes_per_cell = N * N;
+  static constexpr int cells_per_wave = 64 / (N * N);
   static constexpr int cells_per_block = 4 * cells_per_wave;
+  // exchange buffers per cell, N^3 doubles each
+  static constexpr int buffers = 2;
 };
 
 // LDS ordering between the lanes of one wavefront
@@ -183,339 +212,395 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Merged-table coefficients of one interval j for the properties the cell
-// term needs (density, specific heat, conductivities), for states 0 and 2.
-struct SolidCoef {
-  double b;        // left breakpoint of the interval
-  double c[5][2][2];  // [prop][state 0 or 2][value, slope]
+// Block-shared copies of the small constant tables (read by lane-varying
+// indices from LDS, which global stores cannot alias).
+struct CellConstLds {
+  double S[kMaxN][kMaxN], Dc[kMaxN][kMaxN], xq[kMaxN], wq[kMaxN];
+  double solidus[ADM_MAX_MATERIALS], liquidus[ADM_MAX_MATERIALS];
+  double latent[ADM_MAX_MATERIALS], iso[ADM_MAX_MATERIALS];
 };
 
 template <int P, int NS, int PK>
 __global__ void __launch_bounds__(256)
     cell_kernel(const CellArgs a, const double *__restrict__ src,
                 double *__restrict__ dst) {
-  constexpr int N = P + 1, NP = N * N;
+  constexpr int N = P + 1, NL = N * N, N3 = N * N * N;
   constexpr int CPW = CellLanes<P>::cells_per_wave;
-  // ensemble member of this block (uniform), else the single operator
-  const MemberArgs *mb = a.members ? a.members + blockIdx.y : nullptr;
-  const KConst &kc = *(mb ? mb->kc : a.kc);
-  const MatConst &m = kc.mat;
-  const uint32_t *meta_p = mb ? mb->meta : a.meta;
-  const int32_t *slot_p = mb ? mb->powder_slot : a.powder_slot;
-  double *powder_p = mb ? mb->powder : a.powder;
-  double *liquid_p = mb ? mb->liquid : a.liquid;
-  uint8_t *dirty_p = mb ? mb->liquid_dirty : a.liquid_dirty;
-  const double *cos_p = mb ? mb->dep_cos : a.dep_cos;
-  const double *sin_p = mb ? mb->dep_sin : a.dep_sin;
+  // ensemble member of this block (uniform), else the single operator.
+  // The pointers carried in the argument structs are generic; they are
+  // re-qualified so that uniform constants become scalar loads and per-cell
+  // data global (not flat) loads, which LDS waits do not have to wait for.
+  const ADM_CONSTANT MemberArgs *mb =
+      a.members ? as_constant(a.members) + blockIdx.y : nullptr;
+  const ADM_CONSTANT KConst &kc = *as_constant(mb ? mb->kc : a.kc);
+  const MatConst &m = *(const MatConst *)&kc.mat;
+  const ADM_GLOBAL uint32_t *meta_p = as_global(mb ? mb->meta : a.meta);
+  const ADM_GLOBAL int32_t *slot_p =
+      as_global(mb ? mb->powder_slot : a.powder_slot);
+  ADM_GLOBAL double *powder_p = as_global(mb ? mb->powder : a.powder);
+  ADM_GLOBAL double *liquid_p = as_global(mb ? mb->liquid : a.liquid);
+  ADM_GLOBAL uint8_t *dirty_p =
+      as_global(mb ? mb->liquid_dirty : a.liquid_dirty);
+  const ADM_GLOBAL double *cos_p = as_global(mb ? mb->dep_cos : a.dep_cos);
+  const ADM_GLOBAL double *sin_p = as_global(mb ? mb->dep_sin : a.dep_sin);
   if (mb) {
     src += (int64_t)blockIdx.y * a.member_stride;
     dst += (int64_t)blockIdx.y * a.member_stride;
   }
   extern __shared__ double lds[];
   __shared__ adm_beam_state sbeams[ADM_MAX_BEAMS];
-  // one lane copies the beam states with uniform (scalar) loads
-  if (threadIdx.x == 0) {
-    static_for<0, ADM_MAX_BEAMS>([&](auto bc) {
-      constexpr int bb = decltype(bc)::value;
-      sbeams[bb] = a.beams[bb];
-      if (mb)
-        sbeams[bb].alpha = member_alpha(a.beams[bb], mb->beam_gain[bb],
-                                        a.beam_pmod[bb], a.beam_den[bb],
-                                        kc.log_01);
-    });
-  }
-  if constexpr (PK == 2) {
-    const int total = m.n_materials * kMergedLds;
-    for (int i = threadIdx.x; i < total; i += blockDim.x) {
-      const int mm = i / kMergedLds, k = i % kMergedLds;
-      lds[i] = (&m.merged[mm].B[0])[k];
-    }
-  }
-  __syncthreads();
-  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-  const int j = lane / N, kz = lane % N;
-  const int64_t c = ((int64_t)blockIdx.x * 4 + wave) * CPW + j;
-  const bool inside = j < CPW && c < a.n_cells;
-  const uint32_t meta = inside ? meta_p[c] : 0u;
-  if (!(meta & kActive)) return;  // whole cells (all N lanes) leave together
-  // exchange buffer of this cell: [N planes][NP]
-  double *xb = lds + (PK == 2 ? m.n_materials * kMergedLds : 0) +
-               ((int64_t)wave * CPW + j) * N * NP;
-  const int mid = (meta >> kMaterialShift) & 3;
-  int cc[3];
-  cc[0] = (int)(c % a.nc[0]);
-  cc[1] = (int)((c / a.nc[0]) % a.nc[1]);
-  cc[2] = (int)(c / ((int64_t)a.nc[0] * a.nc[1]));
-  const int64_t sy = a.nn[0], sz = (int64_t)a.nn[0] * a.nn[1];
-  const int64_t base = (int64_t)cc[0] * P + sy * cc[1] * P +
-                       sz * ((int64_t)cc[2] * P + kz);
-
-  // node plane kz, then S along x and y (the first two sweeps of the oracle)
-  double u[NP], t[NP];
-#pragma unroll
-  for (int jj = 0; jj < N; ++jj)
-#pragma unroll
-    for (int i = 0; i < N; ++i) u[i + N * jj] = src[base + i + sy * jj];
-  sweep<N, 0, false, false, 2>(kc.S, u, t);
-  sweep<N, 1, false, false, 2>(kc.S, t, u);
-  // S along z gives the values on quadrature plane kz (same chain as the
-  // oracle); Dc*S along z gives the z-derivative from the same planes
-  double V[NP], gzv[NP];
-#pragma unroll
-  for (int i = 0; i < NP; ++i) xb[kz * NP + i] = u[i];
-  wave_lds_sync();
+  __shared__ CellConstLds sk;
+  const int n_mat = kc.mat.n_materials;
+  // block prologue: beams (one 8-byte word per lane), constant tables and
+  // merged tables
   {
-    double srow[N], drow[N];
+    constexpr int words = sizeof(adm_beam_state) * ADM_MAX_BEAMS / 8;
+    static_assert(sizeof(adm_beam_state) % 8 == 0, "beam state layout");
+    const int tid = threadIdx.x;
+    if (tid < words)
+      reinterpret_cast<uint64_t *>(sbeams)[tid] =
+          reinterpret_cast<const uint64_t *>(a.beams)[tid];
+    constexpr int nn = kMaxN * kMaxN;
+    if (tid < nn) {
+      (&sk.S[0][0])[tid] = (&kc.S[0][0])[tid];
+      (&sk.Dc[0][0])[tid] = (&kc.Dc[0][0])[tid];
+    } else if (tid < nn + kMaxN) {
+      sk.xq[tid - nn] = kc.xq[tid - nn];
+      sk.wq[tid - nn] = kc.wq[tid - nn];
+    } else if (tid >= 64 && tid < 64 + ADM_MAX_MATERIALS) {
+      const int k = tid - 64;
+      sk.solidus[k] = kc.mat.prop[k][ADM_SOLIDUS];
+      sk.liquidus[k] = kc.mat.prop[k][ADM_LIQUIDUS];
+      sk.latent[k] = kc.mat.latent_factor[k];
+      sk.iso[k] = kc.mat.isotropic[k] ? 1.0 : 0.0;
+    }
+    if constexpr (PK == 2) {
+      const int total = n_mat * kMergedLds;
+      for (int i = tid; i < total; i += blockDim.x) {
+        const int mm = i / kMergedLds, k = i % kMergedLds;
+        lds[i] = (&kc.mat.merged[mm].B[0])[k];
+      }
+    }
+    __syncthreads();
+    if (mb && tid < ADM_MAX_BEAMS)
+      sbeams[tid].alpha = member_alpha(sbeams[tid], mb->beam_gain[tid],
+                                       a.beam_pmod[tid], a.beam_den[tid],
+                                       kc.log_01);
+    if (mb) __syncthreads();
+  }
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const int j = lane / NL, L = lane % NL;
+  const int la = L % N, lb = L / N;  // the line's y and z indices
+  const int64_t ci = ((int64_t)blockIdx.x * 4 + wave) * CPW + j;
+  int cc[3];
+  int64_t c;
+  bool inside;
+  if (a.color >= 0) {
+    inside = j < CPW && ci < a.n_color_cells;
+    const int64_t q0 = ci % a.ncc[0], q1 = (ci / a.ncc[0]) % a.ncc[1],
+                  q2 = ci / ((int64_t)a.ncc[0] * a.ncc[1]);
+    cc[0] = (int)(2 * q0 + (a.color & 1));
+    cc[1] = (int)(2 * q1 + ((a.color >> 1) & 1));
+    cc[2] = (int)(2 * q2 + ((a.color >> 2) & 1));
+    c = cc[0] + (int64_t)a.nc[0] * (cc[1] + (int64_t)a.nc[1] * cc[2]);
+  } else {
+    inside = j < CPW && ci < a.n_cells;
+    c = ci;
+    cc[0] = (int)(c % a.nc[0]);
+    cc[1] = (int)((c / a.nc[0]) % a.nc[1]);
+    cc[2] = (int)(c / ((int64_t)a.nc[0] * a.nc[1]));
+  }
+  if (!inside) return;  // whole cells (all lanes) leave together
+  const int64_t sy = a.nn[0], sz = (int64_t)a.nn[0] * a.nn[1];
+  // node line (y = la, z = lb) of the cell
+  const int64_t base = (int64_t)cc[0] * P + sy * ((int64_t)cc[1] * P + la) +
+                       sz * ((int64_t)cc[2] * P + lb);
+  // independent loads first: the node line and the cell's words
+  double u[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) u[i] = src[base + i];
+  const uint32_t meta = meta_p[c];
+  const int32_t slot = slot_p[c];
+  bool dirty = false;
+  if constexpr (NS > 1)
+    if (liquid_p) dirty = dirty_p[c] != 0;
+  // uniform constants (scalar loads)
+  double S[N][N], Dc[N][N], xq[N], wq[N];
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    xq[q] = kc.xq[q];
+    wq[q] = kc.wq[q];
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-      srow[k] = kc.S[kz][k];
-      drow[k] = kc.DcS[kz][k] * kc.inv_h[2];
+      S[q][k] = kc.S[q][k];
+      Dc[q][k] = kc.Dc[q][k];
     }
+  }
+  const double det = kc.det, log_01 = kc.log_01;
+  double ih[3], hh[3], org[3];
 #pragma unroll
-    for (int i = 0; i < NP; ++i) {
-      const double u0 = xb[i];
-      double acc = srow[0] * u0, gz = drow[0] * u0;
+  for (int d = 0; d < 3; ++d) {
+    ih[d] = kc.inv_h[d];
+    hh[d] = kc.h[d];
+    org[d] = kc.origin[d];
+  }
+  // lane-indexed entries come from the LDS copy
+  const double wa = sk.wq[la], wb = sk.wq[lb];
+  const double xqa = sk.xq[la], xqb = sk.xq[lb];
+  if (!(meta & kActive)) return;
+  double *xb = lds + (PK == 2 ? n_mat * kMergedLds : 0) +
+               ((int64_t)wave * CPW + j) * (CellLanes<P>::buffers * N3);
+  double *xb2 = xb + N3;
+  const int mid = (meta >> kMaterialShift) & 3;
+  const double solidus = sk.solidus[mid], liquidus = sk.liquidus[mid],
+               latent = sk.latent[mid];
+  const bool iso = sk.iso[mid] != 0.0;
+  // this lane's slot in an [z][y][x] buffer of the cell
+  const int own = N * la + NL * lb;
+
+  // values: S along x (registers), y and z (LDS)
+  double t[N], V[N];
 #pragma unroll
-      for (int k = 1; k < N; ++k) {
-        const double uk = xb[k * NP + i];
-        acc = __builtin_fma(srow[k], uk, acc);
-        gz = __builtin_fma(drow[k], uk, gz);
-      }
-      V[i] = acc;
-      gzv[i] = gz;
+  for (int q = 0; q < N; ++q) {
+    double acc = S[q][0] * u[0];
+#pragma unroll
+    for (int k = 1; k < N; ++k) acc = __builtin_fma(S[q][k], u[k], acc);
+    t[q] = acc;
+  }
+#pragma unroll
+  for (int q = 0; q < N; ++q) xb[own + q] = t[q];
+  wave_lds_sync();
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    double acc = sk.S[la][0] * xb[NL * lb + q];
+#pragma unroll
+    for (int k = 1; k < N; ++k)
+      acc = __builtin_fma(sk.S[la][k], xb[N * k + NL * lb + q], acc);
+    u[q] = acc;
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int q = 0; q < N; ++q) xb[own + q] = u[q];
+  wave_lds_sync();
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    double acc = sk.S[lb][0] * xb[N * la + q];
+#pragma unroll
+    for (int k = 1; k < N; ++k)
+      acc = __builtin_fma(sk.S[lb][k], xb[N * la + NL * k + q], acc);
+    V[q] = acc;
+  }
+  // gradients by the collocation derivative
+  wave_lds_sync();
+#pragma unroll
+  for (int q = 0; q < N; ++q) xb[own + q] = V[q];
+  wave_lds_sync();
+  double gx[N], gy[N], gz[N];
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    double ax = Dc[q][0] * V[0];
+    double ay = sk.Dc[la][0] * xb[NL * lb + q];
+    double az = sk.Dc[lb][0] * xb[N * la + q];
+#pragma unroll
+    for (int k = 1; k < N; ++k) {
+      ax = __builtin_fma(Dc[q][k], V[k], ax);
+      ay = __builtin_fma(sk.Dc[la][k], xb[N * k + NL * lb + q], ay);
+      az = __builtin_fma(sk.Dc[lb][k], xb[N * la + NL * k + q], az);
     }
+    gx[q] = ax * ih[0];
+    gy[q] = ay * ih[1];
+    gz[q] = az * ih[2];
   }
 
-  const int64_t pslot = (meta & kHasPowder) ? (int64_t)slot_p[c] : -1;
-  double *pw = pslot >= 0 ? powder_p + (pslot * N + kz) * NP : nullptr;
-  double *lq = nullptr;
-  bool dirty = false;
-  if constexpr (NS > 1) {
-    if (liquid_p) {
-      dirty = dirty_p[c] != 0;
-      lq = liquid_p + (c * N + kz) * NP;
-    }
-  }
+  // per-point physics
+  const int64_t pslot = (meta & kHasPowder) ? (int64_t)slot : -1;
+  ADM_GLOBAL double *pw = pslot >= 0 ? powder_p + pslot * N3 + own : nullptr;
+  ADM_GLOBAL double *lq = nullptr;
+  if constexpr (NS > 1)
+    if (liquid_p) lq = liquid_p + c * N3 + own;
   const bool rotated = (meta & kRotated) != 0;
   double cs = 1.0, sn = 0.0;
   if (rotated) {
     cs = cos_p[c];
     sn = sin_p[c];
   }
-  const bool iso = m.isotropic[mid] != 0;
   const double *mB = lds + mid * kMergedLds;
   const double *mC = mB + kMergedMax;
   bool near = false;
   if (a.sources_on) {
     double lo[3], hi[3];
     for (int d = 0; d < 3; ++d) {
-      lo[d] = kc.origin[d] + cc[d] * kc.h[d];
-      hi[d] = kc.origin[d] + (cc[d] + 1) * kc.h[d];
+      lo[d] = org[d] + cc[d] * hh[d];
+      hi[d] = org[d] + (cc[d] + 1) * hh[d];
     }
 #pragma unroll 1
     for (int b = 0; b < a.n_beams; ++b) near |= beam_may_touch(sbeams[b], lo, hi);
   }
-  const double z = kc.origin[2] + (cc[2] + kc.xq[kz]) * kc.h[2];
-
-  // Fast path: every point of the plane below the solidus and inside one
-  // merged interval. Then the liquid ratio is 0 and the ratios are
-  // (1 - p, 0, p) with p the stored powder, the liquid terms of the
-  // reference's mixture are r*X = 0 (dropped exactly), and the table
-  // coefficients are loaded once for the plane.
-  double tmin = V[0], tmax = V[0];
+  const double py = org[1] + (cc[1] + xqa) * hh[1];
+  const double pz = org[2] + (cc[2] + xqb) * hh[2];
+  double p_in[N];
 #pragma unroll
-  for (int i = 1; i < NP; ++i) {
-    tmin = fmin(tmin, V[i]);
-    tmax = fmax(tmax, V[i]);
-  }
-  bool fast = PK == 2;
-  int jint = 0;
-  double bj = 0.0;
-  if constexpr (PK == 2) {
-    if constexpr (NS > 1) fast = tmax < m.prop[mid][ADM_SOLIDUS];
-    const MergedEval elo(mB, mC, tmin), ehi(mB, mC, tmax);
-    fast = fast && elo.j == ehi.j && !rotated;
-    jint = elo.j;
-    bj = mB[jint > 0 ? jint - 1 : 0];
-  }
-
-  double A[NP], w2v[NP];
+  for (int q = 0; q < N; ++q) p_in[q] = pw ? pw[q] : 0.0;
+  double w0[N], w1[N], w2[N], val[N];
+  bool line_liquid = false;
 #pragma unroll
-  for (int i = 0; i < NP; ++i) A[i] = 0.0;
-  bool plane_liquid = false;
-
-  // flux and source of one point from (1/rho c_p, k_x, k_y, k_z)
-  auto finish = [&](auto ic, double T, double irc, double kx, double ky,
-                    double kzz, double gx, double gy) {
-    constexpr int i = decltype(ic)::value;
-    constexpr int qx = i % N, qy = i / N;
+  for (int q = 0; q < N; ++q) {
+    const double T = V[q];
+    double p = p_in[q];
+    double r[NS];
+    double l = 0.0;
+    // update_state_ratios (source/ThermalOperator.templates.hh:227-317),
+    // branch-free
+    if constexpr (NS == 1) {
+      r[0] = 1.0;
+    } else {
+      const double mushy = (T - solidus) / (liquidus - solidus);
+      l = T < solidus ? 0.0 : (T > liquidus ? 1.0 : mushy);
+      if constexpr (NS == 2) {
+        r[1] = l;
+        r[0] = 1.0 - l;
+      } else {
+        const double am = 1.0 - l;
+        p = (p < am) ? p : am;
+        r[1] = l;
+        r[2] = p;
+        r[0] = 1.0 - l - p;
+      }
+    }
+    if (pw && p != p_in[q]) pw[q] = p;
+    if constexpr (NS > 1) {
+      if (lq && (l != 0.0 || dirty)) lq[q] = l;
+      line_liquid |= l != 0.0;
+    }
+    double irc, kx, ky, kzz;
+    auto props = [&](const auto &ev) {
+      // get_inv_rho_cp (source/ThermalOperator.templates.hh:415-477)
+      const double density = ev(ADM_DENSITY, r);
+      double specific_heat = ev(ADM_SPECIFIC_HEAT, r);
+      if constexpr (NS > 1) {
+        const double is_mushy = r[1] * r[0];
+        if (is_mushy > 0.0) specific_heat += latent;
+      }
+      irc = 1.0 / (density * specific_heat);
+      kx = ev(ADM_THERMAL_CONDUCTIVITY_X, r);
+      if (iso) {
+        ky = kx;
+        kzz = kx;
+      } else {
+        ky = ev(ADM_THERMAL_CONDUCTIVITY_Y, r);
+        kzz = ev(ADM_THERMAL_CONDUCTIVITY_Z, r);
+      }
+    };
+    if constexpr (PK == 2)
+      props(MergedEval(mB, mC, T));
+    else
+      props(typename EvalOf<PK>::type(m, mid, T));
     double fx, fy;
     if (rotated) {
-      fx = (kx * cs * cs + ky * sn * sn) * gx + ((kx - ky) * sn * cs) * gy;
-      fy = ((kx - ky) * sn * cs) * gx + (kx * sn * sn + ky * cs * cs) * gy;
+      fx = (kx * cs * cs + ky * sn * sn) * gx[q] + ((kx - ky) * sn * cs) * gy[q];
+      fy = ((kx - ky) * sn * cs) * gx[q] + (kx * sn * sn + ky * cs * cs) * gy[q];
     } else {
       // cos = 1, sin = 0: the reference's rotation gives k_x g_x, k_y g_y
-      fx = kx * gx;
-      fy = ky * gy;
+      fx = kx * gx[q];
+      fy = ky * gy[q];
     }
-    const double fz = gzv[i] * kzz;
-    const double jxw = kc.jxw[kz * NP + i];
-    const double w0 = ((-irc) * fx) * jxw * kc.inv_h[0];
-    const double w1 = ((-irc) * fy) * jxw * kc.inv_h[1];
-    w2v[i] = ((-irc) * fz) * jxw * kc.inv_h[2];
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-      A[i - qx + k] = __builtin_fma(kc.Dc[qx][k], w0, A[i - qx + k]);
-      A[i - qy * N + k * N] =
-          __builtin_fma(kc.Dc[qy][k], w1, A[i - qy * N + k * N]);
-    }
+    const double fz = gz[q] * kzz;
+    // jxw as the host tabulates it: det * ((1 * w_x) * w_y) * w_z
+    double wxyz = 1.0;
+    wxyz *= wq[q];
+    wxyz *= wa;
+    wxyz *= wb;
+    const double jxw = det * wxyz;
+    w0[q] = ((-irc) * fx) * jxw * ih[0];
+    w1[q] = ((-irc) * fy) * jxw * ih[1];
+    w2[q] = ((-irc) * fz) * jxw * ih[2];
+    val[q] = 0.0;
     if (near) {
-      const double x = kc.origin[0] + (cc[0] + kc.xq[qx]) * kc.h[0];
-      const double y = kc.origin[1] + (cc[1] + kc.xq[qy]) * kc.h[1];
+      const double px = org[0] + (cc[0] + xq[q]) * hh[0];
       double qs = 0.0;
 #pragma unroll 1
       for (int b = 0; b < a.n_beams; ++b)
-        qs += beam_value(sbeams[b], x, y, z, kc.log_01);
+        qs += beam_value(sbeams[b], px, py, pz, log_01);
       qs *= irc;
-      A[i] += qs * jxw;
+      val[q] = qs * jxw;
     }
-  };
-
-  if (fast) {
-    // interval coefficients: [prop][state 0 / 2][value, slope]
-    constexpr int s2 = NS == 3 ? 2 : 0;
-    double cf[5][2][2];
-#pragma unroll
-    for (int pr = 0; pr < 5; ++pr)
-#pragma unroll
-      for (int ss = 0; ss < 2; ++ss) {
-        const double *cp =
-            mC + ((pr * kMergedMax + jint) * ADM_MAX_STATES + (ss ? s2 : 0)) * 2;
-        cf[pr][ss][0] = cp[0];
-        cf[pr][ss][1] = cp[1];
-      }
-    const double latent_mushy = 0.0;  // liquid ratio 0: never mushy
-    (void)latent_mushy;
-    static_for<0, NP>([&](auto ic) {
-      constexpr int i = decltype(ic)::value;
-      constexpr int qx = i % N, qy = i / N;
-      const double T = V[i];
-      const double d = T - bj;
-      const double gx = line_dot<N>(kc.Dc, qx, V, i - qx, 1) * kc.inv_h[0];
-      const double gy = line_dot<N>(kc.Dc, qy, V, i - qy * N, N) * kc.inv_h[1];
-      // ratios (1 - p, 0, p); with p = 0 only the solid term remains
-      double p = 0.0;
-      if (NS == 3 && pw) p = pw[i];
-      auto mix = [&](int pr) {
-        const double x0 = __builtin_fma(d, cf[pr][0][1], cf[pr][0][0]);
-        double v = (1.0 - 0.0 - p) * x0;
-        if constexpr (NS == 3) {
-          const double x2 = __builtin_fma(d, cf[pr][1][1], cf[pr][1][0]);
-          v += p * x2;
-        }
-        return v;
-      };
-      const double irc = 1.0 / (mix(ADM_DENSITY) * mix(ADM_SPECIFIC_HEAT));
-      const double kx = mix(ADM_THERMAL_CONDUCTIVITY_X);
-      double ky = kx, kzz = kx;
-      if (!iso) {
-        ky = mix(ADM_THERMAL_CONDUCTIVITY_Y);
-        kzz = mix(ADM_THERMAL_CONDUCTIVITY_Z);
-      }
-      finish(ic, T, irc, kx, ky, kzz, gx, gy);
-    });
-  } else {
-    static_for<0, NP>([&](auto ic) {
-      constexpr int i = decltype(ic)::value;
-      constexpr int qx = i % N, qy = i / N;
-      const double T = V[i];
-      const double gx = line_dot<N>(kc.Dc, qx, V, i - qx, 1) * kc.inv_h[0];
-      const double gy = line_dot<N>(kc.Dc, qy, V, i - qy * N, N) * kc.inv_h[1];
-      double p = pw ? pw[i] : 0.0;
-      const double p_old = p;
-      double r[NS];
-      const double l = state_ratios_select<NS>(m, mid, T, p, r);
-      if (pw && p != p_old) pw[i] = p;
-      if constexpr (NS > 1) {
-        if (lq && (l != 0.0 || dirty)) lq[i] = l;
-        plane_liquid |= l != 0.0;
-      }
-      double irc, kx, ky, kzz;
-      auto props = [&](const auto &ev) {
-        irc = inv_rho_cp<NS>(m, mid, r, ev);
-        kx = ev(ADM_THERMAL_CONDUCTIVITY_X, r);
-        if (iso) {
-          ky = kx;
-          kzz = kx;
-        } else {
-          ky = ev(ADM_THERMAL_CONDUCTIVITY_Y, r);
-          kzz = ev(ADM_THERMAL_CONDUCTIVITY_Z, r);
-        }
-      };
-      if constexpr (PK == 2)
-        props(MergedEval(mB, mC, T));
-      else
-        props(typename EvalOf<PK>::type(m, mid, T));
-      finish(ic, T, irc, kx, ky, kzz, gx, gy);
-      __builtin_amdgcn_sched_barrier(0);
-    });
   }
   if constexpr (NS > 1) {
-    // fast planes hold no liquid; a plane that held liquid is cleared
-    if (fast && lq && dirty) {
+    // cell-wide "held liquid" flag over the cell's lanes
+    const uint64_t ball = __ballot(line_liquid);
+    const bool any_liquid = ((ball >> (j * NL)) & ((1ull << NL) - 1)) != 0;
+    if (lq && L == 0 && any_liquid != dirty) dirty_p[c] = any_liquid ? 1 : 0;
+  }
+
+  // integration: Dc^T along x (registers), y and z (LDS), the source
+  // term, then S^T along z, y, x
 #pragma unroll
-      for (int i = 0; i < NP; ++i) lq[i] = 0.0;
+  for (int i = 0; i < N; ++i) {
+    double acc = Dc[0][i] * w0[0];
+#pragma unroll
+    for (int q = 1; q < N; ++q) acc = __builtin_fma(Dc[q][i], w0[q], acc);
+    t[i] = acc;
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    xb[own + q] = w1[q];
+    xb2[own + q] = w2[q];
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    double ay = sk.Dc[0][la] * xb[NL * lb + q];
+    double az = sk.Dc[0][lb] * xb2[N * la + q];
+#pragma unroll
+    for (int k = 1; k < N; ++k) {
+      ay = __builtin_fma(sk.Dc[k][la], xb[N * k + NL * lb + q], ay);
+      az = __builtin_fma(sk.Dc[k][lb], xb2[N * la + NL * k + q], az);
     }
-    // cell-wide flag over the N lanes of the cell
-    wave_lds_sync();
-    xb[kz * NP] = plane_liquid ? 1.0 : 0.0;
-    wave_lds_sync();
-    bool any_liquid = false;
-#pragma unroll
-    for (int k = 0; k < N; ++k) any_liquid |= xb[k * NP] != 0.0;
-    if (lq && kz == 0 && any_liquid != dirty)
-      dirty_p[c] = any_liquid ? 1 : 0;
-  }
-  // node plane kz: S^T along z of the x/y-integrated planes, plus the
-  // transposed z-derivative of the z-fluxes (Dc*S)^T
-  double out[NP];
-  wave_lds_sync();
-#pragma unroll
-  for (int i = 0; i < NP; ++i) xb[kz * NP + i] = A[i];
-  wave_lds_sync();
-  {
-    double scol[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) scol[k] = kc.S[k][kz];
-#pragma unroll
-    for (int i = 0; i < NP; ++i) {
-      double acc = scol[0] * xb[i];
-#pragma unroll
-      for (int k = 1; k < N; ++k) acc = __builtin_fma(scol[k], xb[k * NP + i], acc);
-      out[i] = acc;
-    }
+    t[q] = t[q] + ay;
+    t[q] = t[q] + az;
+    if (a.sources_on) t[q] += val[q];
   }
   wave_lds_sync();
 #pragma unroll
-  for (int i = 0; i < NP; ++i) xb[kz * NP + i] = w2v[i];
+  for (int q = 0; q < N; ++q) xb[own + q] = t[q];
   wave_lds_sync();
-  {
-    double dcol[N];
 #pragma unroll
-    for (int k = 0; k < N; ++k) dcol[k] = kc.DcS[k][kz];
+  for (int q = 0; q < N; ++q) {  // S^T along z: lane = (y = la, node z = lb)
+    double acc = sk.S[0][lb] * xb[N * la + q];
 #pragma unroll
-    for (int i = 0; i < NP; ++i)
-#pragma unroll
-      for (int k = 0; k < N; ++k) out[i] = __builtin_fma(dcol[k], xb[k * NP + i], out[i]);
+    for (int k = 1; k < N; ++k)
+      acc = __builtin_fma(sk.S[k][lb], xb[N * la + NL * k + q], acc);
+    u[q] = acc;
   }
-  // S^T along y and x
-  sweep<N, 1, true, false, 2>(kc.S, out, t);
-  sweep<N, 0, true, false, 2>(kc.S, t, out);
+  wave_lds_sync();
 #pragma unroll
-  for (int jj = 0; jj < N; ++jj)
+  for (int q = 0; q < N; ++q) xb[own + q] = u[q];
+  wave_lds_sync();
 #pragma unroll
-    for (int i = 0; i < N; ++i)
-      unsafeAtomicAdd(dst + base + i + sy * jj, out[i + N * jj]);
+  for (int q = 0; q < N; ++q) {  // S^T along y: lane = (node y = la, z = lb)
+    double acc = sk.S[0][la] * xb[NL * lb + q];
+#pragma unroll
+    for (int k = 1; k < N; ++k)
+      acc = __builtin_fma(sk.S[k][la], xb[N * k + NL * lb + q], acc);
+    t[q] = acc;
+  }
+  double out[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {  // S^T along x
+    double acc = S[0][i] * t[0];
+#pragma unroll
+    for (int q = 1; q < N; ++q) acc = __builtin_fma(S[q][i], t[q], acc);
+    out[i] = acc;
+  }
+  if (a.color >= 0) {
+    // no other cell of this launch touches these nodes
+#pragma unroll
+    for (int i = 0; i < N; ++i) dst[base + i] += out[i];
+  } else {
+#pragma unroll
+    for (int i = 0; i < N; ++i) unsafeAtomicAdd(dst + base + i, out[i]);
+  }
 }
 
 // Lumped Gauss-Lobatto mass, inverted (source/ThermalOperator.templates.hh:

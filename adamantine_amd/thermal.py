@@ -41,6 +41,10 @@ def load_library():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise AdmError("libadm.so not built: run python -m adamantine_amd.build")
+    # torch's HIP runtime must be the process's one: loaded first, it also
+    # satisfies libadm.so's libamdhip64.so.7 dependency (same soname);
+    # loaded the other way round the process would hold two HIP runtimes
+    import torch  # noqa: F401
     L = C.CDLL(LIB_PATH)
     L.adm_last_error.restype = C.c_char_p
     L.adm_create.argtypes = [C.POINTER(ProblemDesc), C.POINTER(_vp)]

@@ -179,3 +179,28 @@ def ensemble_workload(n_members: int = 64, n_cells=(64, 32, 16),
             conductivity_scale=float(vals["thermal_conductivity_scale"][j]),
             cell_active=active, cell_state=st))
     return p, members
+
+
+def meltpool_temperature(problem: Problem, t: float, peak: float = 2600.0,
+                         ambient: float = 300.0, sigma_xy: float = 150e-6,
+                         sigma_z: float = 80e-6) -> np.ndarray:
+    """A smooth field with a melt pool under the first beam at time t:
+    T = ambient + (peak - ambient) exp(-(dx^2+dy^2)/(2 sxy^2) - dz^2/(2 sz^2)).
+    Solid far from the beam, mushy and liquid in the pool: the temperature
+    pattern of a scan, used as the operator's benchmark input."""
+    from .problem import BeamState  # noqa: F401
+    X = problem.node_coordinates()
+    seg = problem.beams[0].segments
+    # position on the scan path (piecewise linear between segment ends)
+    t0, p0 = 0.0, np.array(seg[0][2], float)
+    center = np.array(seg[-1][2], float)
+    for et, _, xyz in seg:
+        xyz = np.array(xyz, float)
+        if t <= et:
+            f = 0.0 if et == t0 else (t - t0) / (et - t0)
+            center = p0 + f * (xyz - p0)
+            break
+        t0, p0 = et, xyz
+    d2 = ((X[:, 0] - center[0]) ** 2 + (X[:, 1] - center[1]) ** 2) / (
+        2 * sigma_xy ** 2) + (X[:, 2] - center[2]) ** 2 / (2 * sigma_z ** 2)
+    return ambient + (peak - ambient) * np.exp(-d2)

@@ -63,6 +63,83 @@ def cpu_baseline(problem, seconds: float = 12.0):
                       "%d host threads (%.1f s)" % (n, threads, el)}
 
 
+def copy_calibration(steps: int):
+    """A device-to-device copy of 256 MiB (known bytes: 256 MiB read +
+    256 MiB written per launch), the reference point for the HBM counters
+    and the measured copy bandwidth."""
+    import torch
+    n = 256 * 1024 * 1024 // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    return {"copy_bytes": 2 * n * 8, "copy_ms": ms,
+            "copy_GBs": 2 * n * 8 / (ms * 1e-3) / 1e9}
+
+
+def ensemble_bench(n_members: int, n_steps: int):
+    """The ensemble workload (n_members members of the 64x32x16 Q2 box):
+    batched RK4 steps of all members against the serial loop of one
+    single-member solver per member (the reference's run_ensemble
+    schedule), both on the device."""
+    import torch
+    from adamantine_amd.ensemble import EnsembleDevice, member_problem
+    from adamantine_amd.thermal import ThermalPhysicsDevice
+    from adamantine_amd.workloads import ensemble_workload
+    base, members = ensemble_workload(n_members=n_members)
+    stream = torch.cuda.current_stream()
+    dt = 5e-7
+    ens = EnsembleDevice(base, members)
+    ens.compute_inverse_mass_matrix()
+    sol = ens.initialize_dof_vector(300.0)
+    t = ens.evolve(0.0, dt, 3, sol)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    ens.evolve(t, dt, n_steps, sol)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    batched_ms = e0.elapsed_time(e1) / n_steps
+    del ens, sol
+    # serial loop: every member's own solver, one step each, member by member
+    n_loop = min(n_members, 16)
+    phs, sols = [], []
+    for m in members[:n_loop]:
+        ph = ThermalPhysicsDevice(member_problem(base, m))
+        ph.setup()
+        phs.append(ph)
+        sols.append(ph.initialize_dof_vector(300.0))
+    for ph, s in zip(phs, sols):
+        ph.evolve(0.0, dt, 3, s)
+    torch.cuda.synchronize()
+    k_loop = max(2, n_steps // 2)
+    e0.record(stream)
+    for k in range(k_loop):
+        for ph, s in zip(phs, sols):
+            ph.evolve(3 * dt + k * dt, dt, 1, s)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    loop_ms = e0.elapsed_time(e1) / (k_loop * n_loop)
+    batched = n_members / (batched_ms * 1e-3)
+    loop = 1.0 / (loop_ms * 1e-3)
+    return {"members": n_members, "n_dofs_per_member": base.n_dofs,
+            "method": "rk4", "batched_step_ms": batched_ms,
+            "member_steps_per_s": batched,
+            "loop_member_steps_per_s": loop,
+            "speedup_vs_loop": batched / loop,
+            "loop_sample": "%d members x %d steps" % (n_loop, k_loop)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +147,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--only", choices=["all", "vmult", "ensemble", "copy"],
+                    default="all",
+                    help="profiling aid: run one part and print its timing")
+    ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--field", choices=["meltpool", "random"],
+                    default="meltpool",
+                    help="operator input: a melt pool under the beam (solid "
+                         "far away, liquid in the pool) or U(300, 1600) noise")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the operator is not sharded across "
@@ -82,13 +167,23 @@ def main():
     from adamantine_amd.thermal import ThermalOperatorDevice, ThermalPhysicsDevice
     from adamantine_amd.workloads import plain_large, random_temperature
 
+    if args.only == "copy":
+        print(json.dumps(copy_calibration(args.steps)), flush=True)
+        return 0
+    if args.only == "ensemble":
+        print(json.dumps(ensemble_bench(args.members, args.steps)), flush=True)
+        return 0
+
     problem = plain_large()
     op = ThermalOperatorDevice(problem)
     stream = torch.cuda.current_stream()
-    T = torch.tensor(random_temperature(problem, low=300.0, high=1600.0),
-                     device="cuda")
+    from adamantine_amd.workloads import meltpool_temperature
+    t_eval = 1e-4  # beam on (its first track)
+    field = (meltpool_temperature(problem, t_eval) if args.field == "meltpool"
+             else random_temperature(problem, low=300.0, high=1600.0))
+    T = torch.tensor(field, device="cuda")
     dst = op.initialize_dof_vector()
-    op.set_time(1e-4)  # beam on (its first track)
+    op.set_time(t_eval)
 
     for _ in range(args.warmup):
         op.vmult_add(dst, T)
@@ -103,6 +198,11 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = start.elapsed_time(end) / args.steps
+
+    if args.only == "vmult":
+        print(json.dumps({"vmult_ms": ms, "steps": args.steps,
+                          "field": args.field}), flush=True)
+        return 0
 
     liquid, powder = op.qp_state()
     n_powder_qp = int((powder > 0).sum())
@@ -153,6 +253,9 @@ def main():
         "rk4_step_ms": rk4_ms,
         "wall_s": wall,
     }
+    del ph, sol, op, T, dst
+    torch.cuda.empty_cache()
+    result["ensemble"] = ensemble_bench(args.members, 10)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(problem, args.cpu_seconds)
     print(json.dumps(result), flush=True)

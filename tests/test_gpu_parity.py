@@ -94,8 +94,12 @@ def test_vmult_slp_powder_transitions():
     fp = np.zeros(op.n_cells * 6 * op.n_face_q)
     TH.check(TH.load_library().adm_get_face_state(
         op.handle, fp.ctypes.data_as(C.POINTER(C.c_double))))
-    np.testing.assert_array_equal(fp.reshape(op.n_cells, 6, -1),
-                                  o.face_powder())
+    # the faces with a boundary term: the top face (id 5) of the top layer
+    top = np.arange(op.n_cells) >= op.n_cells - p.n_cells[0] * p.n_cells[1]
+    got_fp = fp.reshape(op.n_cells, 6, -1)
+    np.testing.assert_array_equal(got_fp[top, 5], o.face_powder()[top, 5])
+    assert (got_fp[top, 5] < 1.0).any() and (got_fp[top, 5] == 1.0).any()
+    assert not got_fp[~top].any() and not got_fp[top][:, :5].any()
 
 
 def test_vmult_full_size_plain_large():
