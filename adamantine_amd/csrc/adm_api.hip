@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -275,6 +276,14 @@ int Context::property_kind() const {
 Context::Context(const adm_problem_desc &d) {
   if (const char *e = std::getenv("ADM_CELL_ATOMICS"))
     use_coloring = !(e[0] == '1');
+  {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                              dev) == hipSuccess &&
+        cus > 0)
+      n_cu = cus;
+  }
   const adm_mesh_desc &m = d.mesh;
   if (m.dim != 3)
     throw AdmError(ADM_ERR_UNSUPPORTED, "the device path is three-dimensional");
@@ -311,6 +320,9 @@ Context::Context(const adm_problem_desc &d) {
     n_cells *= nc[k];
     n_dofs *= nn[k];
   }
+  // cell and row-segment counts are 32-bit in the kernels
+  if (n_cells > (int64_t)INT32_MAX)
+    throw AdmError(ADM_ERR_UNSUPPORTED, "more than 2^31 - 1 cells");
   n_states = mat.n_states;
   use_table = mat.use_table != 0;
 
@@ -687,6 +699,117 @@ void Context::set_qp_state(const double *liquid, const double *powder) {
   }
 }
 
+// MaterialProperty::set_state (source/MaterialProperty.templates.hh:
+// 537-620): cell means of the quadrature-point ratios.
+void Context::cell_state_means(double *cs) {
+  const int ns = n_states;
+  std::vector<double> lq((size_t)n_cells * nq), pw((size_t)n_cells * nq);
+  get_qp_state(lq.data(), pw.data());
+  for (int64_t c = 0; c < n_cells; ++c) {
+    if (ns == 1) {
+      cs[c] = 1.0;
+      continue;
+    }
+    double ls = 0.0, ps = 0.0;
+    for (int q = 0; q < nq; ++q) {
+      ls += lq[c * nq + q];
+      ps += pw[c * nq + q];
+    }
+    const double l = ls / nq;
+    if (ns == 2) {
+      cs[c * 2 + 1] = l;
+      cs[c * 2] = 1. - l;
+    } else {
+      const double pp = ps / nq;
+      cs[c * 3 + 1] = l;
+      cs[c * 3 + 2] = pp;
+      cs[c * 3] = 1. - l - pp;
+    }
+  }
+}
+
+// Nodes of active cells.
+std::vector<uint8_t> Context::node_activity() const {
+  std::vector<uint8_t> na((size_t)n_dofs, 0);
+  for (int64_t c = 0; c < n_cells; ++c) {
+    if (!active[c]) continue;
+    const int64_t cx = c % nc[0], cy = (c / nc[0]) % nc[1],
+                  cz = c / ((int64_t)nc[0] * nc[1]);
+    for (int k = 0; k <= p; ++k)
+      for (int j = 0; j <= p; ++j)
+        for (int i = 0; i <= p; ++i)
+          na[(size_t)(cx * p + i) +
+             (size_t)nn[0] * ((cy * p + j) + (size_t)nn[1] * (cz * p + k))] = 1;
+  }
+  return na;
+}
+
+// get_elements_to_activate (source/material_deposition.cc:376-435): for
+// every inactive cell, the last box (input order) whose closed box meets the
+// cell's bounding box, else -1. boxes[b] = {min x, y, z, max x, y, z}.
+void Context::cells_to_activate(int n_boxes, const double *boxes,
+                                int32_t *box_of_cell) const {
+  for (int64_t c = 0; c < n_cells; ++c) {
+    box_of_cell[c] = -1;
+    if (active[c]) continue;
+    const int64_t cc[3] = {c % nc[0], (c / nc[0]) % nc[1],
+                           c / ((int64_t)nc[0] * nc[1])};
+    for (int b = 0; b < n_boxes; ++b) {
+      bool hit = true;
+      for (int d = 0; d < 3; ++d) {
+        const double lo = kc.origin[d] + cc[d] * kc.h[d];
+        const double hi = kc.origin[d] + (cc[d] + 1) * kc.h[d];
+        if (!(lo <= boxes[6 * b + 3 + d] && boxes[6 * b + d] <= hi)) hit = false;
+      }
+      if (hit) box_of_cell[c] = b;
+    }
+  }
+}
+
+// add_material_start + add_material_end (source/ThermalPhysics.templates.hh:
+// 595-825) on the structured numbering: phase state to cell means and back
+// to every quadrature point, activation with the box's deposition
+// direction, new_temperature on nodes no previously active cell held, and
+// the lumped mass recomputed.
+void Context::add_material(const int32_t *box_of_cell, const double *box_cos,
+                           const double *box_sin, double new_temperature,
+                           double *T, hipStream_t s) {
+  ADM_HIP(hipStreamSynchronize(s));
+  std::vector<double> cs((size_t)n_cells * n_states);
+  cell_state_means(cs.data());
+  const std::vector<uint8_t> before = node_activity();
+  for (int64_t c = 0; c < n_cells; ++c) {
+    const int b = box_of_cell[c];
+    if (b < 0 || active[c]) continue;
+    active[c] = 1;
+    dep_cos[c] = box_cos[b];
+    dep_sin[c] = box_sin[b];
+  }
+  const std::vector<uint8_t> after = node_activity();
+  std::vector<int64_t> fresh;
+  for (int64_t i = 0; i < n_dofs; ++i)
+    if (after[i] && !before[i]) fresh.push_back(i);
+  std::copy(cs.begin(), cs.end(), cell_state.begin());
+  rebuild_cells();
+  set_cell_state(cell_state.data());
+  if (T && !fresh.empty()) {
+    int64_t *d_idx = nullptr;
+    ADM_HIP(hipMalloc(&d_idx, sizeof(int64_t) * fresh.size()));
+    ADM_HIP(hipMemcpyAsync(d_idx, fresh.data(), sizeof(int64_t) * fresh.size(),
+                           hipMemcpyHostToDevice, s));
+    const int64_t n = (int64_t)fresh.size();
+    hipLaunchKernelGGL(scatter_value_kernel, dim3((unsigned)((n + 255) / 256)),
+                       dim3(256), 0, s, T, d_idx, new_temperature, n);
+    ADM_HIP(hipGetLastError());
+    ADM_HIP(hipStreamSynchronize(s));
+    ADM_HIP(hipFree(d_idx));
+  }
+  if (sb.inv_mass) {
+    compute_inverse_mass(sb.inv_mass, s);
+    ADM_HIP(hipStreamSynchronize(s));
+  }
+}
+
 void Context::set_time(double t) {
   sources_on = false;
   for (size_t b = 0; b < beams.size(); ++b) {
@@ -749,21 +872,22 @@ void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
   // the cell kernel runs once per parity color (cells of one color share no
   // node: plain read-add-write of dst), or once with atomics
   const int n_colors = use_coloring ? 8 : 1;
+  const int cpw = p == 1 ? CellLanes<1>::cells_per_wave
+                         : CellLanes<2>::cells_per_wave;
   for (int col = 0; col < n_colors; ++col) {
     CellArgs ac = a;
-    int64_t nlaunch = n_cells;
-    ac.color = -1;
-    if (use_coloring) {
-      ac.color = col;
-      nlaunch = 1;
-      for (int d = 0; d < 3; ++d) {
-        const int b = (col >> d) & 1;
-        ac.ncc[d] = nc[d] > b ? (nc[d] - b + 1) / 2 : 0;
-        nlaunch *= ac.ncc[d];
-      }
-      ac.n_color_cells = nlaunch;
-      if (nlaunch == 0) continue;
+    ac.color = use_coloring ? col : -1;
+    int64_t cells = 1;
+    for (int d = 0; d < 3; ++d) {
+      const int b = use_coloring ? (col >> d) & 1 : 0;
+      ac.lext[d] = use_coloring ? (nc[d] > b ? (nc[d] - b + 1) / 2 : 0) : nc[d];
+      cells *= ac.lext[d];
     }
+    if (cells == 0) continue;
+    ac.nseg = (ac.lext[0] + cpw - 1) / cpw;
+    ac.n_rows = ac.nseg * ac.lext[1] * ac.lext[2];
+    // one wavefront per row segment, four per block
+    const int64_t nlaunch = (ac.n_rows + 3) / 4;
 #define ADM_DISPATCH(P_, NS_)                                           \
   if (p == P_ && n_states == NS_) {                                      \
     if (pk == 0)                                                         \
@@ -934,9 +1058,17 @@ Ensemble::Ensemble(const adm_problem_desc &base, int n,
     if (m.cell_state) d.cell_state = m.cell_state;
     members.emplace_back(new Context(d));
   }
+  n_dofs = members[0]->n_dofs;
+  refresh_members();
+}
+
+// The device table of per-member pointers (rebuilt after a member's cells,
+// faces or powder storage change).
+void Ensemble::refresh_members() {
+  const int n = n_members;
   const Context &c0 = *members[0];
-  n_dofs = c0.n_dofs;
   std::vector<MemberArgs> h(n);
+  max_faces = 0;
   for (int j = 0; j < n; ++j) {
     const Context &c = *members[j];
     if (c.property_kind() != c0.property_kind())
@@ -961,7 +1093,7 @@ Ensemble::Ensemble(const adm_problem_desc &base, int n,
       m.beam_gain[b] =
           c.beams[b].desc.absorption_efficiency * c.beams[b].desc.max_power;
   }
-  ADM_HIP(hipMalloc(&d_members, sizeof(MemberArgs) * n));
+  if (!d_members) ADM_HIP(hipMalloc(&d_members, sizeof(MemberArgs) * n));
   ADM_HIP(hipMemcpy(d_members, h.data(), sizeof(MemberArgs) * n,
                     hipMemcpyHostToDevice));
 }
@@ -1138,32 +1270,7 @@ int adm_get_cell_state(void *ptr, double *cs) {
   ADM_API_BEGIN
   ADM_CTX(ptr)
   if (!cs) return adm::set_error(ADM_ERR_INVALID, "null state");
-  const int ns = ctx.n_states;
-  const int nq = ctx.nq;
-  std::vector<double> lq((size_t)ctx.n_cells * nq), pw((size_t)ctx.n_cells * nq);
-  ctx.get_qp_state(lq.data(), pw.data());
-  // MaterialProperty::set_state (source/MaterialProperty.templates.hh:537-620)
-  for (int64_t c = 0; c < ctx.n_cells; ++c) {
-    if (ns == 1) {
-      cs[c] = 1.0;
-      continue;
-    }
-    double ls = 0.0, ps = 0.0;
-    for (int q = 0; q < nq; ++q) {
-      ls += lq[c * nq + q];
-      ps += pw[c * nq + q];
-    }
-    const double l = ls / nq;
-    if (ns == 2) {
-      cs[c * 2 + 1] = l;
-      cs[c * 2] = 1. - l;
-    } else {
-      const double pp = ps / nq;
-      cs[c * 3 + 1] = l;
-      cs[c * 3 + 2] = pp;
-      cs[c * 3] = 1. - l - pp;
-    }
-  }
+  ctx.cell_state_means(cs);
   ADM_API_END
 }
 
@@ -1349,6 +1456,64 @@ int adm_ensemble_get_qp_state(void *ptr, int32_t member, double *liquid,
     return adm::set_error(ADM_ERR_INVALID, "member index out of range");
   ADM_HIP(hipDeviceSynchronize());
   ens.members[member]->get_qp_state(liquid, powder);
+  ADM_API_END
+}
+
+int adm_cells_to_activate(void *ptr, int32_t n_boxes, const double *boxes,
+                          int32_t *box_of_cell) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  if (n_boxes < 0 || (n_boxes && !boxes) || !box_of_cell)
+    return adm::set_error(ADM_ERR_INVALID, "bad box list");
+  ctx.cells_to_activate(n_boxes, boxes, box_of_cell);
+  ADM_API_END
+}
+
+int adm_add_material(void *ptr, const int32_t *box_of_cell,
+                     const double *box_cos, const double *box_sin,
+                     double new_temperature, double *T, void *stream) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  if (!box_of_cell || !box_cos || !box_sin)
+    return adm::set_error(ADM_ERR_INVALID, "null argument");
+  ctx.add_material(box_of_cell, box_cos, box_sin, new_temperature, T,
+                   static_cast<hipStream_t>(stream));
+  ADM_API_END
+}
+
+int adm_ensemble_cells_to_activate(void *ptr, int32_t member, int32_t n_boxes,
+                                   const double *boxes,
+                                   int32_t *box_of_cell) {
+  ADM_API_BEGIN
+  ADM_ENS(ptr)
+  if (member < 0 || member >= ens.n_members)
+    return adm::set_error(ADM_ERR_INVALID, "member index out of range");
+  if (n_boxes < 0 || (n_boxes && !boxes) || !box_of_cell)
+    return adm::set_error(ADM_ERR_INVALID, "bad box list");
+  ens.members[member]->cells_to_activate(n_boxes, boxes, box_of_cell);
+  ADM_API_END
+}
+
+int adm_ensemble_add_material(void *ptr, int32_t member,
+                              const int32_t *box_of_cell,
+                              const double *box_cos, const double *box_sin,
+                              double new_temperature, double *T,
+                              void *stream) {
+  ADM_API_BEGIN
+  ADM_ENS(ptr)
+  if (member < 0 || member >= ens.n_members)
+    return adm::set_error(ADM_ERR_INVALID, "member index out of range");
+  if (!box_of_cell || !box_cos || !box_sin)
+    return adm::set_error(ADM_ERR_INVALID, "null argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  adm::Context &c = *ens.members[member];
+  c.add_material(box_of_cell, box_cos, box_sin, new_temperature,
+                 T ? T + (int64_t)member * ens.n_dofs : nullptr, s);
+  ens.refresh_members();
+  if (ens.sb.inv_mass) {
+    c.compute_inverse_mass(ens.sb.inv_mass + (int64_t)member * ens.n_dofs, s);
+    ADM_HIP(hipStreamSynchronize(s));
+  }
   ADM_API_END
 }
 

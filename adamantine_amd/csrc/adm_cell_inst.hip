@@ -7,10 +7,10 @@ namespace adm {
 
 template <int P, int NS, int PK>
 void launch_cell(const CellArgs &a, const double *src, double *dst,
-                 int64_t n_cells, size_t lds_bytes, unsigned n_members,
+                 int64_t n_blocks, size_t lds_bytes, unsigned n_members,
                  hipStream_t s) {
-  constexpr int cpb = CellLanes<P>::cells_per_block;
-  const unsigned grid = (unsigned)((n_cells + cpb - 1) / cpb);
+  constexpr int cpb = 4 * CellLanes<P>::cells_per_wave;
+  const unsigned grid = (unsigned)n_blocks;
   const size_t xbytes = sizeof(double) * cpb * CellLanes<P>::buffers *
                         (P + 1) * (P + 1) * (P + 1);
   hipLaunchKernelGGL((cell_kernel<P, NS, PK>), dim3(grid, n_members), dim3(256),

@@ -61,6 +61,13 @@ struct Context {
   void get_qp_state(double *liquid, double *powder);
   void set_qp_state(const double *liquid, const double *powder);
   void set_time(double t);
+  void cell_state_means(double *cs);
+  std::vector<uint8_t> node_activity() const;
+  void cells_to_activate(int n_boxes, const double *boxes,
+                         int32_t *box_of_cell) const;
+  void add_material(const int32_t *box_of_cell, const double *box_cos,
+                    const double *box_sin, double new_temperature, double *T,
+                    hipStream_t s);
   CellArgs cell_args() const;
   FaceArgs face_args() const;
   void build_merged_tables();
@@ -97,6 +104,7 @@ struct Context {
   // cell kernel in 8 parity colors without atomics (default), or one
   // atomic launch (ADM_CELL_ATOMICS=1 in the environment at creation)
   bool use_coloring = true;
+  int n_cu = 256;  // compute units of the device (grid sizing)
 
   // device buffers
   KConst *d_kc = nullptr;
@@ -123,6 +131,7 @@ struct Ensemble {
   void compute_inverse_mass(double *out, hipStream_t s);
   void ensure_step_buffers(int method);
   void step(int method, double t, double dt, double *T, hipStream_t s);
+  void refresh_members();
 
   int n_members = 0;
   int64_t n_dofs = 0;

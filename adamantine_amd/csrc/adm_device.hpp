@@ -133,12 +133,15 @@ struct CellArgs {
   double beam_pmod[ADM_MAX_BEAMS];
   double beam_den[ADM_MAX_BEAMS];
   // color >= 0: the launch covers the cells whose coordinate parities are
-  // (color & 1, color >> 1 & 1, color >> 2 & 1), ncc[d] of them per axis.
+  // (color & 1, color >> 1 & 1, color >> 2 & 1), lext[d] of them per axis.
   // Such cells share no node, so dst is updated by plain read-add-write;
   // color < 0: every cell, atomic accumulation.
   int32_t color;
-  int32_t ncc[3];
-  int64_t n_color_cells;
+  // cells of the launch per axis (every cell, or one color's), row
+  // segments per x-row and row segments of the launch
+  int32_t lext[3];
+  int32_t nseg;
+  int32_t n_rows;
 };
 
 struct FaceArgs {
@@ -260,10 +263,12 @@ struct MergedEval {
   __device__ __forceinline__ MergedEval(const double *B, const double *coef_,
                                         double T)
       : coef(coef_) {
+    // branch-free binary search (selects only, so the searches of
+    // independent points interleave)
     int k = 0;
 #pragma unroll
     for (int step = kMergedMax / 2; step >= 1; step >>= 1)
-      if (B[k + step - 1] <= T) k += step;
+      k += (B[k + step - 1] <= T) ? step : 0;
     j = k;
     d = T - B[k > 0 ? k - 1 : 0];
   }

@@ -192,15 +192,16 @@ constexpr int kMergedLds = kMergedMax + 5 * kMergedMax * ADM_MAX_STATES * 2;
 // source term, then S^T along z, y, x. A lane does the physics of its line's
 // p+1 quadrature points.
 //
-// Uniform constants are read into registers before the first global store
-// (the stores could otherwise alias them and force reloads); entries indexed
-// by the lane's (a, b) come from a block-shared LDS copy.
+// A wavefront handles a row segment of 64 / (p+1)^2 cells along x at fixed
+// (y, z), so cell coordinates are uniform scalar arithmetic. Uniform constants
+// are read into registers before the first global store (the stores could
+// otherwise alias them and force reloads); entries indexed by the lane's
+// (a, b) come from a block-shared LDS copy.
 template <int P>
 struct CellLanes {
   static constexpr int N = P + 1;
   static constexpr int lanes_per_cell = N * N;
   static constexpr int cells_per_wave = 64 / (N * N);
-  static constexpr int cells_per_block = 4 * cells_per_wave;
   // exchange buffers per cell, N^3 doubles each
   static constexpr int buffers = 2;
 };
@@ -220,16 +221,37 @@ struct CellConstLds {
   double latent[ADM_MAX_MATERIALS], iso[ADM_MAX_MATERIALS];
 };
 
+// Cell coordinates of lane-cell j of row segment r of a launch.
+struct RowCell {
+  int cc[3];
+  bool ok;
+};
+__device__ __forceinline__ RowCell row_cell(const CellArgs &a, int r, int j,
+                                            int cpw) {
+  RowCell rc;
+  const int seg = r % a.nseg, yz = r / a.nseg;
+  const int ry = yz % a.lext[1], rz = yz / a.lext[1];
+  const int rx = seg * cpw + j;
+  rc.ok = j < cpw && rx < a.lext[0];
+  if (a.color >= 0) {
+    rc.cc[0] = 2 * rx + (a.color & 1);
+    rc.cc[1] = 2 * ry + ((a.color >> 1) & 1);
+    rc.cc[2] = 2 * rz + ((a.color >> 2) & 1);
+  } else {
+    rc.cc[0] = rx;
+    rc.cc[1] = ry;
+    rc.cc[2] = rz;
+  }
+  return rc;
+}
+
 template <int P, int NS, int PK>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
     cell_kernel(const CellArgs a, const double *__restrict__ src,
                 double *__restrict__ dst) {
   constexpr int N = P + 1, NL = N * N, N3 = N * N * N;
   constexpr int CPW = CellLanes<P>::cells_per_wave;
-  // ensemble member of this block (uniform), else the single operator.
-  // The pointers carried in the argument structs are generic; they are
-  // re-qualified so that uniform constants become scalar loads and per-cell
-  // data global (not flat) loads, which LDS waits do not have to wait for.
+  // ensemble member of this block (uniform), else the single operator
   const ADM_CONSTANT MemberArgs *mb =
       a.members ? as_constant(a.members) + blockIdx.y : nullptr;
   const ADM_CONSTANT KConst &kc = *as_constant(mb ? mb->kc : a.kc);
@@ -288,53 +310,12 @@ __global__ void __launch_bounds__(256)
                                        kc.log_01);
     if (mb) __syncthreads();
   }
-  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-  const int j = lane / NL, L = lane % NL;
-  const int la = L % N, lb = L / N;  // the line's y and z indices
-  const int64_t ci = ((int64_t)blockIdx.x * 4 + wave) * CPW + j;
-  int cc[3];
-  int64_t c;
-  bool inside;
-  if (a.color >= 0) {
-    inside = j < CPW && ci < a.n_color_cells;
-    const int64_t q0 = ci % a.ncc[0], q1 = (ci / a.ncc[0]) % a.ncc[1],
-                  q2 = ci / ((int64_t)a.ncc[0] * a.ncc[1]);
-    cc[0] = (int)(2 * q0 + (a.color & 1));
-    cc[1] = (int)(2 * q1 + ((a.color >> 1) & 1));
-    cc[2] = (int)(2 * q2 + ((a.color >> 2) & 1));
-    c = cc[0] + (int64_t)a.nc[0] * (cc[1] + (int64_t)a.nc[1] * cc[2]);
-  } else {
-    inside = j < CPW && ci < a.n_cells;
-    c = ci;
-    cc[0] = (int)(c % a.nc[0]);
-    cc[1] = (int)((c / a.nc[0]) % a.nc[1]);
-    cc[2] = (int)(c / ((int64_t)a.nc[0] * a.nc[1]));
-  }
-  if (!inside) return;  // whole cells (all lanes) leave together
-  const int64_t sy = a.nn[0], sz = (int64_t)a.nn[0] * a.nn[1];
-  // node line (y = la, z = lb) of the cell
-  const int64_t base = (int64_t)cc[0] * P + sy * ((int64_t)cc[1] * P + la) +
-                       sz * ((int64_t)cc[2] * P + lb);
-  // independent loads first: the node line and the cell's words
-  double u[N];
-#pragma unroll
-  for (int i = 0; i < N; ++i) u[i] = src[base + i];
-  const uint32_t meta = meta_p[c];
-  const int32_t slot = slot_p[c];
-  bool dirty = false;
-  if constexpr (NS > 1)
-    if (liquid_p) dirty = dirty_p[c] != 0;
-  // uniform constants (scalar loads)
-  double S[N][N], Dc[N][N], xq[N], wq[N];
+  // uniform constants (scalar loads; S and Dc are read from the LDS copy)
+  double xq[N], wq[N];
 #pragma unroll
   for (int q = 0; q < N; ++q) {
     xq[q] = kc.xq[q];
     wq[q] = kc.wq[q];
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-      S[q][k] = kc.S[q][k];
-      Dc[q][k] = kc.Dc[q][k];
-    }
   }
   const double det = kc.det, log_01 = kc.log_01;
   double ih[3], hh[3], org[3];
@@ -344,262 +325,296 @@ __global__ void __launch_bounds__(256)
     hh[d] = kc.h[d];
     org[d] = kc.origin[d];
   }
-  // lane-indexed entries come from the LDS copy
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const int j = lane / NL, L = lane % NL;
+  const int la = L % N, lb = L / N;  // the line's y and z indices
   const double wa = sk.wq[la], wb = sk.wq[lb];
   const double xqa = sk.xq[la], xqb = sk.xq[lb];
-  if (!(meta & kActive)) return;
+  const int own = N * la + NL * lb;  // lane's slot in a [z][y][x] buffer
   double *xb = lds + (PK == 2 ? n_mat * kMergedLds : 0) +
-               ((int64_t)wave * CPW + j) * (CellLanes<P>::buffers * N3);
+               (wave * CPW + (j < CPW ? j : 0)) * (CellLanes<P>::buffers * N3);
   double *xb2 = xb + N3;
-  const int mid = (meta >> kMaterialShift) & 3;
-  const double solidus = sk.solidus[mid], liquidus = sk.liquidus[mid],
-               latent = sk.latent[mid];
-  const bool iso = sk.iso[mid] != 0.0;
-  // this lane's slot in an [z][y][x] buffer of the cell
-  const int own = N * la + NL * lb;
+  const int64_t sy = a.nn[0], sz = (int64_t)a.nn[0] * a.nn[1];
+  const double *mBall = lds;
 
-  // values: S along x (registers), y and z (LDS)
-  double t[N], V[N];
+  // this wavefront's row segment
+  const int r = blockIdx.x * 4 + wave;
+  if (r >= a.n_rows) return;
+  const RowCell rc = row_cell(a, r, j, CPW);
+  if (!rc.ok) return;  // whole cells (all lanes) leave together
+  const int64_t c =
+      rc.cc[0] + (int64_t)a.nc[0] * (rc.cc[1] + (int64_t)a.nc[1] * rc.cc[2]);
+  const int64_t base = (int64_t)rc.cc[0] * P +
+                       sy * ((int64_t)rc.cc[1] * P + la) +
+                       sz * ((int64_t)rc.cc[2] * P + lb);
+  double u[N];
 #pragma unroll
-  for (int q = 0; q < N; ++q) {
-    double acc = S[q][0] * u[0];
+  for (int i = 0; i < N; ++i) u[i] = src[base + i];
+  const uint32_t meta = meta_p[c];
+  const int32_t slot = slot_p[c];
+  {
+    if (!(meta & kActive)) return;
+      const int mid = (meta >> kMaterialShift) & 3;
+      const double solidus = sk.solidus[mid], liquidus = sk.liquidus[mid],
+                   latent = sk.latent[mid];
+      const bool iso = sk.iso[mid] != 0.0;
+      // powder and liquid state of the line's points (loaded early)
+      const int64_t pslot = (meta & kHasPowder) ? (int64_t)slot : -1;
+      ADM_GLOBAL double *pw =
+          pslot >= 0 ? powder_p + pslot * N3 + own : nullptr;
+      double p_in[N];
 #pragma unroll
-    for (int k = 1; k < N; ++k) acc = __builtin_fma(S[q][k], u[k], acc);
-    t[q] = acc;
-  }
-#pragma unroll
-  for (int q = 0; q < N; ++q) xb[own + q] = t[q];
-  wave_lds_sync();
-#pragma unroll
-  for (int q = 0; q < N; ++q) {
-    double acc = sk.S[la][0] * xb[NL * lb + q];
-#pragma unroll
-    for (int k = 1; k < N; ++k)
-      acc = __builtin_fma(sk.S[la][k], xb[N * k + NL * lb + q], acc);
-    u[q] = acc;
-  }
-  wave_lds_sync();
-#pragma unroll
-  for (int q = 0; q < N; ++q) xb[own + q] = u[q];
-  wave_lds_sync();
-#pragma unroll
-  for (int q = 0; q < N; ++q) {
-    double acc = sk.S[lb][0] * xb[N * la + q];
-#pragma unroll
-    for (int k = 1; k < N; ++k)
-      acc = __builtin_fma(sk.S[lb][k], xb[N * la + NL * k + q], acc);
-    V[q] = acc;
-  }
-  // gradients by the collocation derivative
-  wave_lds_sync();
-#pragma unroll
-  for (int q = 0; q < N; ++q) xb[own + q] = V[q];
-  wave_lds_sync();
-  double gx[N], gy[N], gz[N];
-#pragma unroll
-  for (int q = 0; q < N; ++q) {
-    double ax = Dc[q][0] * V[0];
-    double ay = sk.Dc[la][0] * xb[NL * lb + q];
-    double az = sk.Dc[lb][0] * xb[N * la + q];
-#pragma unroll
-    for (int k = 1; k < N; ++k) {
-      ax = __builtin_fma(Dc[q][k], V[k], ax);
-      ay = __builtin_fma(sk.Dc[la][k], xb[N * k + NL * lb + q], ay);
-      az = __builtin_fma(sk.Dc[lb][k], xb[N * la + NL * k + q], az);
-    }
-    gx[q] = ax * ih[0];
-    gy[q] = ay * ih[1];
-    gz[q] = az * ih[2];
-  }
-
-  // per-point physics
-  const int64_t pslot = (meta & kHasPowder) ? (int64_t)slot : -1;
-  ADM_GLOBAL double *pw = pslot >= 0 ? powder_p + pslot * N3 + own : nullptr;
-  ADM_GLOBAL double *lq = nullptr;
-  if constexpr (NS > 1)
-    if (liquid_p) lq = liquid_p + c * N3 + own;
-  const bool rotated = (meta & kRotated) != 0;
-  double cs = 1.0, sn = 0.0;
-  if (rotated) {
-    cs = cos_p[c];
-    sn = sin_p[c];
-  }
-  const double *mB = lds + mid * kMergedLds;
-  const double *mC = mB + kMergedMax;
-  bool near = false;
-  if (a.sources_on) {
-    double lo[3], hi[3];
-    for (int d = 0; d < 3; ++d) {
-      lo[d] = org[d] + cc[d] * hh[d];
-      hi[d] = org[d] + (cc[d] + 1) * hh[d];
-    }
-#pragma unroll 1
-    for (int b = 0; b < a.n_beams; ++b) near |= beam_may_touch(sbeams[b], lo, hi);
-  }
-  const double py = org[1] + (cc[1] + xqa) * hh[1];
-  const double pz = org[2] + (cc[2] + xqb) * hh[2];
-  double p_in[N];
-#pragma unroll
-  for (int q = 0; q < N; ++q) p_in[q] = pw ? pw[q] : 0.0;
-  double w0[N], w1[N], w2[N], val[N];
-  bool line_liquid = false;
-#pragma unroll
-  for (int q = 0; q < N; ++q) {
-    const double T = V[q];
-    double p = p_in[q];
-    double r[NS];
-    double l = 0.0;
-    // update_state_ratios (source/ThermalOperator.templates.hh:227-317),
-    // branch-free
-    if constexpr (NS == 1) {
-      r[0] = 1.0;
-    } else {
-      const double mushy = (T - solidus) / (liquidus - solidus);
-      l = T < solidus ? 0.0 : (T > liquidus ? 1.0 : mushy);
-      if constexpr (NS == 2) {
-        r[1] = l;
-        r[0] = 1.0 - l;
-      } else {
-        const double am = 1.0 - l;
-        p = (p < am) ? p : am;
-        r[1] = l;
-        r[2] = p;
-        r[0] = 1.0 - l - p;
-      }
-    }
-    if (pw && p != p_in[q]) pw[q] = p;
-    if constexpr (NS > 1) {
-      if (lq && (l != 0.0 || dirty)) lq[q] = l;
-      line_liquid |= l != 0.0;
-    }
-    double irc, kx, ky, kzz;
-    auto props = [&](const auto &ev) {
-      // get_inv_rho_cp (source/ThermalOperator.templates.hh:415-477)
-      const double density = ev(ADM_DENSITY, r);
-      double specific_heat = ev(ADM_SPECIFIC_HEAT, r);
+      for (int q = 0; q < N; ++q) p_in[q] = pw ? pw[q] : 0.0;
+      bool dirty = false;
+      ADM_GLOBAL double *lq = nullptr;
       if constexpr (NS > 1) {
-        const double is_mushy = r[1] * r[0];
-        if (is_mushy > 0.0) specific_heat += latent;
+        if (liquid_p) {
+          dirty = dirty_p[c] != 0;
+          lq = liquid_p + c * N3 + own;
+        }
       }
-      irc = 1.0 / (density * specific_heat);
-      kx = ev(ADM_THERMAL_CONDUCTIVITY_X, r);
-      if (iso) {
-        ky = kx;
-        kzz = kx;
-      } else {
-        ky = ev(ADM_THERMAL_CONDUCTIVITY_Y, r);
-        kzz = ev(ADM_THERMAL_CONDUCTIVITY_Z, r);
+      const bool rotated = (meta & kRotated) != 0;
+      double cs = 1.0, sn = 0.0;
+      if (rotated) {
+        cs = cos_p[c];
+        sn = sin_p[c];
       }
-    };
-    if constexpr (PK == 2)
-      props(MergedEval(mB, mC, T));
-    else
-      props(typename EvalOf<PK>::type(m, mid, T));
-    double fx, fy;
-    if (rotated) {
-      fx = (kx * cs * cs + ky * sn * sn) * gx[q] + ((kx - ky) * sn * cs) * gy[q];
-      fy = ((kx - ky) * sn * cs) * gx[q] + (kx * sn * sn + ky * cs * cs) * gy[q];
-    } else {
-      // cos = 1, sin = 0: the reference's rotation gives k_x g_x, k_y g_y
-      fx = kx * gx[q];
-      fy = ky * gy[q];
-    }
-    const double fz = gz[q] * kzz;
-    // jxw as the host tabulates it: det * ((1 * w_x) * w_y) * w_z
-    double wxyz = 1.0;
-    wxyz *= wq[q];
-    wxyz *= wa;
-    wxyz *= wb;
-    const double jxw = det * wxyz;
-    w0[q] = ((-irc) * fx) * jxw * ih[0];
-    w1[q] = ((-irc) * fy) * jxw * ih[1];
-    w2[q] = ((-irc) * fz) * jxw * ih[2];
-    val[q] = 0.0;
-    if (near) {
-      const double px = org[0] + (cc[0] + xq[q]) * hh[0];
-      double qs = 0.0;
-#pragma unroll 1
-      for (int b = 0; b < a.n_beams; ++b)
-        qs += beam_value(sbeams[b], px, py, pz, log_01);
-      qs *= irc;
-      val[q] = qs * jxw;
-    }
-  }
-  if constexpr (NS > 1) {
-    // cell-wide "held liquid" flag over the cell's lanes
-    const uint64_t ball = __ballot(line_liquid);
-    const bool any_liquid = ((ball >> (j * NL)) & ((1ull << NL) - 1)) != 0;
-    if (lq && L == 0 && any_liquid != dirty) dirty_p[c] = any_liquid ? 1 : 0;
-  }
 
-  // integration: Dc^T along x (registers), y and z (LDS), the source
-  // term, then S^T along z, y, x
+      // values: S along x (registers), y and z (LDS)
+      double t[N], V[N];
 #pragma unroll
-  for (int i = 0; i < N; ++i) {
-    double acc = Dc[0][i] * w0[0];
+      for (int q = 0; q < N; ++q) {
+        double acc = sk.S[q][0] * u[0];
 #pragma unroll
-    for (int q = 1; q < N; ++q) acc = __builtin_fma(Dc[q][i], w0[q], acc);
-    t[i] = acc;
-  }
-  wave_lds_sync();
+        for (int k = 1; k < N; ++k) acc = __builtin_fma(sk.S[q][k], u[k], acc);
+        t[q] = acc;
+      }
+      wave_lds_sync();
 #pragma unroll
-  for (int q = 0; q < N; ++q) {
-    xb[own + q] = w1[q];
-    xb2[own + q] = w2[q];
-  }
-  wave_lds_sync();
+      for (int q = 0; q < N; ++q) xb[own + q] = t[q];
+      wave_lds_sync();
 #pragma unroll
-  for (int q = 0; q < N; ++q) {
-    double ay = sk.Dc[0][la] * xb[NL * lb + q];
-    double az = sk.Dc[0][lb] * xb2[N * la + q];
+      for (int q = 0; q < N; ++q) {
+        double acc = sk.S[la][0] * xb[NL * lb + q];
 #pragma unroll
-    for (int k = 1; k < N; ++k) {
-      ay = __builtin_fma(sk.Dc[k][la], xb[N * k + NL * lb + q], ay);
-      az = __builtin_fma(sk.Dc[k][lb], xb2[N * la + NL * k + q], az);
-    }
-    t[q] = t[q] + ay;
-    t[q] = t[q] + az;
-    if (a.sources_on) t[q] += val[q];
-  }
-  wave_lds_sync();
+        for (int k = 1; k < N; ++k)
+          acc = __builtin_fma(sk.S[la][k], xb[N * k + NL * lb + q], acc);
+        t[q] = acc;
+      }
+      wave_lds_sync();
 #pragma unroll
-  for (int q = 0; q < N; ++q) xb[own + q] = t[q];
-  wave_lds_sync();
+      for (int q = 0; q < N; ++q) xb[own + q] = t[q];
+      wave_lds_sync();
 #pragma unroll
-  for (int q = 0; q < N; ++q) {  // S^T along z: lane = (y = la, node z = lb)
-    double acc = sk.S[0][lb] * xb[N * la + q];
+      for (int q = 0; q < N; ++q) {
+        double acc = sk.S[lb][0] * xb[N * la + q];
 #pragma unroll
-    for (int k = 1; k < N; ++k)
-      acc = __builtin_fma(sk.S[k][lb], xb[N * la + NL * k + q], acc);
-    u[q] = acc;
-  }
-  wave_lds_sync();
+        for (int k = 1; k < N; ++k)
+          acc = __builtin_fma(sk.S[lb][k], xb[N * la + NL * k + q], acc);
+        V[q] = acc;
+      }
+      // gradients by the collocation derivative
+      wave_lds_sync();
 #pragma unroll
-  for (int q = 0; q < N; ++q) xb[own + q] = u[q];
-  wave_lds_sync();
+      for (int q = 0; q < N; ++q) xb[own + q] = V[q];
+      wave_lds_sync();
+      double gx[N], gy[N], gz[N];
 #pragma unroll
-  for (int q = 0; q < N; ++q) {  // S^T along y: lane = (node y = la, z = lb)
-    double acc = sk.S[0][la] * xb[NL * lb + q];
+      for (int q = 0; q < N; ++q) {
+        double ax = sk.Dc[q][0] * V[0];
+        double ay = sk.Dc[la][0] * xb[NL * lb + q];
+        double az = sk.Dc[lb][0] * xb[N * la + q];
 #pragma unroll
-    for (int k = 1; k < N; ++k)
-      acc = __builtin_fma(sk.S[k][la], xb[N * k + NL * lb + q], acc);
-    t[q] = acc;
-  }
-  double out[N];
+        for (int k = 1; k < N; ++k) {
+          ax = __builtin_fma(sk.Dc[q][k], V[k], ax);
+          ay = __builtin_fma(sk.Dc[la][k], xb[N * k + NL * lb + q], ay);
+          az = __builtin_fma(sk.Dc[lb][k], xb[N * la + NL * k + q], az);
+        }
+        gx[q] = ax * ih[0];
+        gy[q] = ay * ih[1];
+        gz[q] = az * ih[2];
+      }
+
+      // per-point physics
+      const double *mB = mBall + mid * kMergedLds;
+      const double *mC = mB + kMergedMax;
+      bool near = false;
+      if (a.sources_on) {
+        double lo[3], hi[3];
 #pragma unroll
-  for (int i = 0; i < N; ++i) {  // S^T along x
-    double acc = S[0][i] * t[0];
+        for (int d = 0; d < 3; ++d) {
+          lo[d] = org[d] + rc.cc[d] * hh[d];
+          hi[d] = org[d] + (rc.cc[d] + 1) * hh[d];
+        }
+#pragma unroll 1
+        for (int b = 0; b < a.n_beams; ++b)
+          near |= beam_may_touch(sbeams[b], lo, hi);
+      }
+      const double py = org[1] + (rc.cc[1] + xqa) * hh[1];
+      const double pz = org[2] + (rc.cc[2] + xqb) * hh[2];
+      double w0[N], w1[N], w2[N], val[N];
+      bool line_liquid = false;
 #pragma unroll
-    for (int q = 1; q < N; ++q) acc = __builtin_fma(S[q][i], t[q], acc);
-    out[i] = acc;
-  }
-  if (a.color >= 0) {
-    // no other cell of this launch touches these nodes
+      for (int q = 0; q < N; ++q) {
+        const double T = V[q];
+        double p = p_in[q];
+        double r_[NS];
+        double l = 0.0;
+        // update_state_ratios (source/ThermalOperator.templates.hh:227-317),
+        // branch-free
+        if constexpr (NS == 1) {
+          r_[0] = 1.0;
+        } else {
+          const double mushy = (T - solidus) / (liquidus - solidus);
+          l = T < solidus ? 0.0 : (T > liquidus ? 1.0 : mushy);
+          if constexpr (NS == 2) {
+            r_[1] = l;
+            r_[0] = 1.0 - l;
+          } else {
+            const double am = 1.0 - l;
+            p = (p < am) ? p : am;
+            r_[1] = l;
+            r_[2] = p;
+            r_[0] = 1.0 - l - p;
+          }
+        }
+        if (pw && p != p_in[q]) pw[q] = p;
+        if constexpr (NS > 1) {
+          if (lq && (l != 0.0 || dirty)) lq[q] = l;
+          line_liquid |= l != 0.0;
+        }
+        double irc, kx, ky, kzz;
+        auto props = [&](const auto &ev) {
+          // get_inv_rho_cp (source/ThermalOperator.templates.hh:415-477)
+          const double density = ev(ADM_DENSITY, r_);
+          double specific_heat = ev(ADM_SPECIFIC_HEAT, r_);
+          if constexpr (NS > 1) {
+            const double is_mushy = r_[1] * r_[0];
+            if (is_mushy > 0.0) specific_heat += latent;
+          }
+          irc = 1.0 / (density * specific_heat);
+          kx = ev(ADM_THERMAL_CONDUCTIVITY_X, r_);
+          if (iso) {
+            ky = kx;
+            kzz = kx;
+          } else {
+            ky = ev(ADM_THERMAL_CONDUCTIVITY_Y, r_);
+            kzz = ev(ADM_THERMAL_CONDUCTIVITY_Z, r_);
+          }
+        };
+        if constexpr (PK == 2)
+          props(MergedEval(mB, mC, T));
+        else
+          props(typename EvalOf<PK>::type(m, mid, T));
+        double fx, fy;
+        if (rotated) {
+          fx = (kx * cs * cs + ky * sn * sn) * gx[q] +
+               ((kx - ky) * sn * cs) * gy[q];
+          fy = ((kx - ky) * sn * cs) * gx[q] +
+               (kx * sn * sn + ky * cs * cs) * gy[q];
+        } else {
+          // cos = 1, sin = 0: the reference's rotation gives k_x g_x, k_y g_y
+          fx = kx * gx[q];
+          fy = ky * gy[q];
+        }
+        const double fz = gz[q] * kzz;
+        // jxw as the host tabulates it: det * ((1 * w_x) * w_y) * w_z
+        double wxyz = 1.0;
+        wxyz *= wq[q];
+        wxyz *= wa;
+        wxyz *= wb;
+        const double jxw = det * wxyz;
+        w0[q] = ((-irc) * fx) * jxw * ih[0];
+        w1[q] = ((-irc) * fy) * jxw * ih[1];
+        w2[q] = ((-irc) * fz) * jxw * ih[2];
+        val[q] = 0.0;
+        if (near) {
+          const double px = org[0] + (rc.cc[0] + xq[q]) * hh[0];
+          double qs = 0.0;
+#pragma unroll 1
+          for (int b = 0; b < a.n_beams; ++b)
+            qs += beam_value(sbeams[b], px, py, pz, log_01);
+          qs *= irc;
+          val[q] = qs * jxw;
+        }
+      }
+      if constexpr (NS > 1) {
+        // cell-wide "held liquid" flag over the cell's lanes
+        const uint64_t ball = __ballot(line_liquid);
+        const bool any_liquid = ((ball >> (j * NL)) & ((1ull << NL) - 1)) != 0;
+        if (lq && L == 0 && any_liquid != dirty)
+          dirty_p[c] = any_liquid ? 1 : 0;
+      }
+
+      // integration: Dc^T along x (registers), y and z (LDS), the source
+      // term, then S^T along z, y, x
 #pragma unroll
-    for (int i = 0; i < N; ++i) dst[base + i] += out[i];
-  } else {
+      for (int i = 0; i < N; ++i) {
+        double acc = sk.Dc[0][i] * w0[0];
 #pragma unroll
-    for (int i = 0; i < N; ++i) unsafeAtomicAdd(dst + base + i, out[i]);
+        for (int q = 1; q < N; ++q) acc = __builtin_fma(sk.Dc[q][i], w0[q], acc);
+        t[i] = acc;
+      }
+      wave_lds_sync();
+#pragma unroll
+      for (int q = 0; q < N; ++q) {
+        xb[own + q] = w1[q];
+        xb2[own + q] = w2[q];
+      }
+      wave_lds_sync();
+#pragma unroll
+      for (int q = 0; q < N; ++q) {
+        double ay = sk.Dc[0][la] * xb[NL * lb + q];
+        double az = sk.Dc[0][lb] * xb2[N * la + q];
+#pragma unroll
+        for (int k = 1; k < N; ++k) {
+          ay = __builtin_fma(sk.Dc[k][la], xb[N * k + NL * lb + q], ay);
+          az = __builtin_fma(sk.Dc[k][lb], xb2[N * la + NL * k + q], az);
+        }
+        t[q] = t[q] + ay;
+        t[q] = t[q] + az;
+        if (a.sources_on) t[q] += val[q];
+      }
+      wave_lds_sync();
+#pragma unroll
+      for (int q = 0; q < N; ++q) xb[own + q] = t[q];
+      wave_lds_sync();
+#pragma unroll
+      for (int q = 0; q < N; ++q) {  // S^T along z: lane = (y = la, node z = lb)
+        double acc = sk.S[0][lb] * xb[N * la + q];
+#pragma unroll
+        for (int k = 1; k < N; ++k)
+          acc = __builtin_fma(sk.S[k][lb], xb[N * la + NL * k + q], acc);
+        u[q] = acc;
+      }
+      wave_lds_sync();
+#pragma unroll
+      for (int q = 0; q < N; ++q) xb[own + q] = u[q];
+      wave_lds_sync();
+#pragma unroll
+      for (int q = 0; q < N; ++q) {  // S^T along y: lane = (node y = la, z = lb)
+        double acc = sk.S[0][la] * xb[NL * lb + q];
+#pragma unroll
+        for (int k = 1; k < N; ++k)
+          acc = __builtin_fma(sk.S[k][la], xb[N * k + NL * lb + q], acc);
+        t[q] = acc;
+      }
+      double out[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) {  // S^T along x
+        double acc = sk.S[0][i] * t[0];
+#pragma unroll
+        for (int q = 1; q < N; ++q) acc = __builtin_fma(sk.S[q][i], t[q], acc);
+        out[i] = acc;
+      }
+      if (a.color >= 0) {
+        // no other cell of this launch touches these nodes
+#pragma unroll
+        for (int i = 0; i < N; ++i) dst[base + i] += out[i];
+      } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) unsafeAtomicAdd(dst + base + i, out[i]);
+      }
   }
 }
 

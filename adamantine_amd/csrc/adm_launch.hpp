@@ -12,7 +12,7 @@ namespace adm {
 // n_members > 1 only with CellArgs/FaceArgs::members set (grid y = member).
 template <int P, int NS, int PK>
 void launch_cell(const CellArgs &a, const double *src, double *dst,
-                 int64_t n_cells, size_t lds_bytes, unsigned n_members,
+                 int64_t n_blocks, size_t lds_bytes, unsigned n_members,
                  hipStream_t s);
 
 template <int P, int NS, int PK>

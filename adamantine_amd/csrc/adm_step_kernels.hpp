@@ -12,6 +12,14 @@ __global__ void fill_kernel(double *__restrict__ x, double v, int64_t n) {
   if (i < n) x[i] = v;
 }
 
+// T[idx[k]] = v (values of newly activated nodes)
+__global__ void scatter_value_kernel(double *__restrict__ T,
+                                     const int64_t *__restrict__ idx, double v,
+                                     int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) T[idx[k]] = v;
+}
+
 // Forward Euler (source/ThermalPhysics.templates.hh:871-876):
 // T = 1*T + dt*(R*Minv); the residual buffer is cleared for the next stage.
 __global__ void fe_update_kernel(double *__restrict__ T, double *__restrict__ R,

@@ -73,6 +73,11 @@ def _bind(L):
                                         _ip, _dp, _dp, _dp, C.c_int32,
                                         C.c_double, _vp]
     L.adm_ensemble_transform.argtypes = [_vp, _vp, _dp, _vp]
+    _ip32 = C.POINTER(C.c_int32)
+    L.adm_ensemble_cells_to_activate.argtypes = [_vp, C.c_int32, C.c_int32,
+                                                 _dp, _ip32]
+    L.adm_ensemble_add_material.argtypes = [_vp, C.c_int32, _ip32, _dp, _dp,
+                                            C.c_double, _vp, _vp]
     L._ens_bound = True
     return L
 
@@ -250,6 +255,34 @@ class EnsembleDevice:
 
     def evolve_one_time_step(self, t, delta_t, solution, stream=None):
         return self.evolve(t, delta_t, 1, solution, stream=stream)
+
+    def elements_to_activate(self, member: int, boxes) -> np.ndarray:
+        """Per cell of `member`: the last deposition box meeting an inactive
+        cell, else -1 (source/material_deposition.cc:376-435)."""
+        b = np.ascontiguousarray(boxes, dtype=np.float64).reshape(-1, 6)
+        out = np.zeros(self.n_cells, dtype=np.int32)
+        check(_lib().adm_ensemble_cells_to_activate(
+            self._h, int(member), b.shape[0], b.ctypes.data_as(_dp),
+            out.ctypes.data_as(C.POINTER(C.c_int32))))
+        return out
+
+    def add_material(self, member: int, box_of_cell, box_cos, box_sin,
+                     new_material_temperature: float, solution, stream=None):
+        """Activation for one member (add_material_start/end,
+        source/ThermalPhysics.templates.hh:595-825); `solution` is the whole
+        ensemble vector."""
+        bc = np.ascontiguousarray(box_of_cell, dtype=np.int32)
+        c = np.ascontiguousarray(box_cos, dtype=np.float64)
+        s_ = np.ascontiguousarray(box_sin, dtype=np.float64)
+        if solution is not None:
+            self._check_vec(solution)
+        check(_lib().adm_ensemble_add_material(
+            self._h, int(member), bc.ctypes.data_as(C.POINTER(C.c_int32)),
+            c.ctypes.data_as(_dp), s_.ctypes.data_as(_dp),
+            float(new_material_temperature),
+            None if solution is None else _dev_ptr(solution),
+            _stream_ptr(stream)))
+        self._inverse_mass = None
 
     def qp_state(self, member: int):
         liq = np.zeros((self.n_cells, self.n_q))

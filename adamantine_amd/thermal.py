@@ -66,6 +66,9 @@ def load_library():
                                           C.c_double, C.c_double]
     L.adm_beam_state_at.argtypes = [_vp, C.c_int, C.c_double,
                                     C.POINTER(BeamState)]
+    _ip32 = C.POINTER(C.c_int32)
+    L.adm_cells_to_activate.argtypes = [_vp, C.c_int32, _dp, _ip32]
+    L.adm_add_material.argtypes = [_vp, _ip32, _dp, _dp, C.c_double, _vp, _vp]
     _lib = L
     return L
 
@@ -239,6 +242,34 @@ class ThermalPhysicsDevice:
 
     def get_state_from_material_properties(self, cell_state):
         self.operator.get_state_from_material_properties(cell_state)
+
+    def elements_to_activate(self, boxes) -> np.ndarray:
+        """get_elements_to_activate (source/material_deposition.cc:376-435):
+        per cell, the last deposition box (rows of [min x, y, z, max x, y,
+        z]) meeting an inactive cell, else -1."""
+        b = np.ascontiguousarray(boxes, dtype=np.float64).reshape(-1, 6)
+        out = np.zeros(self.operator.n_cells, dtype=np.int32)
+        check(load_library().adm_cells_to_activate(
+            self.operator.handle, b.shape[0], b.ctypes.data_as(_dp),
+            out.ctypes.data_as(C.POINTER(C.c_int32))))
+        return out
+
+    def add_material(self, box_of_cell, box_cos, box_sin,
+                     new_material_temperature: float, solution,
+                     stream=None):
+        """add_material_start + add_material_end
+        (source/ThermalPhysics.templates.hh:595-825)."""
+        bc = np.ascontiguousarray(box_of_cell, dtype=np.int32)
+        c = _host(box_cos)
+        s_ = _host(box_sin)
+        check(load_library().adm_add_material(
+            self.operator.handle, bc.ctypes.data_as(C.POINTER(C.c_int32)),
+            c.ctypes.data_as(_dp), s_.ctypes.data_as(_dp),
+            float(new_material_temperature),
+            None if solution is None else _dev_ptr(solution),
+            _stream_ptr(stream)))
+        # compute_inverse_mass_matrix (source/ThermalPhysics.templates.hh:741)
+        self.operator._inverse_mass = None
 
     def set_state_to_material_properties(self):
         return self.operator.set_state_to_material_properties()

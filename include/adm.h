@@ -282,6 +282,29 @@ int adm_set_beam_parameters(void *ctx, int beam, double absorption_efficiency,
 int adm_beam_state_at(void *ctx, int beam, double t, adm_beam_state *out);
 
 /* ------------------------------------------------------------------------ */
+/* Material deposition (cell activation)                                     */
+/* ------------------------------------------------------------------------ */
+
+/* get_elements_to_activate (source/material_deposition.cc:376-435): for every
+ * inactive cell, the index of the last deposition box (input order) whose
+ * closed box meets the cell's bounding box, or -1.
+ * boxes: host, n_boxes x {min x, min y, min z, max x, max y, max z};
+ * box_of_cell: host out, n_cells. */
+int adm_cells_to_activate(void *ctx, int32_t n_boxes, const double *boxes,
+                          int32_t *box_of_cell);
+
+/* add_material_start + add_material_end (source/ThermalPhysics.templates.hh:
+ * 595-825): cells with box_of_cell >= 0 become active with that box's
+ * deposition cos/sin (host, per box); the phase state of every cell goes to
+ * its cell mean and back to every quadrature point, as the reference's
+ * state transfer does; nodes that no previously active cell held take
+ * new_temperature in T (device, n_dofs; may be NULL), the others keep their
+ * value; the lumped mass used by adm_evolve is recomputed. */
+int adm_add_material(void *ctx, const int32_t *box_of_cell,
+                     const double *box_cos, const double *box_sin,
+                     double new_temperature, double *T, void *stream);
+
+/* ------------------------------------------------------------------------ */
 /* Ensemble                                                                  */
 /* ------------------------------------------------------------------------ */
 
@@ -345,6 +368,17 @@ int adm_ensemble_evolve(void *ens, int method, double t, double dt,
  * [n_cells][n_q]; either pointer may be NULL). */
 int adm_ensemble_get_qp_state(void *ens, int32_t member, double *liquid,
                               double *powder);
+
+/* Material deposition for one member (see adm_cells_to_activate and
+ * adm_add_material; T is the whole ensemble vector, or NULL). Replaces the
+ * per-member activation of run_ensemble (application/adamantine.hh:
+ * 2070-2115). */
+int adm_ensemble_cells_to_activate(void *ens, int32_t member, int32_t n_boxes,
+                                   const double *boxes, int32_t *box_of_cell);
+int adm_ensemble_add_material(void *ens, int32_t member,
+                              const int32_t *box_of_cell,
+                              const double *box_cos, const double *box_sin,
+                              double new_temperature, double *T, void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Data assimilation (EnKF)                                                  */

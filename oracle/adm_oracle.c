@@ -404,6 +404,38 @@ static int ipow(int b, int e) {
   return r;
 }
 
+/* lumped mass with QGaussLobatto(p+1) over the active cells
+ * (ThermalOperator.templates.hh:118-164), inverted; shape functions at their
+ * own support points are the identity */
+static void compute_inverse_mass(oracle_t *o) {
+  double det = 1.0;
+  for (int d = 0; d < o->dim; ++d) det *= o->h[d];
+  memset(o->inv_mass, 0, sizeof(double) * o->n_dofs);
+  for (int64_t c = 0; c < o->n_cells; ++c) {
+    if (!o->active[c]) continue;
+    int64_t cc[3] = {0, 0, 0}, r = c;
+    for (int d = 0; d < o->dim; ++d) {
+      cc[d] = r % o->nc[d];
+      r /= o->nc[d];
+    }
+    for (int i = 0; i < o->nq; ++i) {
+      int64_t node = 0, stride = 1;
+      int ri = i;
+      double w = 1.0;
+      for (int d = 0; d < o->dim; ++d) {
+        int li = ri % o->n1;
+        ri /= o->n1;
+        node += (cc[d] * o->p + li) * stride;
+        stride *= o->nn[d];
+        w *= o->wgl[li];
+      }
+      o->inv_mass[node] += det * w;
+    }
+  }
+  for (int64_t i = 0; i < o->n_dofs; ++i)
+    o->inv_mass[i] = o->inv_mass[i] > 0.0 ? 1.0 / o->inv_mass[i] : 0.0;
+}
+
 oracle_t *oracle_create(const adm_problem_desc *desc) {
   oracle_t *o = (oracle_t *)calloc(1, sizeof(oracle_t));
   const adm_mesh_desc *m = &desc->mesh;
@@ -505,32 +537,8 @@ oracle_t *oracle_create(const adm_problem_desc *desc) {
       (double *)calloc(o->n_cells * 2 * o->dim * o->nfq, sizeof(double));
   if (desc->cell_state) oracle_set_cell_state(o, desc->cell_state);
 
-  /* lumped mass with QGaussLobatto(p+1) (ThermalOperator.templates.hh:
-   * 118-164); shape functions at their own support points are the identity */
   o->inv_mass = (double *)calloc(o->n_dofs, sizeof(double));
-  for (int64_t c = 0; c < o->n_cells; ++c) {
-    if (!o->active[c]) continue;
-    int64_t cc[3] = {0, 0, 0}, r = c;
-    for (int d = 0; d < o->dim; ++d) {
-      cc[d] = r % o->nc[d];
-      r /= o->nc[d];
-    }
-    for (int i = 0; i < o->nq; ++i) {
-      int64_t node = 0, stride = 1;
-      int ri = i;
-      double w = 1.0;
-      for (int d = 0; d < o->dim; ++d) {
-        int li = ri % o->n1;
-        ri /= o->n1;
-        node += (cc[d] * o->p + li) * stride;
-        stride *= o->nn[d];
-        w *= o->wgl[li];
-      }
-      o->inv_mass[node] += det * w;
-    }
-  }
-  for (int64_t i = 0; i < o->n_dofs; ++i)
-    o->inv_mass[i] = o->inv_mass[i] > 0.0 ? 1.0 / o->inv_mass[i] : 0.0;
+  compute_inverse_mass(o);
   return o;
 }
 
@@ -945,4 +953,85 @@ double oracle_evolve(oracle_t *o, int method, double t, double dt, int n_steps,
   free(f);
   free(Y);
   return t;
+}
+
+/* Nodes of active cells (1) and the rest (0). */
+static uint8_t *node_activity(const oracle_t *o) {
+  uint8_t *na = (uint8_t *)calloc(o->n_dofs, 1);
+  for (int64_t c = 0; c < o->n_cells; ++c) {
+    if (!o->active[c]) continue;
+    int64_t cc[3] = {0, 0, 0}, r = c;
+    for (int d = 0; d < o->dim; ++d) {
+      cc[d] = r % o->nc[d];
+      r /= o->nc[d];
+    }
+    for (int i = 0; i < o->nq; ++i) {
+      int64_t node = 0, stride = 1;
+      int ri = i;
+      for (int d = 0; d < o->dim; ++d) {
+        int li = ri % o->n1;
+        ri /= o->n1;
+        node += (cc[d] * o->p + li) * stride;
+        stride *= o->nn[d];
+      }
+      na[node] = 1;
+    }
+  }
+  return na;
+}
+
+/* get_elements_to_activate (source/material_deposition.cc:376-435): for every
+ * inactive cell, the last box (in input order) whose closed box intersects
+ * the cell's bounding box, or -1. boxes[b] = {min x, y, z, max x, y, z}. */
+void oracle_cells_to_activate(const oracle_t *o, int n_boxes,
+                              const double *boxes, int32_t *box_of_cell) {
+  for (int64_t c = 0; c < o->n_cells; ++c) {
+    box_of_cell[c] = -1;
+    if (o->active[c]) continue;
+    int64_t cc[3] = {0, 0, 0}, r = c;
+    for (int d = 0; d < o->dim; ++d) {
+      cc[d] = r % o->nc[d];
+      r /= o->nc[d];
+    }
+    for (int b = 0; b < n_boxes; ++b) {
+      int hit = 1;
+      for (int d = 0; d < o->dim; ++d) {
+        const int ax = o->axis[d];
+        const double lo = o->org[d] + cc[d] * o->h[d];
+        const double hi = o->org[d] + (cc[d] + 1) * o->h[d];
+        if (!(lo <= boxes[6 * b + 3 + ax] && boxes[6 * b + ax] <= hi)) hit = 0;
+      }
+      if (hit) box_of_cell[c] = b;
+    }
+  }
+}
+
+/* add_material_start + add_material_end (source/ThermalPhysics.templates.hh:
+ * 595-825): the phase state goes to cell means and back to every quadrature
+ * point (set/get_state_to/from_material_properties), the selected cells
+ * become active with their box's deposition direction, nodes that no
+ * previously active cell held take new_temperature, the others keep their
+ * value, and the lumped mass is recomputed. */
+void oracle_add_material(oracle_t *o, const int32_t *box_of_cell,
+                         const double *box_cos, const double *box_sin,
+                         double new_temperature, double *T) {
+  const int ns = o->mat.n_states;
+  double *cs = (double *)malloc(sizeof(double) * o->n_cells * ns);
+  oracle_get_cell_state(o, cs);
+  uint8_t *before = node_activity(o);
+  for (int64_t c = 0; c < o->n_cells; ++c) {
+    const int b = box_of_cell[c];
+    if (b < 0 || o->active[c]) continue;
+    o->active[c] = 1;
+    o->dcos[c] = box_cos[b];
+    o->dsin[c] = box_sin[b];
+  }
+  uint8_t *after = node_activity(o);
+  for (int64_t i = 0; i < o->n_dofs; ++i)
+    if (after[i] && !before[i]) T[i] = new_temperature;
+  oracle_set_cell_state(o, cs);
+  compute_inverse_mass(o);
+  free(after);
+  free(before);
+  free(cs);
 }

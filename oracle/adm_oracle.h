@@ -72,6 +72,14 @@ void oracle_basis_1d(const oracle_t *o, double *support, double *qpoints,
                      double *qweights, double *shape, double *coll_deriv,
                      double *gl_weights);
 
+/* Material activation (source/material_deposition.cc:376-435,
+ * source/ThermalPhysics.templates.hh:595-825). */
+void oracle_cells_to_activate(const oracle_t *o, int n_boxes,
+                              const double *boxes, int32_t *box_of_cell);
+void oracle_add_material(oracle_t *o, const int32_t *box_of_cell,
+                         const double *box_cos, const double *box_sin,
+                         double new_temperature, double *T);
+
 /* Ensemble draws (adm_oracle_da.c): get_normal_random_vector of
  * source/ensemble_management.cc:18-52 and the raw default-seeded MT19937. */
 void oracle_normal_random_vector(uint32_t length, uint32_t n_rejected_draws,

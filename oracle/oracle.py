@@ -75,6 +75,10 @@ def lib():
                                                  C.c_int, C.c_int, C.c_int,
                                                  C.c_double]
         L.oracle_basis_1d.argtypes = [C.c_void_p] + [_dp] * 6
+        _ip32 = C.POINTER(C.c_int32)
+        L.oracle_cells_to_activate.argtypes = [C.c_void_p, C.c_int, _dp, _ip32]
+        L.oracle_add_material.argtypes = [C.c_void_p, _ip32, _dp, _dp,
+                                          C.c_double, _dp]
         _lib = L
     return _lib
 
@@ -170,6 +174,23 @@ class Oracle:
     def set_beam_parameters(self, beam, absorption, max_power, radius, depth):
         lib().oracle_set_beam_parameters(self._h, beam, absorption, max_power,
                                          radius, depth)
+
+    def cells_to_activate(self, boxes):
+        b = np.ascontiguousarray(boxes, dtype=np.float64).reshape(-1, 6)
+        out = np.zeros(self.n_cells, dtype=np.int32)
+        lib().oracle_cells_to_activate(self._h, b.shape[0], _ptr(b),
+                                       out.ctypes.data_as(C.POINTER(C.c_int32)))
+        return out
+
+    def add_material(self, box_of_cell, box_cos, box_sin, new_temperature, T):
+        """Activation; T (host float64) is updated in place."""
+        bc = np.ascontiguousarray(box_of_cell, dtype=np.int32)
+        c = np.ascontiguousarray(box_cos, dtype=np.float64)
+        s_ = np.ascontiguousarray(box_sin, dtype=np.float64)
+        assert T.dtype == np.float64 and T.flags.c_contiguous
+        lib().oracle_add_material(self._h, bc.ctypes.data_as(C.POINTER(C.c_int32)),
+                                  _ptr(c), _ptr(s_), float(new_temperature),
+                                  _ptr(T))
 
     def basis_1d(self):
         n = self.problem.fe_degree + 1

@@ -1,0 +1,100 @@
+"""Material activation on the device against the oracle
+(source/material_deposition.cc:376-435, source/ThermalPhysics.templates.hh:
+595-825): the same cells, bit-identical lumped mass and phase state after
+the transfer, new nodes at the new-material temperature, and explicit steps
+before and after the activation within 1e-10."""
+import numpy as np
+import pytest
+
+from adamantine_amd.problem import FORWARD_EULER
+from adamantine_amd.workloads import plain_large
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+STEP_TOL = 1e-10
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def layered_problem():
+    p = plain_large(n_cells=(12, 8, 8), method=FORWARD_EULER)
+    hz = p.extent[2] / p.n_cells[2]
+    p.set_material_height(5 * hz)      # 5 of 8 layers active
+    p.set_powder_layer(5 * hz, 2 * hz)
+    p.beams[0].segments = [(s[0], s[1], (s[2][0], s[2][1], 5 * hz))
+                           for s in p.beams[0].segments]
+    return p, hz
+
+
+def test_activation_matches_oracle():
+    import torch
+    from adamantine_amd.thermal import ThermalPhysicsDevice
+    p, hz = layered_problem()
+    print("activation: 12x8x8 Q2, FE", flush=True)
+    ph = ThermalPhysicsDevice(p)
+    ph.setup()
+    o = Oracle(p)
+    T0 = np.full(p.n_dofs, 300.0)
+    dt, n = 1e-6, 15
+    sol = torch.tensor(T0, device="cuda")
+    t = ph.evolve(0.0, dt, n, sol)
+    Tref, t_ref = o.evolve(FORWARD_EULER, 0.0, dt, n, T0.copy())
+    torch.cuda.synchronize()
+    assert rel(sol.cpu().numpy(), Tref) < STEP_TOL
+    # deposit layer 5 (z in [5 hz, 6 hz]); a box touching it from inside
+    boxes = np.array([[0.0, 0.0, 5.2 * hz, p.extent[0], p.extent[1], 5.8 * hz]])
+    b = ph.elements_to_activate(boxes)
+    np.testing.assert_array_equal(b, o.cells_to_activate(boxes))
+    assert (b >= 0).sum() == p.n_cells[0] * p.n_cells[1]
+    ph.add_material(b, [np.cos(0.3)], [np.sin(0.3)], 350.0, sol)
+    o.add_material(b, [np.cos(0.3)], [np.sin(0.3)], 350.0, Tref)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(sol.cpu().numpy(), Tref)
+    np.testing.assert_array_equal(
+        ph.operator.get_inverse_mass_matrix().cpu().numpy(), o.inverse_mass())
+    lq, pw = ph.operator.qp_state()
+    olq, opw = o.qp_state()
+    np.testing.assert_array_equal(pw, opw)
+    np.testing.assert_array_equal(lq, olq)
+    t2 = ph.evolve(t, dt, n, sol)
+    Tref, _ = o.evolve(FORWARD_EULER, t_ref, dt, n, Tref)
+    torch.cuda.synchronize()
+    assert rel(sol.cpu().numpy(), Tref) < STEP_TOL
+    assert t2 > t
+
+
+def test_ensemble_activation_per_member():
+    """Members deposit the same layer at different steps (the per-member
+    activation delays of the deposition workload)."""
+    import torch
+    from adamantine_amd.ensemble import EnsembleDevice, Member, member_problem
+    p, hz = layered_problem()
+    members = [Member(absorption_efficiency=[0.3 + 0.02 * j],
+                      max_power=[400.0]) for j in range(3)]
+    ens = EnsembleDevice(p, members, method=FORWARD_EULER)
+    ens.compute_inverse_mass_matrix()
+    oracles = [Oracle(member_problem(p, m)) for m in members]
+    sol = ens.initialize_dof_vector(300.0)
+    Tref = [np.full(p.n_dofs, 300.0) for _ in members]
+    boxes = np.array([[0.0, 0.0, 5.2 * hz, p.extent[0], p.extent[1], 5.8 * hz]])
+    dt, t = 1e-6, 0.0
+    for step in range(12):
+        for j in range(3):
+            if step == 4 * j:  # member j deposits at step 4 j
+                b = ens.elements_to_activate(j, boxes)
+                np.testing.assert_array_equal(b, oracles[j].cells_to_activate(boxes))
+                ens.add_material(j, b, [1.0], [0.0], 320.0, sol)
+                oracles[j].add_material(b, [1.0], [0.0], 320.0, Tref[j])
+        t_new = ens.evolve(t, dt, 1, sol)
+        for j in range(3):
+            Tref[j], _ = oracles[j].evolve(FORWARD_EULER, t, dt, 1, Tref[j])
+        t = t_new
+    torch.cuda.synchronize()
+    got = sol.cpu().numpy()
+    for j in range(3):
+        assert rel(got[j], Tref[j]) < STEP_TOL, j
+    np.testing.assert_array_equal(
+        ens.get_inverse_mass_matrix().cpu().numpy()[2], oracles[2].inverse_mass())
