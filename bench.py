@@ -23,6 +23,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes of one operator application of this workload (8 color launches
+# of the cell kernel + the face kernel), from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs scaled by a 256 MiB copy (profiles/hbm_traffic_plain_large.md)
+MEASURED_TRAFFIC_BYTES = 290.5e6
 
 
 def algorithmic_bytes(problem, n_powder_qp: int, n_active_cells: int) -> int:
@@ -248,7 +252,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved_gbs,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS,
-                     "traffic": None,
+                     "traffic": MEASURED_TRAFFIC_BYTES,
                      "bytes_per_launch": bytes_per_vmult},
         "rk4_step_ms": rk4_ms,
         "wall_s": wall,
