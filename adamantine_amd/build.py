@@ -57,6 +57,10 @@ def build_library(verbose: bool = True, jobs: int = 0) -> str:
     jobs_list.append(([hipcc] + HIPCC_FLAGS + inc + ["-c", "-o", da_obj,
                       os.path.join(CSRC, "adm_da.hip")], verbose))
     objs.append(da_obj)
+    adj_obj = os.path.join(objdir, "adm_adjoint.o")
+    jobs_list.append(([hipcc] + HIPCC_FLAGS + inc + ["-c", "-o", adj_obj,
+                      os.path.join(CSRC, "adm_adjoint.hip")], verbose))
+    objs.append(adj_obj)
     for (p, ns, t) in CELL_CONFIGS:
         obj = os.path.join(objdir, "adm_cell_p%d_s%d_t%d.o" % (p, ns, t))
         jobs_list.append(([hipcc] + HIPCC_FLAGS + inc + [

@@ -1,0 +1,974 @@
+// Discrete adjoint of the explicit time steppers and parameter gradients.
+//
+// The reference has no adjoint. For a final-time misfit
+//   J = 1/2 sum_k w_k (T_final[dof_k] - y_k)^2
+// this computes dJ/dT0 and dJ/dtheta for theta = (absorption efficiency of
+// beam 0, a factor on every thermal conductivity at its current value,
+// radius of beam 0) by the exact transposition of the forward Euler or
+// classic RK4 stage sequence (the forward run is the one of
+// ThermalPhysics::evolve_one_time_step, source/ThermalPhysics.templates.hh:
+// 854-888), with the trajectory recomputed from checkpoints.
+//
+// Linearisation of the operator A(T) of source/ThermalOperator.templates.hh:
+// 479-751 at a stage input: per quadrature point, with gradient g, test
+// field mu (mu = M^-1 times the stage adjoint) and its gradient gm,
+//   flux coefficient    c   = -(1/rho c_p) K gm JxW            (as w of A)
+//   value coefficient   c_T = -JxW [d(1/rho c_p) gm.K g + (1/rho c_p) gm.K' g]
+//                             + mu d(1/rho c_p) Q JxW
+// integrated like A's fluxes and source. d/dT of the mixtures includes the
+// liquid ratio (mushy range) and the powder ratio min(p_old, 1 - liquid) of
+// that evaluation. The powder carried from one evaluation to the next is
+// held at its recorded value (its dependence on earlier temperatures is not
+// propagated); the latent-heat switch and the source cut-offs are
+// piecewise constant and contribute nothing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/adm.h"
+#include "adm_context.hpp"
+#include "adm_kernels.hpp"
+
+namespace adm {
+
+#define ADM_HIP_ADJ(call)                                                 \
+  do {                                                                    \
+    hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess)                                                 \
+      throw AdmError(ADM_ERR_HIP, std::string(#call) + ": " +             \
+                                      hipGetErrorString(e_));             \
+  } while (0)
+
+struct AdjArgs {
+  CellArgs c;                 // mesh, constants, beams, row mapping
+  const double *T;            // stage input
+  const double *mu;           // M^-1 * stage adjoint
+  double *out;                // += (dA/dT)^T mu
+  const double *powder_old;   // powder before the evaluation [slots][n_q]
+  double *partials;           // [3]: d/d eta_0, d/d k-factor, d/d R_0
+  double inv_eta0;            // 1 / absorption efficiency of beam 0 (or 0)
+};
+
+// value and T-derivative of a mixed property from the merged tables
+template <int NS>
+__device__ __forceinline__ void merged_vd(const double *B, const double *C,
+                                          double T, int prop,
+                                          const double (&r)[NS],
+                                          const double (&dr)[NS], double &v,
+                                          double &dv) {
+  int k = 0;
+#pragma unroll
+  for (int step = kMergedMax / 2; step >= 1; step >>= 1)
+    k += (B[k + step - 1] <= T) ? step : 0;
+  const double d = T - B[k > 0 ? k - 1 : 0];
+  const double *c = C + ((prop * kMergedMax + k) * ADM_MAX_STATES) * 2;
+  v = 0.0;
+  dv = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const double x = __builtin_fma(d, c[2 * s + 1], c[2 * s]);
+    v += r[s] * x;
+    dv += dr[s] * x + r[s] * c[2 * s + 1];
+  }
+}
+
+// value and T-derivative of a mixed polynomial property
+template <int NS>
+__device__ __forceinline__ void poly_vd(const MatConst &m, int mid, double T,
+                                        int prop, const double (&r)[NS],
+                                        const double (&dr)[NS], double &v,
+                                        double &dv) {
+  v = 0.0;
+  dv = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    double x = 0.0, dx = 0.0, tp = 1.0, tpm = 0.0;
+#pragma unroll
+    for (int i = 0; i < ADM_MAX_POLY; ++i) {
+      if (i > m.p_order) break;
+      const double ci = m.poly[mid][prop][s][i];
+      x += ci * tp;
+      dx += i * ci * tpm;
+      tpm = tp;
+      tp *= T;
+    }
+    v += r[s] * x;
+    dv += dr[s] * x + r[s] * dx;
+  }
+}
+
+// ratios (update_state_ratios) and their T-derivative at one point
+template <int NS>
+__device__ __forceinline__ void ratios_d(double T, double p_old, double sol,
+                                         double liq, double (&r)[NS],
+                                         double (&dr)[NS]) {
+  if constexpr (NS == 1) {
+    r[0] = 1.0;
+    dr[0] = 0.0;
+  } else {
+    const double mushy = (T - sol) / (liq - sol);
+    const double l = T < sol ? 0.0 : (T > liq ? 1.0 : mushy);
+    const double dl = (T < sol || T > liq) ? 0.0 : 1.0 / (liq - sol);
+    if constexpr (NS == 2) {
+      r[1] = l;
+      r[0] = 1.0 - l;
+      dr[1] = dl;
+      dr[0] = -dl;
+    } else {
+      const double am = 1.0 - l;
+      const bool keep = p_old < am;
+      const double p = keep ? p_old : am;
+      const double dp = keep ? 0.0 : -dl;
+      r[1] = l;
+      r[2] = p;
+      r[0] = 1.0 - l - p;
+      dr[1] = dl;
+      dr[2] = dp;
+      dr[0] = -dl - dp;
+    }
+  }
+}
+
+template <int NS, int PK>
+__device__ __forceinline__ void prop_vd(const MatConst &m, int mid,
+                                        const double *mB, const double *mC,
+                                        double T, int prop,
+                                        const double (&r)[NS],
+                                        const double (&dr)[NS], double &v,
+                                        double &dv) {
+  if constexpr (PK == 2)
+    merged_vd<NS>(mB, mC, T, prop, r, dr, v, dv);
+  else
+    poly_vd<NS>(m, mid, T, prop, r, dr, v, dv);
+}
+
+// d Q_0 / d R_0 at a point for beam 0 (Goldak or electron beam; the cut-offs
+// are held fixed)
+__device__ __forceinline__ double beam_value_dR(const adm_beam_state &s,
+                                                double q, double x, double y,
+                                                double log_01) {
+  if (s.type == ADM_SOURCE_CUBE || q == 0.0) return 0.0;
+  const double dx = x - s.center[0], dy = y - s.center[1];
+  const double r2 = dx * dx + dy * dy;
+  const double R = sqrt(s.radius_squared);
+  const double R3 = R * s.radius_squared;
+  if (s.type == ADM_SOURCE_GOLDAK) return q * (-2.0 / R + 6.0 * r2 / R3);
+  return q * (-2.0 / R - 2.0 * log_01 * r2 / R3);
+}
+
+// (dA/dT)^T mu over the cells: same lane layout as cell_kernel (one x-line
+// per lane, (p+1)^2 lanes per cell, one row segment per wavefront), two
+// fields (T, mu) evaluated, atomic accumulation, parameter partials reduced
+// per wavefront.
+template <int P, int NS, int PK>
+__global__ void __launch_bounds__(256)
+    adjoint_cell_kernel(const AdjArgs aa) {
+  constexpr int N = P + 1, NL = N * N, N3 = N * N * N;
+  constexpr int CPW = CellLanes<P>::cells_per_wave;
+  const CellArgs &a = aa.c;
+  const KConst &kc = *(const KConst *)as_global(a.kc);
+  const MatConst &m = kc.mat;
+  extern __shared__ double lds[];
+  __shared__ adm_beam_state sbeams[ADM_MAX_BEAMS];
+  const int n_mat = m.n_materials;
+  {
+    constexpr int words = sizeof(adm_beam_state) * ADM_MAX_BEAMS / 8;
+    const int tid = threadIdx.x;
+    if (tid < words)
+      reinterpret_cast<uint64_t *>(sbeams)[tid] =
+          reinterpret_cast<const uint64_t *>(a.beams)[tid];
+    if constexpr (PK == 2) {
+      const int total = n_mat * kMergedLds;
+      for (int i = tid; i < total; i += blockDim.x) {
+        const int mm = i / kMergedLds, k = i % kMergedLds;
+        lds[i] = (&m.merged[mm].B[0])[k];
+      }
+    }
+    __syncthreads();
+  }
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const int j = lane / NL, L = lane % NL;
+  const int la = L % N, lb = L / N;
+  const int own = N * la + NL * lb;
+  double *xb = lds + (PK == 2 ? n_mat * kMergedLds : 0) +
+               (wave * CPW + (j < CPW ? j : 0)) * (4 * N3);
+  double *xb2 = xb + N3, *xb3 = xb + 2 * N3, *xb4 = xb + 3 * N3;
+  const int r = blockIdx.x * 4 + wave;
+  double pe = 0.0, ps = 0.0, pr = 0.0;
+  bool ok = r < a.n_rows;
+  RowCell rc;
+  if (ok) {
+    rc = row_cell(a, r, j, CPW);
+    ok = rc.ok;
+  }
+  int64_t c = 0;
+  uint32_t meta = 0;
+  if (ok) {
+    c = rc.cc[0] + (int64_t)a.nc[0] * (rc.cc[1] + (int64_t)a.nc[1] * rc.cc[2]);
+    meta = a.meta[c];
+    ok = (meta & kActive) != 0;
+  }
+  if (ok) {
+    const int64_t sy = a.nn[0], sz = (int64_t)a.nn[0] * a.nn[1];
+    const int64_t base = (int64_t)rc.cc[0] * P + sy * ((int64_t)rc.cc[1] * P + la) +
+                         sz * ((int64_t)rc.cc[2] * P + lb);
+    const int mid = (meta >> kMaterialShift) & 3;
+    // two fields: values at the points by S along x, y, z
+    double V[2][N];
+    {
+      double u0[N], u1[N], t0[N], t1[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        u0[i] = aa.T[base + i];
+        u1[i] = aa.mu[base + i];
+      }
+#pragma unroll
+      for (int q = 0; q < N; ++q) {
+        double s0 = kc.S[q][0] * u0[0], s1 = kc.S[q][0] * u1[0];
+#pragma unroll
+        for (int k = 1; k < N; ++k) {
+          s0 = __builtin_fma(kc.S[q][k], u0[k], s0);
+          s1 = __builtin_fma(kc.S[q][k], u1[k], s1);
+        }
+        t0[q] = s0;
+        t1[q] = s1;
+      }
+#pragma unroll
+      for (int q = 0; q < N; ++q) {
+        xb[own + q] = t0[q];
+        xb2[own + q] = t1[q];
+      }
+      wave_lds_sync();
+#pragma unroll
+      for (int q = 0; q < N; ++q) {
+        double s0 = kc.S[la][0] * xb[NL * lb + q];
+        double s1 = kc.S[la][0] * xb2[NL * lb + q];
+#pragma unroll
+        for (int k = 1; k < N; ++k) {
+          s0 = __builtin_fma(kc.S[la][k], xb[N * k + NL * lb + q], s0);
+          s1 = __builtin_fma(kc.S[la][k], xb2[N * k + NL * lb + q], s1);
+        }
+        t0[q] = s0;
+        t1[q] = s1;
+      }
+      wave_lds_sync();
+#pragma unroll
+      for (int q = 0; q < N; ++q) {
+        xb[own + q] = t0[q];
+        xb2[own + q] = t1[q];
+      }
+      wave_lds_sync();
+#pragma unroll
+      for (int q = 0; q < N; ++q) {
+        double s0 = kc.S[lb][0] * xb[N * la + q];
+        double s1 = kc.S[lb][0] * xb2[N * la + q];
+#pragma unroll
+        for (int k = 1; k < N; ++k) {
+          s0 = __builtin_fma(kc.S[lb][k], xb[N * la + NL * k + q], s0);
+          s1 = __builtin_fma(kc.S[lb][k], xb2[N * la + NL * k + q], s1);
+        }
+        V[0][q] = s0;
+        V[1][q] = s1;
+      }
+    }
+    // physical gradients of both fields
+    wave_lds_sync();
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      xb[own + q] = V[0][q];
+      xb2[own + q] = V[1][q];
+    }
+    wave_lds_sync();
+    double G[2][3][N];
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+      const double *buf = f == 0 ? xb : xb2;
+#pragma unroll
+      for (int q = 0; q < N; ++q) {
+        double ax = kc.Dc[q][0] * V[f][0];
+        double ay = kc.Dc[la][0] * buf[NL * lb + q];
+        double az = kc.Dc[lb][0] * buf[N * la + q];
+#pragma unroll
+        for (int k = 1; k < N; ++k) {
+          ax = __builtin_fma(kc.Dc[q][k], V[f][k], ax);
+          ay = __builtin_fma(kc.Dc[la][k], buf[N * k + NL * lb + q], ay);
+          az = __builtin_fma(kc.Dc[lb][k], buf[N * la + NL * k + q], az);
+        }
+        G[f][0][q] = ax * kc.inv_h[0];
+        G[f][1][q] = ay * kc.inv_h[1];
+        G[f][2][q] = az * kc.inv_h[2];
+      }
+    }
+    // per-point coefficients
+    const double sol = m.prop[mid][ADM_SOLIDUS], liq = m.prop[mid][ADM_LIQUIDUS];
+    const double latent = m.latent_factor[mid];
+    const int64_t slot =
+        (meta & kHasPowder) ? (int64_t)a.powder_slot[c] : -1;
+    const double *pw = slot >= 0 ? aa.powder_old + slot * N3 + own : nullptr;
+    const bool rotated = (meta & kRotated) != 0;
+    double cs = 1.0, sn = 0.0;
+    if (rotated) {
+      cs = a.dep_cos[c];
+      sn = a.dep_sin[c];
+    }
+    const double *mB = lds + mid * kMergedLds;
+    const double *mC = mB + kMergedMax;
+    const double py = kc.origin[1] + (rc.cc[1] + kc.xq[la]) * kc.h[1];
+    const double pz = kc.origin[2] + (rc.cc[2] + kc.xq[lb]) * kc.h[2];
+    double cT[N], cw[3][N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      const double T = V[0][q], mu = V[1][q];
+      double rr[NS], dr[NS];
+      ratios_d<NS>(T, pw ? pw[q] : 0.0, sol, liq, rr, dr);
+      double rho, drho, cp, dcp, kx, dkx, ky, dky, kz, dkz;
+      prop_vd<NS, PK>(m, mid, mB, mC, T, ADM_DENSITY, rr, dr, rho, drho);
+      prop_vd<NS, PK>(m, mid, mB, mC, T, ADM_SPECIFIC_HEAT, rr, dr, cp, dcp);
+      if constexpr (NS > 1)
+        if (rr[1] * rr[0] > 0.0) cp += latent;
+      prop_vd<NS, PK>(m, mid, mB, mC, T, ADM_THERMAL_CONDUCTIVITY_X, rr, dr,
+                      kx, dkx);
+      prop_vd<NS, PK>(m, mid, mB, mC, T, ADM_THERMAL_CONDUCTIVITY_Y, rr, dr,
+                      ky, dky);
+      prop_vd<NS, PK>(m, mid, mB, mC, T, ADM_THERMAL_CONDUCTIVITY_Z, rr, dr,
+                      kz, dkz);
+      const double irc = 1.0 / (rho * cp);
+      const double dirc = -irc * irc * (drho * cp + rho * dcp);
+      // K and K' in x, y (rotation by the deposition angle), z
+      const double kxx = kx * cs * cs + ky * sn * sn;
+      const double kyy = kx * sn * sn + ky * cs * cs;
+      const double kxy = (kx - ky) * sn * cs;
+      const double dkxx = dkx * cs * cs + dky * sn * sn;
+      const double dkyy = dkx * sn * sn + dky * cs * cs;
+      const double dkxy = (dkx - dky) * sn * cs;
+      const double g0 = G[0][0][q], g1 = G[0][1][q], g2 = G[0][2][q];
+      const double m0 = G[1][0][q], m1 = G[1][1][q], m2 = G[1][2][q];
+      const double Kg0 = kxx * g0 + kxy * g1, Kg1 = kxy * g0 + kyy * g1,
+                   Kg2 = kz * g2;
+      const double dKg0 = dkxx * g0 + dkxy * g1, dKg1 = dkxy * g0 + dkyy * g1,
+                   dKg2 = dkz * g2;
+      const double Km0 = kxx * m0 + kxy * m1, Km1 = kxy * m0 + kyy * m1,
+                   Km2 = kz * m2;
+      double wxyz = 1.0;
+      wxyz *= kc.wq[q];
+      wxyz *= kc.wq[la];
+      wxyz *= kc.wq[lb];
+      const double jxw = kc.det * wxyz;
+      const double mKg = m0 * Kg0 + m1 * Kg1 + m2 * Kg2;
+      const double mdKg = m0 * dKg0 + m1 * dKg1 + m2 * dKg2;
+      cw[0][q] = ((-irc) * Km0) * jxw * kc.inv_h[0];
+      cw[1][q] = ((-irc) * Km1) * jxw * kc.inv_h[1];
+      cw[2][q] = ((-irc) * Km2) * jxw * kc.inv_h[2];
+      double c_T = -jxw * (dirc * mKg + irc * mdKg);
+      ps += -irc * jxw * mKg;
+      if (a.sources_on) {
+        const double px = kc.origin[0] + (rc.cc[0] + kc.xq[q]) * kc.h[0];
+        double qs = 0.0, q0 = 0.0;
+#pragma unroll 1
+        for (int b = 0; b < a.n_beams; ++b) {
+          const double v = beam_value(sbeams[b], px, py, pz, kc.log_01);
+          if (b == 0) q0 = v;
+          qs += v;
+        }
+        c_T += mu * dirc * qs * jxw;
+        pe += mu * irc * q0 * aa.inv_eta0 * jxw;
+        pr += mu * irc * beam_value_dR(sbeams[0], q0, px, py, kc.log_01) * jxw;
+      }
+      cT[q] = c_T;
+    }
+    // integration as in cell_kernel
+    double t[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      double acc = kc.Dc[0][i] * cw[0][0];
+#pragma unroll
+      for (int q = 1; q < N; ++q) acc = __builtin_fma(kc.Dc[q][i], cw[0][q], acc);
+      t[i] = acc;
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      xb3[own + q] = cw[1][q];
+      xb4[own + q] = cw[2][q];
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      double ay = kc.Dc[0][la] * xb3[NL * lb + q];
+      double az = kc.Dc[0][lb] * xb4[N * la + q];
+#pragma unroll
+      for (int k = 1; k < N; ++k) {
+        ay = __builtin_fma(kc.Dc[k][la], xb3[N * k + NL * lb + q], ay);
+        az = __builtin_fma(kc.Dc[k][lb], xb4[N * la + NL * k + q], az);
+      }
+      t[q] = t[q] + ay + az + cT[q];
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int q = 0; q < N; ++q) xb3[own + q] = t[q];
+    wave_lds_sync();
+    double u[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      double acc = kc.S[0][lb] * xb3[N * la + q];
+#pragma unroll
+      for (int k = 1; k < N; ++k)
+        acc = __builtin_fma(kc.S[k][lb], xb3[N * la + NL * k + q], acc);
+      u[q] = acc;
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int q = 0; q < N; ++q) xb3[own + q] = u[q];
+    wave_lds_sync();
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      double acc = kc.S[0][la] * xb3[NL * lb + q];
+#pragma unroll
+      for (int k = 1; k < N; ++k)
+        acc = __builtin_fma(kc.S[k][la], xb3[N * k + NL * lb + q], acc);
+      t[q] = acc;
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      double acc = kc.S[0][i] * t[0];
+#pragma unroll
+      for (int q = 1; q < N; ++q) acc = __builtin_fma(kc.S[q][i], t[q], acc);
+      unsafeAtomicAdd(aa.out + base + i, acc);
+    }
+  }
+  // parameter partials: wavefront sum, one atomic per wavefront
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    pe += __shfl_xor(pe, off);
+    ps += __shfl_xor(ps, off);
+    pr += __shfl_xor(pr, off);
+  }
+  if (lane == 0 && (pe != 0.0 || ps != 0.0 || pr != 0.0)) {
+    unsafeAtomicAdd(aa.partials + 0, pe);
+    unsafeAtomicAdd(aa.partials + 1, ps);
+    unsafeAtomicAdd(aa.partials + 2, pr);
+  }
+}
+
+// (dA_face/dT)^T mu over the boundary faces (one lane per face, as
+// face_kernel), with the face powder recorded before the evaluation.
+template <int P, int NS, int PK>
+__global__ void __launch_bounds__(256)
+    adjoint_face_kernel(const FaceArgs a, const double *__restrict__ T,
+                        const double *__restrict__ mu,
+                        const double *__restrict__ face_powder_old,
+                        double *__restrict__ out) {
+  constexpr int N = P + 1, NF = N * N;
+  const int64_t fi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (fi >= a.n_faces) return;
+  const KConst &kc = *(const KConst *)as_global(a.kc);
+  const MatConst &m = kc.mat;
+  const int64_t c = a.faces[fi].cell;
+  const int info = a.faces[fi].info;
+  const int f = info & 7, bt = (info >> 3) & 31, mid = (info >> 8) & 3;
+  const int d = f / 2, side = f % 2;
+  const int cc[3] = {(int)(c % a.nc[0]), (int)((c / a.nc[0]) % a.nc[1]),
+                     (int)(c / ((int64_t)a.nc[0] * a.nc[1]))};
+  const int64_t ns[3] = {1, (int64_t)a.nn[0],
+                         (int64_t)a.nn[0] * (int64_t)a.nn[1]};
+  const int64_t base = (int64_t)cc[0] * P + ns[1] * cc[1] * P +
+                       ns[2] * cc[2] * P + (side ? P : 0) * ns[d];
+  const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+  double fu[NF], fm[NF], ft[NF], fv[NF], gv[NF];
+#pragma unroll
+  for (int jj = 0; jj < NF; ++jj) {
+    const int64_t node = base + (jj % N) * ns[t1] + (jj / N) * ns[t2];
+    fu[jj] = T[node];
+    fm[jj] = mu[node];
+  }
+  sweep<N, 0, false, false, 2>(kc.S, fu, ft);
+  sweep<N, 1, false, false, 2>(kc.S, ft, fv);
+  sweep<N, 0, false, false, 2>(kc.S, fm, ft);
+  sweep<N, 1, false, false, 2>(kc.S, ft, gv);
+  const double sol = m.prop[mid][ADM_SOLIDUS], liq = m.prop[mid][ADM_LIQUIDUS];
+  const double latent = m.latent_factor[mid];
+  const double *mB = &m.merged[mid].B[0];
+  const double *mC = &m.merged[mid].coef[0][0][0][0];
+  const double *fp = NS == 3 ? face_powder_old + fi * NF : nullptr;
+#pragma unroll
+  for (int q = 0; q < NF; ++q) {
+    const double Tq = fv[q];
+    double rr[NS], dr[NS];
+    ratios_d<NS>(Tq, fp ? fp[q] : 0.0, sol, liq, rr, dr);
+    double rho, drho, cp, dcp;
+    prop_vd<NS, PK>(m, mid, mB, mC, Tq, ADM_DENSITY, rr, dr, rho, drho);
+    prop_vd<NS, PK>(m, mid, mB, mC, Tq, ADM_SPECIFIC_HEAT, rr, dr, cp, dcp);
+    if constexpr (NS > 1)
+      if (rr[1] * rr[0] > 0.0) cp += latent;
+    const double irc = 1.0 / (rho * cp);
+    const double dirc = -irc * irc * (drho * cp + rho * dcp);
+    double s1 = 0.0, ds1 = 0.0;
+    if (bt & ADM_BC_CONVECTIVE) {
+      const double cT = m.prop[mid][ADM_CONVECTION_TEMPERATURE_INFTY];
+      double h, dh;
+      prop_vd<NS, PK>(m, mid, mB, mC, Tq, ADM_CONVECTION_HEAT_TRANSFER_COEF,
+                      rr, dr, h, dh);
+      s1 += h * (Tq - cT);
+      ds1 += dh * (Tq - cT) + h;
+    }
+    if (bt & ADM_BC_RADIATIVE) {
+      const double rT = m.prop[mid][ADM_RADIATION_TEMPERATURE_INFTY];
+      double e, de;
+      prop_vd<NS, PK>(m, mid, mB, mC, Tq, ADM_EMISSIVITY, rr, dr, e, de);
+      const double a1 = Tq + rT, a2 = Tq * Tq + rT * rT;
+      const double rh = e * kStefanBoltzmann * a1 * a2;
+      const double drh = kStefanBoltzmann * (de * a1 * a2 + e * a2 +
+                                             e * a1 * 2.0 * Tq);
+      s1 += rh * (Tq - rT);
+      ds1 += drh * (Tq - rT) + rh;
+    }
+    const double dval = -dirc * s1 - irc * ds1;
+    fm[q] = gv[q] * dval * kc.fjxw[d][q];
+  }
+  sweep<N, 1, true, false, 2>(kc.S, fm, ft);
+  sweep<N, 0, true, false, 2>(kc.S, ft, fv);
+#pragma unroll
+  for (int jj = 0; jj < NF; ++jj) {
+    const int64_t node = base + (jj % N) * ns[t1] + (jj / N) * ns[t2];
+    unsafeAtomicAdd(out + node, fv[jj]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// vector kernels
+// ---------------------------------------------------------------------------
+
+// y = x (+ a * z)
+__global__ void axpy_copy_kernel(double *__restrict__ y,
+                                 const double *__restrict__ x, double a,
+                                 const double *__restrict__ z, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  y[i] = z ? 1. * x[i] + a * z[i] : x[i];
+}
+
+// k = R * inv_mass; R = 0
+__global__ void stage_value_kernel(double *__restrict__ k,
+                                   double *__restrict__ R,
+                                   const double *__restrict__ inv_mass,
+                                   int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  k[i] = R[i] * inv_mass[i];
+  R[i] = 0.0;
+}
+
+// y = 1 * y + a * x
+__global__ void axpy_kernel(double *__restrict__ y,
+                            const double *__restrict__ x, double a,
+                            int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  y[i] = 1. * y[i] + a * x[i];
+}
+
+// y = a * x
+__global__ void scale_kernel(double *__restrict__ y,
+                             const double *__restrict__ x, double a,
+                             int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  y[i] = a * x[i];
+}
+
+// mu = a * inv_mass * v
+__global__ void scaled_mass_kernel(double *__restrict__ mu,
+                                   const double *__restrict__ v,
+                                   const double *__restrict__ inv_mass,
+                                   double a, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  mu[i] = a * v[i] * inv_mass[i];
+}
+
+// ---------------------------------------------------------------------------
+// driver
+// ---------------------------------------------------------------------------
+
+struct DevBuf {
+  double *p = nullptr;
+  size_t n = 0;
+  void alloc(size_t n_) {
+    if (n_ == 0) return;
+    ADM_HIP_ADJ(hipMalloc(&p, sizeof(double) * n_));
+    n = n_;
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+// beam states of the context at time t (segment search from the start)
+inline void eval_time_reset(Context &c, double t) {
+  for (size_t b = 0; b < c.old_segment.size(); ++b) c.old_segment[b] = 0;
+  c.set_time(t);
+}
+
+// out += (dA/dT)^T mu at the stage input Y with the phase state (pw, fpw)
+// recorded before the evaluation, at the context's current time; the
+// parameter partials are added to partials[3].
+inline void launch_vjp(Context &c, const double *Y, const double *pw,
+                     const double *fpw, const double *mu, double *out,
+                     double *partials, double inv_eta0, hipStream_t s) {
+  const int pk = c.property_kind();
+  if (pk == 1)
+    throw AdmError(ADM_ERR_UNSUPPORTED,
+                   "the adjoint supports polynomial and merged-table "
+                   "properties");
+  AdjArgs aa;
+  std::memset(&aa, 0, sizeof(aa));
+  aa.c = c.cell_args();
+  aa.c.color = -1;
+  const int cpw = c.p == 1 ? CellLanes<1>::cells_per_wave
+                           : CellLanes<2>::cells_per_wave;
+  for (int d = 0; d < 3; ++d) aa.c.lext[d] = c.nc[d];
+  aa.c.nseg = (c.nc[0] + cpw - 1) / cpw;
+  aa.c.n_rows = aa.c.nseg * c.nc[1] * c.nc[2];
+  aa.T = Y;
+  aa.mu = mu;
+  aa.out = out;
+  aa.powder_old = pw;
+  aa.partials = partials;
+  aa.inv_eta0 = inv_eta0;
+  const unsigned blocks = (unsigned)((aa.c.n_rows + 3) / 4);
+  const size_t lds =
+      (pk == 2 ? sizeof(double) * kMergedLds * c.kc.mat.n_materials : 0) +
+      sizeof(double) * 4 * (c.p == 1 ? CellLanes<1>::cells_per_wave
+                                     : CellLanes<2>::cells_per_wave) *
+          4 * (c.p + 1) * (c.p + 1) * (c.p + 1);
+  FaceArgs fa = c.face_args();
+  const unsigned face_blocks = (unsigned)((fa.n_faces + 255) / 256);
+#define ADM_ADJ(P_, NS_, PK_)                                              \
+  if (c.p == P_ && c.n_states == NS_ && pk == PK_) {                       \
+    hipLaunchKernelGGL((adjoint_cell_kernel<P_, NS_, PK_>), dim3(blocks),  \
+                       dim3(256), lds, s, aa);                              \
+    if (fa.n_faces)                                                        \
+      hipLaunchKernelGGL((adjoint_face_kernel<P_, NS_, PK_>),              \
+                         dim3(face_blocks), dim3(256), 0, s, fa, Y, mu, fpw, \
+                         out);                                              \
+    launched = true;                                                       \
+  }
+  bool launched = false;
+  ADM_ADJ(1, 1, 0)
+  ADM_ADJ(1, 2, 0)
+  ADM_ADJ(1, 3, 0)
+  ADM_ADJ(2, 1, 0)
+  ADM_ADJ(2, 2, 0)
+  ADM_ADJ(2, 3, 0)
+  ADM_ADJ(1, 1, 2)
+  ADM_ADJ(1, 2, 2)
+  ADM_ADJ(1, 3, 2)
+  ADM_ADJ(2, 1, 2)
+  ADM_ADJ(2, 2, 2)
+  ADM_ADJ(2, 3, 2)
+#undef ADM_ADJ
+  if (!launched)
+    throw AdmError(ADM_ERR_UNSUPPORTED,
+                   "no adjoint kernel for this degree / state count");
+  ADM_HIP_ADJ(hipGetLastError());
+}
+
+struct AdjointRun {
+  Context &c;
+  hipStream_t s;
+  int method;
+  int64_t n;
+  int64_t npw, nfpw;  // powder values of cells and faces
+  int n_stages;
+  double a_next[4], b[4], cst[4];
+  DevBuf partials;
+  DevBuf K[4], Y, R;
+
+  AdjointRun(Context &c_, int method_, hipStream_t s_)
+      : c(c_), s(s_), method(method_) {
+    n = c.n_dofs;
+    npw = c.n_powder_cells * c.nq;
+    nfpw = (c.n_states == 3) ? (int64_t)c.faces.size() * c.nfq : 0;
+    if (method == ADM_FORWARD_EULER) {
+      n_stages = 1;
+      b[0] = 1.0;
+      cst[0] = 0.0;
+    } else if (method == ADM_RK_FOURTH_ORDER) {
+      n_stages = 4;
+      const double bb[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+      const double cc[4] = {0.0, 0.5, 0.5, 1.0};
+      const double an[4] = {0.5, 0.5, 1.0, 0.0};
+      for (int i = 0; i < 4; ++i) {
+        b[i] = bb[i];
+        cst[i] = cc[i];
+        a_next[i] = an[i];
+      }
+    } else {
+      throw AdmError(ADM_ERR_UNSUPPORTED,
+                     "the adjoint supports forward Euler and RK4");
+    }
+    for (int i = 0; i < n_stages; ++i) K[i].alloc(n);
+    Y.alloc(n);
+    R.alloc(n);
+    ADM_HIP_ADJ(hipMemsetAsync(R.p, 0, sizeof(double) * n, s));
+    partials.alloc(3);
+  }
+
+  unsigned grid(int64_t m) const { return (unsigned)((m + 255) / 256); }
+
+  void eval_time(double t) { eval_time_reset(c, t); }
+
+  // one forward step T -> T' (stage inputs Y_i, stage values K_i); with
+  // record != nullptr, stage i's input and the powder before its
+  // evaluation go to record[i]
+  struct StageRecord {
+    double *Y, *pw, *fpw;
+  };
+  void forward_step(double t, double dt, double *T, const StageRecord *rec) {
+    for (int i = 0; i < n_stages; ++i) {
+      // stage input: Y_i = T + dt a_{i,i-1} K_{i-1} (zero a_ij of RK4 skipped)
+      const double *Yi = T;
+      if (i > 0) {
+        hipLaunchKernelGGL(axpy_copy_kernel, dim3(grid(n)), dim3(256), 0, s,
+                           Y.p, T, dt * a_next[i - 1], K[i - 1].p, n);
+        Yi = Y.p;
+      }
+      if (rec) {
+        ADM_HIP_ADJ(hipMemcpyAsync(rec[i].Y, Yi, sizeof(double) * n,
+                                   hipMemcpyDeviceToDevice, s));
+        if (npw)
+          ADM_HIP_ADJ(hipMemcpyAsync(rec[i].pw, c.d_powder,
+                                     sizeof(double) * npw,
+                                     hipMemcpyDeviceToDevice, s));
+        if (nfpw)
+          ADM_HIP_ADJ(hipMemcpyAsync(rec[i].fpw, c.d_face_powder,
+                                     sizeof(double) * nfpw,
+                                     hipMemcpyDeviceToDevice, s));
+      }
+      eval_time(t + cst[i] * dt);
+      c.apply(Yi, R.p, 0, s);
+      hipLaunchKernelGGL(stage_value_kernel, dim3(grid(n)), dim3(256), 0, s,
+                         K[i].p, R.p, c.sb.inv_mass, n);
+    }
+    // T' = T + dt sum b_i K_i, in stage order
+    for (int i = 0; i < n_stages; ++i)
+      hipLaunchKernelGGL(axpy_kernel, dim3(grid(n)), dim3(256), 0, s, T,
+                         K[i].p, dt * b[i], n);
+    ADM_HIP_ADJ(hipGetLastError());
+  }
+
+  void vjp(const StageRecord &rec, double t_eval, const double *mu,
+           double *out, double inv_eta0) {
+    eval_time(t_eval);
+    launch_vjp(c, rec.Y, rec.pw, rec.fpw, mu, out, partials.p, inv_eta0, s);
+  }
+};
+
+}  // namespace adm
+
+using adm::AdmError;
+
+extern "C" {
+
+int adm_adjoint_gradient(void *ptr, int method, double t0, double dt,
+                         int n_steps, int checkpoint_every, const double *T0,
+                         int32_t n_obs, const int64_t *obs_dof,
+                         const double *obs, const double *obs_weight,
+                         double *T_final, double *J, double *grad,
+                         double *lambda0, void *stream) {
+  try {
+    if (!ptr) return adm::set_error(ADM_ERR_STATE, "null context");
+    adm::Context &c = *static_cast<adm::Context *>(ptr);
+    if (!T0 || !obs_dof || !obs || !J || !grad || n_obs < 1 || n_steps < 1)
+      return adm::set_error(ADM_ERR_INVALID, "bad adjoint arguments");
+    for (int k = 0; k < n_obs; ++k)
+      if (obs_dof[k] < 0 || obs_dof[k] >= c.n_dofs)
+        return adm::set_error(ADM_ERR_INVALID, "observation DoF out of range");
+    const int K = checkpoint_every > 0 ? checkpoint_every : n_steps;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    c.ensure_step_buffers(ADM_FORWARD_EULER);
+    adm::AdjointRun run(c, method, s);
+    const int64_t n = run.n, npw = run.npw, nfpw = run.nfpw;
+    const int n_seg = (n_steps + K - 1) / K;
+    // checkpoints: T, cell powder, face powder at the start of every segment
+    adm::DevBuf cpT, cpP, cpF, T, Tb, mu, Ybar;
+    cpT.alloc((size_t)n * n_seg);
+    cpP.alloc((size_t)npw * n_seg);
+    cpF.alloc((size_t)nfpw * n_seg);
+    T.alloc(n);
+    Tb.alloc(n);
+    mu.alloc(n);
+    Ybar.alloc(n);
+    // per-segment recording
+    const int S = run.n_stages;
+    adm::DevBuf recY, recP, recF;
+    recY.alloc((size_t)n * K * S);
+    recP.alloc((size_t)npw * K * S);
+    recF.alloc((size_t)nfpw * K * S);
+    std::vector<adm::AdjointRun::StageRecord> rec((size_t)K * S);
+    for (int64_t k = 0; k < (int64_t)K * S; ++k) {
+      rec[k].Y = recY.p + k * n;
+      rec[k].pw = npw ? recP.p + k * npw : nullptr;
+      rec[k].fpw = nfpw ? recF.p + k * nfpw : nullptr;
+    }
+    // forward sweep
+    ADM_HIP_ADJ(hipMemcpyAsync(T.p, T0, sizeof(double) * n,
+                               hipMemcpyDeviceToDevice, s));
+    std::vector<double> times((size_t)n_steps + 1);
+    double t = t0;
+    for (int k = 0; k < n_steps; ++k) {
+      times[k] = t;
+      if (k % K == 0) {
+        const int sg = k / K;
+        ADM_HIP_ADJ(hipMemcpyAsync(cpT.p + (int64_t)sg * n, T.p,
+                                   sizeof(double) * n, hipMemcpyDeviceToDevice,
+                                   s));
+        if (npw)
+          ADM_HIP_ADJ(hipMemcpyAsync(cpP.p + (int64_t)sg * npw, c.d_powder,
+                                     sizeof(double) * npw,
+                                     hipMemcpyDeviceToDevice, s));
+        if (nfpw)
+          ADM_HIP_ADJ(hipMemcpyAsync(cpF.p + (int64_t)sg * nfpw,
+                                     c.d_face_powder, sizeof(double) * nfpw,
+                                     hipMemcpyDeviceToDevice, s));
+      }
+      run.forward_step(t, dt, T.p, nullptr);
+      t = t + dt;
+    }
+    times[n_steps] = t;
+    // phase state at the final time, restored after the backward sweep
+    const int64_t nlq = c.d_liquid ? c.n_cells * c.nq : 0;
+    const int64_t ndirty = c.d_liquid_dirty ? (c.n_cells + 7) / 8 : 0;
+    adm::DevBuf finP, finF, finL, finD;
+    finP.alloc(npw);
+    finF.alloc(nfpw);
+    finL.alloc(nlq);
+    finD.alloc(ndirty);
+    auto copy_state = [&](bool save) {
+      auto cp = [&](void *dst_save, void *live, size_t bytes) {
+        if (!bytes) return;
+        ADM_HIP_ADJ(hipMemcpyAsync(save ? dst_save : live,
+                                   save ? live : dst_save, bytes,
+                                   hipMemcpyDeviceToDevice, s));
+      };
+      cp(finP.p, c.d_powder, sizeof(double) * npw);
+      cp(finF.p, c.d_face_powder, sizeof(double) * nfpw);
+      cp(finL.p, c.d_liquid, sizeof(double) * nlq);
+      cp(finD.p, c.d_liquid_dirty, ndirty ? (size_t)c.n_cells : 0);
+    };
+    copy_state(true);
+    if (T_final)
+      ADM_HIP_ADJ(hipMemcpyAsync(T_final, T.p, sizeof(double) * n,
+                                 hipMemcpyDeviceToDevice, s));
+    // misfit and its gradient at the final time
+    std::vector<double> Tf((size_t)n);
+    ADM_HIP_ADJ(hipMemcpyAsync(Tf.data(), T.p, sizeof(double) * n,
+                               hipMemcpyDeviceToHost, s));
+    ADM_HIP_ADJ(hipStreamSynchronize(s));
+    std::vector<double> tb((size_t)n, 0.0);
+    double Jv = 0.0;
+    for (int k = 0; k < n_obs; ++k) {
+      const double w = obs_weight ? obs_weight[k] : 1.0;
+      const double r = Tf[obs_dof[k]] - obs[k];
+      Jv += 0.5 * w * r * r;
+      tb[obs_dof[k]] += w * r;
+    }
+    *J = Jv;
+    ADM_HIP_ADJ(hipMemcpyAsync(Tb.p, tb.data(), sizeof(double) * n,
+                               hipMemcpyHostToDevice, s));
+    ADM_HIP_ADJ(hipMemsetAsync(run.partials.p, 0, sizeof(double) * 3, s));
+    const double eta0 =
+        c.beams.empty() ? 0.0 : c.beams[0].desc.absorption_efficiency;
+    const double inv_eta0 = eta0 != 0.0 ? 1.0 / eta0 : 0.0;
+    const unsigned gn = run.grid(n);
+    // backward sweep over the segments
+    for (int sg = n_seg - 1; sg >= 0; --sg) {
+      const int k0 = sg * K, k1 = std::min(n_steps, k0 + K);
+      ADM_HIP_ADJ(hipMemcpyAsync(T.p, cpT.p + (int64_t)sg * n,
+                                 sizeof(double) * n, hipMemcpyDeviceToDevice,
+                                 s));
+      if (npw)
+        ADM_HIP_ADJ(hipMemcpyAsync(c.d_powder, cpP.p + (int64_t)sg * npw,
+                                   sizeof(double) * npw,
+                                   hipMemcpyDeviceToDevice, s));
+      if (nfpw)
+        ADM_HIP_ADJ(hipMemcpyAsync(c.d_face_powder, cpF.p + (int64_t)sg * nfpw,
+                                   sizeof(double) * nfpw,
+                                   hipMemcpyDeviceToDevice, s));
+      for (int k = k0; k < k1; ++k)
+        run.forward_step(times[k], dt, T.p, &rec[(size_t)(k - k0) * S]);
+      for (int k = k1 - 1; k >= k0; --k) {
+        const adm::AdjointRun::StageRecord *rk = &rec[(size_t)(k - k0) * S];
+        // stage adjoints kb_i start at dt b_i Tb; Tb_new = Tb + sum Ybar_i;
+        // kb_{i-1} += dt a_{i,i-1} Ybar_i (reverse stage order)
+        for (int i = 0; i < S; ++i)
+          hipLaunchKernelGGL(adm::scale_kernel, dim3(gn), dim3(256), 0, s,
+                             run.K[i].p, Tb.p, dt * run.b[i], n);
+        for (int i = S - 1; i >= 0; --i) {
+          hipLaunchKernelGGL(adm::scaled_mass_kernel, dim3(gn), dim3(256), 0,
+                             s, mu.p, run.K[i].p, c.sb.inv_mass, 1.0, n);
+          ADM_HIP_ADJ(hipMemsetAsync(Ybar.p, 0, sizeof(double) * n, s));
+          run.vjp(rk[i], times[k] + run.cst[i] * dt, mu.p, Ybar.p, inv_eta0);
+          hipLaunchKernelGGL(adm::axpy_kernel, dim3(gn), dim3(256), 0, s, Tb.p,
+                             Ybar.p, 1.0, n);
+          if (i > 0)
+            hipLaunchKernelGGL(adm::axpy_kernel, dim3(gn), dim3(256), 0, s,
+                               run.K[i - 1].p, Ybar.p, dt * run.a_next[i - 1],
+                               n);
+        }
+      }
+    }
+    copy_state(false);
+    adm::eval_time_reset(c, times[n_steps]);
+    ADM_HIP_ADJ(hipGetLastError());
+    double pp[3];
+    ADM_HIP_ADJ(hipMemcpyAsync(pp, run.partials.p, sizeof(pp),
+                               hipMemcpyDeviceToHost, s));
+    if (lambda0)
+      ADM_HIP_ADJ(hipMemcpyAsync(lambda0, Tb.p, sizeof(double) * n,
+                                 hipMemcpyDeviceToDevice, s));
+    ADM_HIP_ADJ(hipStreamSynchronize(s));
+    for (int k = 0; k < 3; ++k) grad[k] = pp[k];
+  } catch (const AdmError &e) {
+    return adm::set_error(e.code, e.what());
+  } catch (const std::exception &e) {
+    return adm::set_error(ADM_ERR_INVALID, e.what());
+  }
+  return ADM_OK;
+}
+
+int adm_operator_vjp(void *ptr, double t, const double *T, const double *mu,
+                     double *out, double *grad, void *stream) {
+  try {
+    if (!ptr) return adm::set_error(ADM_ERR_STATE, "null context");
+    adm::Context &c = *static_cast<adm::Context *>(ptr);
+    if (!T || !mu || !out)
+      return adm::set_error(ADM_ERR_INVALID, "null vector");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    adm::DevBuf partials;
+    partials.alloc(3);
+    ADM_HIP_ADJ(hipMemsetAsync(partials.p, 0, sizeof(double) * 3, s));
+    ADM_HIP_ADJ(hipMemsetAsync(out, 0, sizeof(double) * c.n_dofs, s));
+    adm::eval_time_reset(c, t);
+    const double eta0 =
+        c.beams.empty() ? 0.0 : c.beams[0].desc.absorption_efficiency;
+    adm::launch_vjp(c, T, c.d_powder, c.d_face_powder, mu, out, partials.p,
+                    eta0 != 0.0 ? 1.0 / eta0 : 0.0, s);
+    double pp[3];
+    ADM_HIP_ADJ(hipMemcpyAsync(pp, partials.p, sizeof(pp),
+                               hipMemcpyDeviceToHost, s));
+    ADM_HIP_ADJ(hipStreamSynchronize(s));
+    if (grad)
+      for (int k = 0; k < 3; ++k) grad[k] = pp[k];
+  } catch (const AdmError &e) {
+    return adm::set_error(e.code, e.what());
+  } catch (const std::exception &e) {
+    return adm::set_error(ADM_ERR_INVALID, e.what());
+  }
+  return ADM_OK;
+}
+
+}  // extern "C"

@@ -69,6 +69,10 @@ def load_library():
     _ip32 = C.POINTER(C.c_int32)
     L.adm_cells_to_activate.argtypes = [_vp, C.c_int32, _dp, _ip32]
     L.adm_add_material.argtypes = [_vp, _ip32, _dp, _dp, C.c_double, _vp, _vp]
+    L.adm_adjoint_gradient.argtypes = [
+        _vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, _vp, C.c_int32,
+        C.POINTER(C.c_int64), _dp, _dp, _vp, _dp, _dp, _vp, _vp]
+    L.adm_operator_vjp.argtypes = [_vp, C.c_double, _vp, _vp, _vp, _dp, _vp]
     _lib = L
     return L
 

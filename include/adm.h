@@ -436,6 +436,42 @@ int adm_ensemble_analysis(void *ens, double *T, int32_t n_params,
 int adm_ensemble_transform(void *ens, double *T, const double *Tm,
                            void *stream);
 
+/* ------------------------------------------------------------------------ */
+/* Discrete adjoint                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* No reference counterpart (the reference estimates parameters only through
+ * the EnKF). Runs n_steps of `method` (ADM_FORWARD_EULER or
+ * ADM_RK_FOURTH_ORDER) from T0 at t0, exactly as adm_evolve does, and the
+ * transposed stage sequence backwards from checkpoints taken every
+ * checkpoint_every steps (<= 0: one checkpoint), for the misfit
+ *   J = 1/2 sum_k w_k (T(t_end)[obs_dof_k] - obs_k)^2.
+ *   T0, T_final, lambda0  device, n_dofs (T_final, lambda0 may be NULL)
+ *   obs_dof, obs, obs_weight  host, n_obs (obs_weight NULL: all ones)
+ *   J    host out: the misfit
+ *   grad host out[3]: dJ/d(absorption efficiency of beam 0),
+ *        dJ/ds for every thermal conductivity scaled by s at s = 1,
+ *        dJ/d(radius of beam 0)
+ *   lambda0 device out: dJ/dT0.
+ * Powder ratios carried between operator evaluations are held at their
+ * recorded values in the linearisation; at return the context's phase state
+ * is the one at the final time, as after adm_evolve. Polynomial or tabulated
+ * properties with the merged-table form (not the general-table evaluator). */
+int adm_adjoint_gradient(void *ctx, int method, double t0, double dt,
+                         int n_steps, int checkpoint_every, const double *T0,
+                         int32_t n_obs, const int64_t *obs_dof,
+                         const double *obs, const double *obs_weight,
+                         double *T_final, double *J, double *grad,
+                         double *lambda0, void *stream);
+
+/* The linearised operator by itself (no reference counterpart): with the
+ * beams at time t and the context's current phase state, out (device,
+ * overwritten) = (dA/dT)^T mu at T, A being adm_vmult's operator; grad
+ * (host out[3], may be NULL) = mu . dA/dtheta for the parameters of
+ * adm_adjoint_gradient. Reads the phase state without updating it. */
+int adm_operator_vjp(void *ctx, double t, const double *T, const double *mu,
+                     double *out, double *grad, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

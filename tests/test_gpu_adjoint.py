@@ -1,0 +1,237 @@
+"""Discrete adjoint (no reference counterpart) against finite differences.
+
+The linearised operator is checked by directional derivatives of the device
+operator itself (fresh operators, so every evaluation starts from the same
+phase state); the time-stepping gradients by central differences of the
+misfit computed with the CPU oracle's forward run, which is pinned to the
+reference elsewhere. Central differences are exact for the linear
+dependence on the absorption efficiency and second-order accurate for the
+others; points that straddle a kink of the piecewise-linear tables within
+the step are rare at the step sizes used."""
+import numpy as np
+import pytest
+
+from adamantine_amd.problem import (BC_ADIABATIC, BC_CONVECTIVE, BC_RADIATIVE,
+                                    FORWARD_EULER, RK_FOURTH_ORDER,
+                                    SOURCE_ELECTRON_BEAM, SOURCE_GOLDAK, Beam,
+                                    Material, Problem, segments_from_path)
+from adamantine_amd.workloads import MM, UM, slp_tables
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def small_problem(kind="q2_slp", k_scale=1.0, source=SOURCE_GOLDAK,
+                  powder=True, method=RK_FOURTH_ORDER, rotate=True):
+    """10x6x5 Q2 (or Q1) box of 1 x 0.6 x 0.5 mm with a beam crossing the
+    top along x, radiative + convective top and convective bottom."""
+    ext = (1.0 * MM, 0.6 * MM, 0.5 * MM)
+    if kind == "q2_slp":
+        m = slp_tables(k_scale=k_scale)
+        deg, nc = 2, (10, 6, 5)
+    else:
+        m = Material(n_materials=1, n_states=1, use_table=False, p_order=2,
+                     single_material_layout=True)
+        m.set_poly(0, "density", "solid", [7904.0])
+        m.set_poly(0, "specific_heat", "solid", [714.0, 0.05])
+        for k in ("thermal_conductivity_x", "thermal_conductivity_y",
+                  "thermal_conductivity_z"):
+            m.set_poly(0, k, "solid", [11.8 * k_scale, 0.0138 * k_scale,
+                                       1e-6 * k_scale])
+        m.set_poly(0, "emissivity", "solid", [0.3])
+        m.set_poly(0, "convection_heat_transfer_coef", "solid", [10.0])
+        m.set_property(0, "radiation_temperature_infty", 300.0)
+        m.set_property(0, "convection_temperature_infty", 300.0)
+        deg, nc = 1, (12, 8, 6)
+    path = segments_from_path([
+        {"mode": "point", "xyz": (0.3 * MM, 0.3 * MM, ext[2]), "pmod": 0.0,
+         "param": 1e-9},
+        {"mode": "line", "xyz": (0.8 * MM, 0.3 * MM, ext[2]), "pmod": 1.0,
+         "param": 1.0},
+    ])
+    beam = Beam(type=source, absorption_efficiency=0.3, max_power=300.0,
+                radius=80 * UM, depth=60 * UM, segments=path)
+    bt = [BC_ADIABATIC] * 6
+    bt[4] = BC_CONVECTIVE
+    bt[5] = BC_RADIATIVE | BC_CONVECTIVE
+    p = Problem(n_cells=nc, extent=ext, fe_degree=deg, quadrature=0, dim=3,
+                material=m, boundary_types=bt, beams=[beam], method=method,
+                initial_temperature=300.0)
+    if kind == "q2_slp" and powder:
+        hz = ext[2] / nc[2]
+        p.set_powder_layer(ext[2], 2 * hz)
+    if rotate:
+        ang = np.random.RandomState(7).uniform(0, np.pi, p.n_cells_total)
+        p.deposition_cos = np.cos(ang)
+        p.deposition_sin = np.sin(ang)
+    return p
+
+
+def hot_field(p, lo, hi, seed=3):
+    """Smooth field from lo to hi plus a small random part."""
+    X = p.node_coordinates()
+    L, W, H = p.extent
+    s = 0.5 + 0.5 * np.sin(2.1 * X[:, 0] / L + 0.3) * np.cos(1.3 * X[:, 1] / W) \
+        * (0.5 + 0.5 * X[:, 2] / H)
+    r = np.random.RandomState(seed).uniform(-1, 1, p.n_dofs)
+    return lo + (hi - lo) * s + 5.0 * r
+
+
+def _op_value(p, t, T, mu, beam=None):
+    """mu . A(T) with a fresh device operator (initial phase state)."""
+    import torch
+    from adamantine_amd.thermal import ThermalOperatorDevice
+    op = ThermalOperatorDevice(p)
+    if beam is not None:
+        op.set_beam_parameters(0, *beam)
+    op.set_time(t)
+    dst = torch.zeros_like(T)
+    op.vmult(dst, T)
+    return float(torch.dot(mu, dst))
+
+
+@pytest.mark.parametrize("kind,source", [("q2_slp", SOURCE_GOLDAK),
+                                         ("q2_slp", SOURCE_ELECTRON_BEAM),
+                                         ("q1_poly", SOURCE_GOLDAK)])
+def test_operator_vjp_directional(kind, source):
+    import torch
+    from adamantine_amd.adjoint import operator_vjp
+    from adamantine_amd.thermal import ThermalOperatorDevice
+    print(f"operator vjp: {kind} source {source}", flush=True)
+    p = small_problem(kind, source=source)
+    t = 2e-5
+    lo, hi = (1560.0, 1800.0) if kind == "q2_slp" else (300.0, 900.0)
+    T = torch.tensor(hot_field(p, lo, hi), device="cuda")
+    rs = np.random.RandomState(11)
+    mu = torch.tensor(rs.uniform(-1, 1, p.n_dofs), device="cuda")
+    v = torch.tensor(rs.uniform(-1, 1, p.n_dofs), device="cuda")
+    op = ThermalOperatorDevice(p)
+    out, g = operator_vjp(op, t, T, mu)
+    torch.cuda.synchronize()
+    # state direction
+    h = 1e-4
+    fd = (_op_value(p, t, T + h * v, mu) - _op_value(p, t, T - h * v, mu)) / \
+        (2 * h)
+    adj = float(torch.dot(out, v))
+    print(f"  T-direction: fd {fd:.12e} adjoint {adj:.12e}", flush=True)
+    assert rel(adj, fd) < 1e-6
+    b = p.beams[0]
+    base = [b.absorption_efficiency, b.max_power, b.radius, b.depth]
+    # absorption efficiency (the operator is affine in it)
+    d = 1e-3 * base[0]
+    fd = (_op_value(p, t, T, mu, [base[0] + d] + base[1:]) -
+          _op_value(p, t, T, mu, [base[0] - d] + base[1:])) / (2 * d)
+    print(f"  eta: fd {fd:.12e} adjoint {g[0]:.12e}", flush=True)
+    assert rel(g[0], fd) < 1e-6
+    # beam radius
+    d = 1e-5 * base[2]
+    fd = (_op_value(p, t, T, mu, base[:2] + [base[2] + d, base[3]]) -
+          _op_value(p, t, T, mu, base[:2] + [base[2] - d, base[3]])) / (2 * d)
+    print(f"  radius: fd {fd:.12e} adjoint {g[2]:.12e}", flush=True)
+    assert rel(g[2], fd) < 1e-5
+    # conductivity factor
+    e = 1e-5
+    fd = (_op_value(small_problem(kind, k_scale=1 + e, source=source), t, T,
+                    mu) -
+          _op_value(small_problem(kind, k_scale=1 - e, source=source), t, T,
+                    mu)) / (2 * e)
+    print(f"  k-scale: fd {fd:.12e} adjoint {g[1]:.12e}", flush=True)
+    assert rel(g[1], fd) < 1e-6
+
+
+def _oracle_misfit(p, method, t0, dt, n, T0, dof, obs, beam=None):
+    from oracle.oracle import Oracle
+    o = Oracle(p)
+    if beam is not None:
+        o.set_beam_parameters(0, *beam)
+    T, _ = o.evolve(method, t0, dt, n, T0.copy())
+    r = T[dof] - obs
+    return 0.5 * float(np.dot(r, r)), T
+
+
+@pytest.mark.parametrize("method", [FORWARD_EULER, RK_FOURTH_ORDER])
+def test_adjoint_gradient_matches_finite_differences(method):
+    import torch
+    from adamantine_amd.adjoint import adjoint_gradient
+    from adamantine_amd.thermal import ThermalPhysicsDevice
+    print(f"adjoint gradient: 10x6x5 Q2 SLP no powder, method {method}",
+          flush=True)
+    p = small_problem("q2_slp", powder=False, method=method)
+    t0, dt, n = 0.0, 1e-6, 7
+    T0 = hot_field(p, 1500.0, 1690.0)
+    X = p.node_coordinates()
+    top = np.nonzero(np.isclose(X[:, 2], p.extent[2]))[0]
+    dof = top[::7]
+    _, Tref = _oracle_misfit(p, method, t0, dt, n, T0, dof, np.zeros(dof.size))
+    obs = Tref[dof] - 3.0 + np.random.RandomState(5).uniform(-1, 1, dof.size)
+    ph = ThermalPhysicsDevice(p, method)
+    ph.setup()
+    res = adjoint_gradient(ph, t0, dt, n, torch.tensor(T0, device="cuda"), dof,
+                           obs, checkpoint_every=3)
+    torch.cuda.synchronize()
+    # the forward run is the evolve of the product and agrees with the oracle
+    ph2 = ThermalPhysicsDevice(p, method)
+    ph2.setup()
+    sol = torch.tensor(T0, device="cuda")
+    ph2.evolve(t0, dt, n, sol)
+    np.testing.assert_array_equal(res.T_final.cpu().numpy(), sol.cpu().numpy())
+    Jo, _ = _oracle_misfit(p, method, t0, dt, n, T0, dof, obs)
+    assert rel(res.J, Jo) < 1e-7
+    b = p.beams[0]
+    base = [b.absorption_efficiency, b.max_power, b.radius, b.depth]
+
+    def J_at(beam=None, prob=None, T=T0):
+        return _oracle_misfit(prob or p, method, t0, dt, n, T, dof, obs,
+                              beam)[0]
+    d = 1e-4 * base[0]
+    fd = (J_at([base[0] + d] + base[1:]) - J_at([base[0] - d] + base[1:])) / \
+        (2 * d)
+    print(f"  eta: fd {fd:.10e} adjoint {res.grad[0]:.10e}", flush=True)
+    assert rel(res.grad[0], fd) < 1e-5
+    d = 1e-5 * base[2]
+    fd = (J_at(base[:2] + [base[2] + d, base[3]]) -
+          J_at(base[:2] + [base[2] - d, base[3]])) / (2 * d)
+    print(f"  radius: fd {fd:.10e} adjoint {res.grad[2]:.10e}", flush=True)
+    assert rel(res.grad[2], fd) < 1e-5
+    e = 1e-5
+    fd = (J_at(prob=small_problem("q2_slp", k_scale=1 + e, powder=False,
+                                  method=method)) -
+          J_at(prob=small_problem("q2_slp", k_scale=1 - e, powder=False,
+                                  method=method))) / (2 * e)
+    print(f"  k-scale: fd {fd:.10e} adjoint {res.grad[1]:.10e}", flush=True)
+    assert rel(res.grad[1], fd) < 1e-5
+    # initial temperature along a random direction
+    v = np.random.RandomState(9).uniform(-1, 1, p.n_dofs)
+    h = 1e-3
+    fd = (J_at(T=T0 + h * v) - J_at(T=T0 - h * v)) / (2 * h)
+    adj = float(np.dot(res.lambda0.cpu().numpy(), v))
+    print(f"  T0: fd {fd:.10e} adjoint {adj:.10e}", flush=True)
+    assert rel(adj, fd) < 1e-5
+    # one checkpoint gives the same gradient (up to atomic-add ordering)
+    ph3 = ThermalPhysicsDevice(p, method)
+    ph3.setup()
+    res1 = adjoint_gradient(ph3, t0, dt, n, torch.tensor(T0, device="cuda"),
+                            dof, obs, checkpoint_every=0)
+    for k in range(3):
+        assert rel(res1.grad[k], res.grad[k]) < 1e-10
+    assert res1.J == res.J
+
+
+def test_adjoint_rejects_bad_arguments():
+    import torch
+    from adamantine_amd.adjoint import adjoint_gradient
+    from adamantine_amd.thermal import AdmError, ThermalPhysicsDevice
+    p = small_problem("q1_poly")
+    ph = ThermalPhysicsDevice(p, FORWARD_EULER)
+    ph.setup()
+    T0 = torch.full((p.n_dofs,), 300.0, dtype=torch.float64, device="cuda")
+    with pytest.raises(AdmError):
+        adjoint_gradient(ph, 0.0, 1e-6, 3, T0, [p.n_dofs], [0.0])
+    with pytest.raises(AdmError):
+        adjoint_gradient(ph, 0.0, 1e-6, 0, T0, [0], [0.0])
+    with pytest.raises(AdmError):
+        adjoint_gradient(ph, 0.0, 1e-6, 3, T0, [0], [0.0],
+                         method=1)  # RK3: not supported by the adjoint
