@@ -16,11 +16,12 @@
 //   value coefficient   c_T = -JxW [d(1/rho c_p) gm.K g + (1/rho c_p) gm.K' g]
 //                             + mu d(1/rho c_p) Q JxW
 // integrated like A's fluxes and source. d/dT of the mixtures includes the
-// liquid ratio (mushy range) and the powder ratio min(p_old, 1 - liquid) of
-// that evaluation. The powder carried from one evaluation to the next is
-// held at its recorded value (its dependence on earlier temperatures is not
-// propagated); the latent-heat switch and the source cut-offs are
-// piecewise constant and contribute nothing.
+// liquid ratio (mushy range). The powder ratio carried from one evaluation
+// to the next, p = min(p_old, 1 - liquid), has its own adjoint per
+// quadrature point (cells and boundary faces), taken backwards through the
+// evaluations: where p was clamped to 1 - liquid it feeds the temperature
+// of that evaluation, elsewhere the powder before it. The latent-heat switch
+// and the source cut-offs are piecewise constant and contribute nothing.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -48,17 +49,26 @@ struct AdjArgs {
   const double *mu;           // M^-1 * stage adjoint
   double *out;                // += (dA/dT)^T mu
   const double *powder_old;   // powder before the evaluation [slots][n_q]
+  double *powder_bar;         // adjoint of the carried powder [slots][n_q]
+                              // (in: after, out: before the evaluation);
+                              // NULL: zero after, not stored
   double *partials;           // [3]: d/d eta_0, d/d k-factor, d/d R_0
   double inv_eta0;            // 1 / absorption efficiency of beam 0 (or 0)
 };
 
-// value and T-derivative of a mixed property from the merged tables
+// Mixed property value v with its partials: dT at fixed carried powder and
+// dp with respect to the carried powder ratio.
+template <int NS>
+struct Mix {
+  double r[NS], rT[NS], rp[NS];
+};
+
+// value, T- and powder-partials of a mixed property from the merged tables
 template <int NS>
 __device__ __forceinline__ void merged_vd(const double *B, const double *C,
                                           double T, int prop,
-                                          const double (&r)[NS],
-                                          const double (&dr)[NS], double &v,
-                                          double &dv) {
+                                          const Mix<NS> &x, double &v,
+                                          double &vT, double &vp) {
   int k = 0;
 #pragma unroll
   for (int step = kMergedMax / 2; step >= 1; step >>= 1)
@@ -66,68 +76,77 @@ __device__ __forceinline__ void merged_vd(const double *B, const double *C,
   const double d = T - B[k > 0 ? k - 1 : 0];
   const double *c = C + ((prop * kMergedMax + k) * ADM_MAX_STATES) * 2;
   v = 0.0;
-  dv = 0.0;
+  vT = 0.0;
+  vp = 0.0;
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    const double x = __builtin_fma(d, c[2 * s + 1], c[2 * s]);
-    v += r[s] * x;
-    dv += dr[s] * x + r[s] * c[2 * s + 1];
+    const double xs = __builtin_fma(d, c[2 * s + 1], c[2 * s]);
+    v += x.r[s] * xs;
+    vT += x.rT[s] * xs + x.r[s] * c[2 * s + 1];
+    vp += x.rp[s] * xs;
   }
 }
 
-// value and T-derivative of a mixed polynomial property
+// the same for a mixed polynomial property
 template <int NS>
 __device__ __forceinline__ void poly_vd(const MatConst &m, int mid, double T,
-                                        int prop, const double (&r)[NS],
-                                        const double (&dr)[NS], double &v,
-                                        double &dv) {
+                                        int prop, const Mix<NS> &x, double &v,
+                                        double &vT, double &vp) {
   v = 0.0;
-  dv = 0.0;
+  vT = 0.0;
+  vp = 0.0;
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    double x = 0.0, dx = 0.0, tp = 1.0, tpm = 0.0;
+    double xs = 0.0, dxs = 0.0, tp = 1.0, tpm = 0.0;
 #pragma unroll
     for (int i = 0; i < ADM_MAX_POLY; ++i) {
       if (i > m.p_order) break;
       const double ci = m.poly[mid][prop][s][i];
-      x += ci * tp;
-      dx += i * ci * tpm;
+      xs += ci * tp;
+      dxs += i * ci * tpm;
       tpm = tp;
       tp *= T;
     }
-    v += r[s] * x;
-    dv += dr[s] * x + r[s] * dx;
+    v += x.r[s] * xs;
+    vT += x.rT[s] * xs + x.r[s] * dxs;
+    vp += x.rp[s] * xs;
   }
 }
 
-// ratios (update_state_ratios) and their T-derivative at one point
+// update_state_ratios at one point with its partials. The carried powder
+// p = min(p_old, 1 - liquid) is returned through `clamped` (p = 1 - liquid,
+// dp/dT = -dl) or not (p = p_old).
 template <int NS>
-__device__ __forceinline__ void ratios_d(double T, double p_old, double sol,
-                                         double liq, double (&r)[NS],
-                                         double (&dr)[NS]) {
+__device__ __forceinline__ void ratios_lin(double T, double p_old, double sol,
+                                           double liq, Mix<NS> &x, double &dl,
+                                           bool &clamped) {
+  dl = 0.0;
+  clamped = false;
   if constexpr (NS == 1) {
-    r[0] = 1.0;
-    dr[0] = 0.0;
+    x.r[0] = 1.0;
+    x.rT[0] = 0.0;
+    x.rp[0] = 0.0;
   } else {
     const double mushy = (T - sol) / (liq - sol);
     const double l = T < sol ? 0.0 : (T > liq ? 1.0 : mushy);
-    const double dl = (T < sol || T > liq) ? 0.0 : 1.0 / (liq - sol);
+    dl = (T < sol || T > liq) ? 0.0 : 1.0 / (liq - sol);
+    x.r[1] = l;
+    x.rT[1] = dl;
+    x.rp[1] = 0.0;
     if constexpr (NS == 2) {
-      r[1] = l;
-      r[0] = 1.0 - l;
-      dr[1] = dl;
-      dr[0] = -dl;
+      x.r[0] = 1.0 - l;
+      x.rT[0] = -dl;
+      x.rp[0] = 0.0;
     } else {
       const double am = 1.0 - l;
-      const bool keep = p_old < am;
-      const double p = keep ? p_old : am;
-      const double dp = keep ? 0.0 : -dl;
-      r[1] = l;
-      r[2] = p;
-      r[0] = 1.0 - l - p;
-      dr[1] = dl;
-      dr[2] = dp;
-      dr[0] = -dl - dp;
+      clamped = !(p_old < am);
+      const double p = clamped ? am : p_old;
+      x.r[2] = p;
+      x.r[0] = 1.0 - l - p;
+      x.rT[2] = 0.0;
+      x.rT[0] = -dl;
+      x.rp[2] = 1.0;
+      x.rp[0] = -1.0;
     }
   }
 }
@@ -135,14 +154,31 @@ __device__ __forceinline__ void ratios_d(double T, double p_old, double sol,
 template <int NS, int PK>
 __device__ __forceinline__ void prop_vd(const MatConst &m, int mid,
                                         const double *mB, const double *mC,
-                                        double T, int prop,
-                                        const double (&r)[NS],
-                                        const double (&dr)[NS], double &v,
-                                        double &dv) {
+                                        double T, int prop, const Mix<NS> &x,
+                                        double &v, double &vT, double &vp) {
   if constexpr (PK == 2)
-    merged_vd<NS>(mB, mC, T, prop, r, dr, v, dv);
+    merged_vd<NS>(mB, mC, T, prop, x, v, vT, vp);
   else
-    poly_vd<NS>(m, mid, T, prop, r, dr, v, dv);
+    poly_vd<NS>(m, mid, T, prop, x, v, vT, vp);
+}
+
+// 1/(rho c_p) (get_inv_rho_cp, latent heat in the mushy range) and its
+// partials
+template <int NS, int PK>
+__device__ __forceinline__ void inv_rho_cp_lin(const MatConst &m, int mid,
+                                               const double *mB,
+                                               const double *mC, double T,
+                                               double latent,
+                                               const Mix<NS> &x, double &irc,
+                                               double &ircT, double &ircp) {
+  double rho, rhoT, rhop, cp, cpT, cpp;
+  prop_vd<NS, PK>(m, mid, mB, mC, T, ADM_DENSITY, x, rho, rhoT, rhop);
+  prop_vd<NS, PK>(m, mid, mB, mC, T, ADM_SPECIFIC_HEAT, x, cp, cpT, cpp);
+  if constexpr (NS > 1)
+    if (x.r[1] * x.r[0] > 0.0) cp += latent;
+  irc = 1.0 / (rho * cp);
+  ircT = -irc * irc * (rhoT * cp + rho * cpT);
+  ircp = -irc * irc * (rhop * cp + rho * cpp);
 }
 
 // d Q_0 / d R_0 at a point for beam 0 (Goldak or electron beam; the cut-offs
@@ -308,6 +344,8 @@ __global__ void __launch_bounds__(256)
     const int64_t slot =
         (meta & kHasPowder) ? (int64_t)a.powder_slot[c] : -1;
     const double *pw = slot >= 0 ? aa.powder_old + slot * N3 + own : nullptr;
+    double *pb = (slot >= 0 && aa.powder_bar) ? aa.powder_bar + slot * N3 + own
+                                              : nullptr;
     const bool rotated = (meta & kRotated) != 0;
     double cs = 1.0, sn = 0.0;
     if (rotated) {
@@ -322,34 +360,33 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int q = 0; q < N; ++q) {
       const double T = V[0][q], mu = V[1][q];
-      double rr[NS], dr[NS];
-      ratios_d<NS>(T, pw ? pw[q] : 0.0, sol, liq, rr, dr);
-      double rho, drho, cp, dcp, kx, dkx, ky, dky, kz, dkz;
-      prop_vd<NS, PK>(m, mid, mB, mC, T, ADM_DENSITY, rr, dr, rho, drho);
-      prop_vd<NS, PK>(m, mid, mB, mC, T, ADM_SPECIFIC_HEAT, rr, dr, cp, dcp);
-      if constexpr (NS > 1)
-        if (rr[1] * rr[0] > 0.0) cp += latent;
-      prop_vd<NS, PK>(m, mid, mB, mC, T, ADM_THERMAL_CONDUCTIVITY_X, rr, dr,
-                      kx, dkx);
-      prop_vd<NS, PK>(m, mid, mB, mC, T, ADM_THERMAL_CONDUCTIVITY_Y, rr, dr,
-                      ky, dky);
-      prop_vd<NS, PK>(m, mid, mB, mC, T, ADM_THERMAL_CONDUCTIVITY_Z, rr, dr,
-                      kz, dkz);
-      const double irc = 1.0 / (rho * cp);
-      const double dirc = -irc * irc * (drho * cp + rho * dcp);
-      // K and K' in x, y (rotation by the deposition angle), z
+      Mix<NS> x;
+      double dl;
+      bool clamped;
+      ratios_lin<NS>(T, pw ? pw[q] : 0.0, sol, liq, x, dl, clamped);
+      double irc, ircT, ircp;
+      inv_rho_cp_lin<NS, PK>(m, mid, mB, mC, T, latent, x, irc, ircT, ircp);
+      double kx, kxT, kxp, ky, kyT, kyp, kz, kzT, kzp;
+      prop_vd<NS, PK>(m, mid, mB, mC, T, ADM_THERMAL_CONDUCTIVITY_X, x, kx,
+                      kxT, kxp);
+      prop_vd<NS, PK>(m, mid, mB, mC, T, ADM_THERMAL_CONDUCTIVITY_Y, x, ky,
+                      kyT, kyp);
+      prop_vd<NS, PK>(m, mid, mB, mC, T, ADM_THERMAL_CONDUCTIVITY_Z, x, kz,
+                      kzT, kzp);
+      const double g0 = G[0][0][q], g1 = G[0][1][q], g2 = G[0][2][q];
+      const double m0 = G[1][0][q], m1 = G[1][1][q], m2 = G[1][2][q];
+      // mu-gradient . K g for the conductivities (kx, ky, kz) with the x-y
+      // rotation by the deposition angle
+      auto mKg_of = [&](double ax, double ay, double az) {
+        const double kxx = ax * cs * cs + ay * sn * sn;
+        const double kyy = ax * sn * sn + ay * cs * cs;
+        const double kxy = (ax - ay) * sn * cs;
+        return m0 * (kxx * g0 + kxy * g1) + m1 * (kxy * g0 + kyy * g1) +
+               m2 * (az * g2);
+      };
       const double kxx = kx * cs * cs + ky * sn * sn;
       const double kyy = kx * sn * sn + ky * cs * cs;
       const double kxy = (kx - ky) * sn * cs;
-      const double dkxx = dkx * cs * cs + dky * sn * sn;
-      const double dkyy = dkx * sn * sn + dky * cs * cs;
-      const double dkxy = (dkx - dky) * sn * cs;
-      const double g0 = G[0][0][q], g1 = G[0][1][q], g2 = G[0][2][q];
-      const double m0 = G[1][0][q], m1 = G[1][1][q], m2 = G[1][2][q];
-      const double Kg0 = kxx * g0 + kxy * g1, Kg1 = kxy * g0 + kyy * g1,
-                   Kg2 = kz * g2;
-      const double dKg0 = dkxx * g0 + dkxy * g1, dKg1 = dkxy * g0 + dkyy * g1,
-                   dKg2 = dkz * g2;
       const double Km0 = kxx * m0 + kxy * m1, Km1 = kxy * m0 + kyy * m1,
                    Km2 = kz * m2;
       double wxyz = 1.0;
@@ -357,12 +394,15 @@ __global__ void __launch_bounds__(256)
       wxyz *= kc.wq[la];
       wxyz *= kc.wq[lb];
       const double jxw = kc.det * wxyz;
-      const double mKg = m0 * Kg0 + m1 * Kg1 + m2 * Kg2;
-      const double mdKg = m0 * dKg0 + m1 * dKg1 + m2 * dKg2;
+      const double mKg = mKg_of(kx, ky, kz);
       cw[0][q] = ((-irc) * Km0) * jxw * kc.inv_h[0];
       cw[1][q] = ((-irc) * Km1) * jxw * kc.inv_h[1];
       cw[2][q] = ((-irc) * Km2) * jxw * kc.inv_h[2];
-      double c_T = -jxw * (dirc * mKg + irc * mdKg);
+      // partials at fixed carried powder (T) and with respect to it (p)
+      double c_T = -jxw * (ircT * mKg + irc * mKg_of(kxT, kyT, kzT));
+      double c_p = 0.0;
+      if constexpr (NS == 3)
+        if (pw) c_p = -jxw * (ircp * mKg + irc * mKg_of(kxp, kyp, kzp));
       ps += -irc * jxw * mKg;
       if (a.sources_on) {
         const double px = kc.origin[0] + (rc.cc[0] + kc.xq[q]) * kc.h[0];
@@ -373,9 +413,20 @@ __global__ void __launch_bounds__(256)
           if (b == 0) q0 = v;
           qs += v;
         }
-        c_T += mu * dirc * qs * jxw;
+        c_T += mu * ircT * qs * jxw;
+        if constexpr (NS == 3) c_p += mu * ircp * qs * jxw;
         pe += mu * irc * q0 * aa.inv_eta0 * jxw;
         pr += mu * irc * beam_value_dR(sbeams[0], q0, px, py, kc.log_01) * jxw;
+      }
+      if constexpr (NS == 3) {
+        if (pw) {
+          // the carried powder after this evaluation: its adjoint (later
+          // evaluations) plus this evaluation's dependence on it goes to T
+          // where it was clamped to 1 - liquid, else to the powder before
+          const double tot = c_p + (pb ? pb[q] : 0.0);
+          if (clamped) c_T -= dl * tot;
+          if (pb) pb[q] = clamped ? 0.0 : tot;
+        }
       }
       cT[q] = c_T;
     }
@@ -454,12 +505,15 @@ __global__ void __launch_bounds__(256)
 }
 
 // (dA_face/dT)^T mu over the boundary faces (one lane per face, as
-// face_kernel), with the face powder recorded before the evaluation.
+// face_kernel), with the face powder recorded before the evaluation and the
+// adjoint of the carried face powder (face_powder_bar, NULL: zero, not
+// stored) as in the cell kernel.
 template <int P, int NS, int PK>
 __global__ void __launch_bounds__(256)
     adjoint_face_kernel(const FaceArgs a, const double *__restrict__ T,
                         const double *__restrict__ mu,
                         const double *__restrict__ face_powder_old,
+                        double *__restrict__ face_powder_bar,
                         double *__restrict__ out) {
   constexpr int N = P + 1, NF = N * N;
   const int64_t fi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -493,40 +547,48 @@ __global__ void __launch_bounds__(256)
   const double *mB = &m.merged[mid].B[0];
   const double *mC = &m.merged[mid].coef[0][0][0][0];
   const double *fp = NS == 3 ? face_powder_old + fi * NF : nullptr;
+  double *fpb = (NS == 3 && face_powder_bar) ? face_powder_bar + fi * NF
+                                             : nullptr;
 #pragma unroll
   for (int q = 0; q < NF; ++q) {
     const double Tq = fv[q];
-    double rr[NS], dr[NS];
-    ratios_d<NS>(Tq, fp ? fp[q] : 0.0, sol, liq, rr, dr);
-    double rho, drho, cp, dcp;
-    prop_vd<NS, PK>(m, mid, mB, mC, Tq, ADM_DENSITY, rr, dr, rho, drho);
-    prop_vd<NS, PK>(m, mid, mB, mC, Tq, ADM_SPECIFIC_HEAT, rr, dr, cp, dcp);
-    if constexpr (NS > 1)
-      if (rr[1] * rr[0] > 0.0) cp += latent;
-    const double irc = 1.0 / (rho * cp);
-    const double dirc = -irc * irc * (drho * cp + rho * dcp);
-    double s1 = 0.0, ds1 = 0.0;
+    Mix<NS> x;
+    double dl;
+    bool clamped;
+    ratios_lin<NS>(Tq, fp ? fp[q] : 0.0, sol, liq, x, dl, clamped);
+    double irc, ircT, ircp;
+    inv_rho_cp_lin<NS, PK>(m, mid, mB, mC, Tq, latent, x, irc, ircT, ircp);
+    // s = h (T - T_conv) + eps sigma (T + T_rad)(T^2 + T_rad^2)(T - T_rad)
+    double s1 = 0.0, s1T = 0.0, s1p = 0.0;
     if (bt & ADM_BC_CONVECTIVE) {
       const double cT = m.prop[mid][ADM_CONVECTION_TEMPERATURE_INFTY];
-      double h, dh;
-      prop_vd<NS, PK>(m, mid, mB, mC, Tq, ADM_CONVECTION_HEAT_TRANSFER_COEF,
-                      rr, dr, h, dh);
+      double h, hT, hp;
+      prop_vd<NS, PK>(m, mid, mB, mC, Tq, ADM_CONVECTION_HEAT_TRANSFER_COEF, x,
+                      h, hT, hp);
       s1 += h * (Tq - cT);
-      ds1 += dh * (Tq - cT) + h;
+      s1T += hT * (Tq - cT) + h;
+      s1p += hp * (Tq - cT);
     }
     if (bt & ADM_BC_RADIATIVE) {
       const double rT = m.prop[mid][ADM_RADIATION_TEMPERATURE_INFTY];
-      double e, de;
-      prop_vd<NS, PK>(m, mid, mB, mC, Tq, ADM_EMISSIVITY, rr, dr, e, de);
+      double e, eT, ep;
+      prop_vd<NS, PK>(m, mid, mB, mC, Tq, ADM_EMISSIVITY, x, e, eT, ep);
       const double a1 = Tq + rT, a2 = Tq * Tq + rT * rT;
-      const double rh = e * kStefanBoltzmann * a1 * a2;
-      const double drh = kStefanBoltzmann * (de * a1 * a2 + e * a2 +
-                                             e * a1 * 2.0 * Tq);
-      s1 += rh * (Tq - rT);
-      ds1 += drh * (Tq - rT) + rh;
+      const double g = kStefanBoltzmann * a1 * a2 * (Tq - rT);
+      const double gT = kStefanBoltzmann *
+                        (a2 * (Tq - rT) + a1 * 2.0 * Tq * (Tq - rT) + a1 * a2);
+      s1 += e * g;
+      s1T += eT * g + e * gT;
+      s1p += ep * g;
     }
-    const double dval = -dirc * s1 - irc * ds1;
-    fm[q] = gv[q] * dval * kc.fjxw[d][q];
+    const double w = gv[q] * kc.fjxw[d][q];
+    double c_T = (-ircT * s1 - irc * s1T) * w;
+    if constexpr (NS == 3) {
+      const double tot = (-ircp * s1 - irc * s1p) * w + (fpb ? fpb[q] : 0.0);
+      if (clamped) c_T -= dl * tot;
+      if (fpb) fpb[q] = clamped ? 0.0 : tot;
+    }
+    fm[q] = c_T;
   }
   sweep<N, 1, true, false, 2>(kc.S, fm, ft);
   sweep<N, 0, true, false, 2>(kc.S, ft, fv);
@@ -614,10 +676,13 @@ inline void eval_time_reset(Context &c, double t) {
 
 // out += (dA/dT)^T mu at the stage input Y with the phase state (pw, fpw)
 // recorded before the evaluation, at the context's current time; the
-// parameter partials are added to partials[3].
+// adjoints of the carried cell and face powder (pb, fpb; NULL: zero) are
+// taken from after to before the evaluation, and the parameter partials are
+// added to partials[3].
 inline void launch_vjp(Context &c, const double *Y, const double *pw,
-                     const double *fpw, const double *mu, double *out,
-                     double *partials, double inv_eta0, hipStream_t s) {
+                       const double *fpw, double *pb, double *fpb,
+                       const double *mu, double *out, double *partials,
+                       double inv_eta0, hipStream_t s) {
   const int pk = c.property_kind();
   if (pk == 1)
     throw AdmError(ADM_ERR_UNSUPPORTED,
@@ -636,6 +701,7 @@ inline void launch_vjp(Context &c, const double *Y, const double *pw,
   aa.mu = mu;
   aa.out = out;
   aa.powder_old = pw;
+  aa.powder_bar = pb;
   aa.partials = partials;
   aa.inv_eta0 = inv_eta0;
   const unsigned blocks = (unsigned)((aa.c.n_rows + 3) / 4);
@@ -653,7 +719,7 @@ inline void launch_vjp(Context &c, const double *Y, const double *pw,
     if (fa.n_faces)                                                        \
       hipLaunchKernelGGL((adjoint_face_kernel<P_, NS_, PK_>),              \
                          dim3(face_blocks), dim3(256), 0, s, fa, Y, mu, fpw, \
-                         out);                                              \
+                         fpb, out);                                         \
     launched = true;                                                       \
   }
   bool launched = false;
@@ -686,6 +752,7 @@ struct AdjointRun {
   double a_next[4], b[4], cst[4];
   DevBuf partials;
   DevBuf K[4], Y, R;
+  DevBuf pbar, fpbar;  // adjoints of the carried cell and face powder
 
   AdjointRun(Context &c_, int method_, hipStream_t s_)
       : c(c_), s(s_), method(method_) {
@@ -715,6 +782,11 @@ struct AdjointRun {
     R.alloc(n);
     ADM_HIP_ADJ(hipMemsetAsync(R.p, 0, sizeof(double) * n, s));
     partials.alloc(3);
+    pbar.alloc(npw);
+    fpbar.alloc(nfpw);
+    if (npw) ADM_HIP_ADJ(hipMemsetAsync(pbar.p, 0, sizeof(double) * npw, s));
+    if (nfpw)
+      ADM_HIP_ADJ(hipMemsetAsync(fpbar.p, 0, sizeof(double) * nfpw, s));
   }
 
   unsigned grid(int64_t m) const { return (unsigned)((m + 255) / 256); }
@@ -763,7 +835,8 @@ struct AdjointRun {
   void vjp(const StageRecord &rec, double t_eval, const double *mu,
            double *out, double inv_eta0) {
     eval_time(t_eval);
-    launch_vjp(c, rec.Y, rec.pw, rec.fpw, mu, out, partials.p, inv_eta0, s);
+    launch_vjp(c, rec.Y, rec.pw, rec.fpw, pbar.p, fpbar.p, mu, out,
+               partials.p, inv_eta0, s);
   }
 };
 
@@ -955,8 +1028,8 @@ int adm_operator_vjp(void *ptr, double t, const double *T, const double *mu,
     adm::eval_time_reset(c, t);
     const double eta0 =
         c.beams.empty() ? 0.0 : c.beams[0].desc.absorption_efficiency;
-    adm::launch_vjp(c, T, c.d_powder, c.d_face_powder, mu, out, partials.p,
-                    eta0 != 0.0 ? 1.0 / eta0 : 0.0, s);
+    adm::launch_vjp(c, T, c.d_powder, c.d_face_powder, nullptr, nullptr, mu,
+                    out, partials.p, eta0 != 0.0 ? 1.0 / eta0 : 0.0, s);
     double pp[3];
     ADM_HIP_ADJ(hipMemcpyAsync(pp, partials.p, sizeof(pp),
                                hipMemcpyDeviceToHost, s));

@@ -9,11 +9,14 @@ are numpy's legacy RandomState streams, which numpy keeps stable):
    8-breakpoint k(T), c_p(T) tables, top 4 cell layers powder, Goldak 400 W
    on a 5-track serpentine, radiative + convective top face, RK4 5e-7 s.
 3. ``ensemble``     64x32x16 Q2, same material model, 64 members.
-4. ``adjoint``      64x32x16 Q2, 200 RK4 steps.
+4. ``adjoint``      64x32x16 Q2 (12 of 16 layers active, top 4 powder),
+                     200 RK4 steps of 5e-7 s, checkpoints every 20 steps,
+                     200 thermocouple-like points at the final time.
 5. ``deposition``   96x96x40 Q2, electron beam, layer activation.
 """
 from __future__ import annotations
 
+from dataclasses import dataclass
 from typing import List
 
 import numpy as np
@@ -204,3 +207,54 @@ def meltpool_temperature(problem: Problem, t: float, peak: float = 2600.0,
     d2 = ((X[:, 0] - center[0]) ** 2 + (X[:, 1] - center[1]) ** 2) / (
         2 * sigma_xy ** 2) + (X[:, 2] - center[2]) ** 2 / (2 * sigma_z ** 2)
     return ambient + (peak - ambient) * np.exp(-d2)
+
+
+@dataclass
+class AdjointWorkload:
+    problem: Problem          # the model (what the gradient is taken of)
+    truth: Problem            # the model with the truth conductivities
+    truth_beam: list          # [absorption, max power, radius, depth]
+    obs_dof: np.ndarray       # [n_obs] observed nodes
+    noise: np.ndarray         # [n_obs] N(0, 1) added to the truth values
+    dt: float
+    n_steps: int
+    checkpoint_every: int
+
+
+def adjoint_workload(n_cells=(64, 32, 16), extent=(4 * MM, 2 * MM, 1 * MM),
+                     powder_layers: int = 4, n_steps: int = 200,
+                     dt: float = 5e-7, n_obs: int = 200,
+                     checkpoint_every: int = 20) -> AdjointWorkload:
+    """The adjoint workload: the base problem of ``ensemble_workload``
+    (12 of 16 layers active, the top 4 of them powder, serpentine path on
+    the active top), T0 = 300 K, ``n_steps`` RK4 steps of ``dt``.
+
+    Observations: ``n_obs`` points uniform (numpy RandomState(6)) in the
+    first 0.4 mm of the first track, within 0.1 mm of it in y, on the
+    active top surface, each observed at its nearest node; the data are the
+    final temperatures of a truth run with absorption efficiency 0.32, beam
+    radius 1.1 x the model's and every conductivity x 1.05, plus N(0, 1)
+    noise drawn next from the same stream."""
+    import copy
+    p, _ = ensemble_workload(n_members=1, n_cells=n_cells, extent=extent,
+                             powder_layers=powder_layers)
+    nz = n_cells[2]
+    top = ((3 * nz) // 4) * extent[2] / nz
+    rs = np.random.RandomState(6)
+    seg = p.beams[0].segments
+    x0, y0 = seg[0][2][0], seg[0][2][1]
+    x = rs.uniform(x0, x0 + 0.4 * MM, n_obs)
+    y = rs.uniform(y0 - 0.1 * MM, y0 + 0.1 * MM, n_obs)
+    noise = rs.standard_normal(n_obs)
+    nn = p.n_nodes
+    h = [extent[d] / (n_cells[d] * p.fe_degree) for d in range(3)]
+    ix = np.rint((x - p.origin[0]) / h[0]).astype(np.int64)
+    iy = np.rint((y - p.origin[1]) / h[1]).astype(np.int64)
+    iz = int(round((top - p.origin[2]) / h[2]))
+    obs_dof = ix + nn[0] * (iy + nn[1] * iz)
+    truth = copy.deepcopy(p)
+    truth.material = slp_tables(k_scale=1.05)
+    b = p.beams[0]
+    return AdjointWorkload(p, truth, [0.32, b.max_power, 1.1 * b.radius,
+                                      b.depth], obs_dof, noise, dt, n_steps,
+                           checkpoint_every)

@@ -144,6 +144,54 @@ def ensemble_bench(n_members: int, n_steps: int):
             "loop_sample": "%d members x %d steps" % (n_loop, k_loop)}
 
 
+def adjoint_bench():
+    """The adjoint workload (64x32x16 Q2, 200 RK4 steps, checkpoints every
+    20): one gradient (forward sweep, then the reverse sweep recomputing each
+    segment) against one forward run of the same steps."""
+    import numpy as np
+    import torch
+    from adamantine_amd.adjoint import adjoint_gradient
+    from adamantine_amd.thermal import ThermalPhysicsDevice
+    from adamantine_amd.workloads import adjoint_workload
+    w = adjoint_workload()
+    ph = ThermalPhysicsDevice(w.truth)
+    ph.setup()
+    ph.update_physics_parameters(0, *w.truth_beam)
+    sol = ph.initialize_dof_vector(300.0)
+    ph.evolve(0.0, w.dt, w.n_steps, sol)
+    obs = sol.cpu().numpy()[w.obs_dof] + w.noise
+    del ph, sol
+    # forward alone
+    ph = ThermalPhysicsDevice(w.problem)
+    ph.setup()
+    sol = ph.initialize_dof_vector(300.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ph.evolve(0.0, w.dt, w.n_steps, sol)
+    torch.cuda.synchronize()
+    fwd_s = time.perf_counter() - t0
+    del ph, sol
+    # gradient (fresh state)
+    ph = ThermalPhysicsDevice(w.problem)
+    ph.setup()
+    T0 = ph.initialize_dof_vector(300.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = adjoint_gradient(ph, 0.0, w.dt, w.n_steps, T0, w.obs_dof, obs,
+                           checkpoint_every=w.checkpoint_every)
+    torch.cuda.synchronize()
+    adj_s = time.perf_counter() - t0
+    return {"workload": "64x32x16 Q2 (12 layers active, 4 powder), %d RK4 "
+                        "steps of %g s, checkpoint every %d, %d observations"
+                        % (w.n_steps, w.dt, w.checkpoint_every,
+                           w.obs_dof.size),
+            "n_dofs": w.problem.n_dofs, "forward_ms": fwd_s * 1e3,
+            "gradient_ms": adj_s * 1e3, "gradient_over_forward": adj_s / fwd_s,
+            "J": res.J, "grad": [float(g) for g in res.grad],
+            "grad_names": ["absorption_efficiency", "conductivity_scale",
+                           "beam_radius"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,7 +199,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--only", choices=["all", "vmult", "ensemble", "copy"],
+    ap.add_argument("--only", choices=["all", "vmult", "ensemble", "copy",
+                                       "adjoint"],
                     default="all",
                     help="profiling aid: run one part and print its timing")
     ap.add_argument("--members", type=int, default=64)
@@ -176,6 +225,9 @@ def main():
         return 0
     if args.only == "ensemble":
         print(json.dumps(ensemble_bench(args.members, args.steps)), flush=True)
+        return 0
+    if args.only == "adjoint":
+        print(json.dumps(adjoint_bench()), flush=True)
         return 0
 
     problem = plain_large()
@@ -260,6 +312,8 @@ def main():
     del ph, sol, op, T, dst
     torch.cuda.empty_cache()
     result["ensemble"] = ensemble_bench(args.members, 10)
+    torch.cuda.empty_cache()
+    result["adjoint"] = adjoint_bench()
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(problem, args.cpu_seconds)
     print(json.dumps(result), flush=True)

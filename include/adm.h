@@ -453,10 +453,11 @@ int adm_ensemble_transform(void *ens, double *T, const double *Tm,
  *        dJ/ds for every thermal conductivity scaled by s at s = 1,
  *        dJ/d(radius of beam 0)
  *   lambda0 device out: dJ/dT0.
- * Powder ratios carried between operator evaluations are held at their
- * recorded values in the linearisation; at return the context's phase state
- * is the one at the final time, as after adm_evolve. Polynomial or tabulated
- * properties with the merged-table form (not the general-table evaluator). */
+ * The powder ratio carried between operator evaluations is differentiated
+ * exactly (its own adjoint per quadrature point); at return the context's
+ * phase state is the one at the final time, as after adm_evolve. Polynomial
+ * or tabulated properties with the merged-table form (not the general-table
+ * evaluator). */
 int adm_adjoint_gradient(void *ctx, int method, double t0, double dt,
                          int n_steps, int checkpoint_every, const double *T0,
                          int32_t n_obs, const int64_t *obs_dof,
@@ -468,7 +469,9 @@ int adm_adjoint_gradient(void *ctx, int method, double t0, double dt,
  * beams at time t and the context's current phase state, out (device,
  * overwritten) = (dA/dT)^T mu at T, A being adm_vmult's operator; grad
  * (host out[3], may be NULL) = mu . dA/dtheta for the parameters of
- * adm_adjoint_gradient. Reads the phase state without updating it. */
+ * adm_adjoint_gradient. Reads the phase state without updating it; the
+ * powder ratio this evaluation would carry forward is differentiated as a
+ * function of T (no later evaluation depends on it here). */
 int adm_operator_vjp(void *ctx, double t, const double *T, const double *mu,
                      double *out, double *grad, void *stream);
 

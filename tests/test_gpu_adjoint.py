@@ -117,7 +117,7 @@ def test_operator_vjp_directional(kind, source):
         (2 * h)
     adj = float(torch.dot(out, v))
     print(f"  T-direction: fd {fd:.12e} adjoint {adj:.12e}", flush=True)
-    assert rel(adj, fd) < 1e-6
+    assert abs(fd) > 0 and rel(adj, fd) < 1e-6
     b = p.beams[0]
     base = [b.absorption_efficiency, b.max_power, b.radius, b.depth]
     # absorption efficiency (the operator is affine in it)
@@ -125,13 +125,13 @@ def test_operator_vjp_directional(kind, source):
     fd = (_op_value(p, t, T, mu, [base[0] + d] + base[1:]) -
           _op_value(p, t, T, mu, [base[0] - d] + base[1:])) / (2 * d)
     print(f"  eta: fd {fd:.12e} adjoint {g[0]:.12e}", flush=True)
-    assert rel(g[0], fd) < 1e-6
+    assert abs(fd) > 0 and rel(g[0], fd) < 1e-6
     # beam radius
     d = 1e-5 * base[2]
     fd = (_op_value(p, t, T, mu, base[:2] + [base[2] + d, base[3]]) -
           _op_value(p, t, T, mu, base[:2] + [base[2] - d, base[3]])) / (2 * d)
     print(f"  radius: fd {fd:.12e} adjoint {g[2]:.12e}", flush=True)
-    assert rel(g[2], fd) < 1e-5
+    assert abs(fd) > 0 and rel(g[2], fd) < 1e-5
     # conductivity factor
     e = 1e-5
     fd = (_op_value(small_problem(kind, k_scale=1 + e, source=source), t, T,
@@ -139,7 +139,7 @@ def test_operator_vjp_directional(kind, source):
           _op_value(small_problem(kind, k_scale=1 - e, source=source), t, T,
                     mu)) / (2 * e)
     print(f"  k-scale: fd {fd:.12e} adjoint {g[1]:.12e}", flush=True)
-    assert rel(g[1], fd) < 1e-6
+    assert abs(fd) > 0 and rel(g[1], fd) < 1e-6
 
 
 def _oracle_misfit(p, method, t0, dt, n, T0, dof, obs, beam=None):
@@ -152,16 +152,25 @@ def _oracle_misfit(p, method, t0, dt, n, T0, dof, obs, beam=None):
     return 0.5 * float(np.dot(r, r)), T
 
 
-@pytest.mark.parametrize("method", [FORWARD_EULER, RK_FOURTH_ORDER])
-def test_adjoint_gradient_matches_finite_differences(method):
+@pytest.mark.parametrize("method,powder", [(FORWARD_EULER, False),
+                                           (RK_FOURTH_ORDER, False),
+                                           (RK_FOURTH_ORDER, True)])
+def test_adjoint_gradient_matches_finite_differences(method, powder):
+    """With powder, the top two layers start as powder in the mushy range,
+    so the carried powder ratio is clamped at some evaluations and kept at
+    others along the run. Powder points cool by ~1e-5 K per stage there, so
+    a difference step of 1e-3 K flips that branch at many points (the
+    quotient then mixes both one-sided derivatives: 10.14 against 5.968 on
+    this case); at 1e-4 K and below it agrees with the adjoint to 1e-8."""
     import torch
     from adamantine_amd.adjoint import adjoint_gradient
     from adamantine_amd.thermal import ThermalPhysicsDevice
-    print(f"adjoint gradient: 10x6x5 Q2 SLP no powder, method {method}",
+    print(f"adjoint gradient: 10x6x5 Q2 SLP powder {powder}, method {method}",
           flush=True)
-    p = small_problem("q2_slp", powder=False, method=method)
+    p = small_problem("q2_slp", powder=powder, method=method)
     t0, dt, n = 0.0, 1e-6, 7
-    T0 = hot_field(p, 1500.0, 1690.0)
+    T0 = hot_field(p, 1500.0, 1720.0 if powder else 1690.0)
+    tol = 1e-5
     X = p.node_coordinates()
     top = np.nonzero(np.isclose(X[:, 2], p.extent[2]))[0]
     dof = top[::7]
@@ -190,26 +199,26 @@ def test_adjoint_gradient_matches_finite_differences(method):
     fd = (J_at([base[0] + d] + base[1:]) - J_at([base[0] - d] + base[1:])) / \
         (2 * d)
     print(f"  eta: fd {fd:.10e} adjoint {res.grad[0]:.10e}", flush=True)
-    assert rel(res.grad[0], fd) < 1e-5
+    assert abs(fd) > 0 and rel(res.grad[0], fd) < tol
     d = 1e-5 * base[2]
     fd = (J_at(base[:2] + [base[2] + d, base[3]]) -
           J_at(base[:2] + [base[2] - d, base[3]])) / (2 * d)
     print(f"  radius: fd {fd:.10e} adjoint {res.grad[2]:.10e}", flush=True)
-    assert rel(res.grad[2], fd) < 1e-5
+    assert abs(fd) > 0 and rel(res.grad[2], fd) < tol
     e = 1e-5
-    fd = (J_at(prob=small_problem("q2_slp", k_scale=1 + e, powder=False,
+    fd = (J_at(prob=small_problem("q2_slp", k_scale=1 + e, powder=powder,
                                   method=method)) -
-          J_at(prob=small_problem("q2_slp", k_scale=1 - e, powder=False,
+          J_at(prob=small_problem("q2_slp", k_scale=1 - e, powder=powder,
                                   method=method))) / (2 * e)
     print(f"  k-scale: fd {fd:.10e} adjoint {res.grad[1]:.10e}", flush=True)
-    assert rel(res.grad[1], fd) < 1e-5
+    assert abs(fd) > 0 and rel(res.grad[1], fd) < tol
     # initial temperature along a random direction
     v = np.random.RandomState(9).uniform(-1, 1, p.n_dofs)
-    h = 1e-3
+    h = 1e-4
     fd = (J_at(T=T0 + h * v) - J_at(T=T0 - h * v)) / (2 * h)
     adj = float(np.dot(res.lambda0.cpu().numpy(), v))
     print(f"  T0: fd {fd:.10e} adjoint {adj:.10e}", flush=True)
-    assert rel(adj, fd) < 1e-5
+    assert abs(fd) > 0 and rel(adj, fd) < tol
     # one checkpoint gives the same gradient (up to atomic-add ordering)
     ph3 = ThermalPhysicsDevice(p, method)
     ph3.setup()
@@ -235,3 +244,55 @@ def test_adjoint_rejects_bad_arguments():
     with pytest.raises(AdmError):
         adjoint_gradient(ph, 0.0, 1e-6, 3, T0, [0], [0.0],
                          method=1)  # RK3: not supported by the adjoint
+
+
+def test_adjoint_workload_against_fd():
+    """The 64x32x16 Q2 adjoint workload (powder melting under the beam, 200
+    RK4 steps, checkpoints every 20): each parameter gradient against central
+    differences of the device forward run with relative step 1e-6."""
+    import copy
+    import torch
+    from adamantine_amd.adjoint import adjoint_gradient
+    from adamantine_amd.thermal import ThermalPhysicsDevice
+    from adamantine_amd.workloads import adjoint_workload, slp_tables
+    w = adjoint_workload()
+    print("adjoint workload: 64x32x16 Q2, 200 RK4 steps", flush=True)
+
+    def run(prob, beam=None):
+        ph = ThermalPhysicsDevice(prob)
+        ph.setup()
+        if beam is not None:
+            ph.update_physics_parameters(0, *beam)
+        sol = ph.initialize_dof_vector(300.0)
+        ph.evolve(0.0, w.dt, w.n_steps, sol)
+        return sol.cpu().numpy()
+    obs = run(w.truth, w.truth_beam)[w.obs_dof] + w.noise
+
+    def J(prob=None, beam=None):
+        r = run(prob or w.problem, beam)[w.obs_dof] - obs
+        return 0.5 * float(np.dot(r, r))
+    ph = ThermalPhysicsDevice(w.problem)
+    ph.setup()
+    T0 = ph.initialize_dof_vector(300.0)
+    res = adjoint_gradient(ph, 0.0, w.dt, w.n_steps, T0, w.obs_dof, obs,
+                           checkpoint_every=w.checkpoint_every)
+    assert rel(res.J, J()) < 1e-12
+    b = w.problem.beams[0]
+    base = [b.absorption_efficiency, b.max_power, b.radius, b.depth]
+    eps = 1e-6
+    for k, idx in ((0, 0), (2, 2)):
+        up, dn = list(base), list(base)
+        up[idx] *= 1 + eps
+        dn[idx] *= 1 - eps
+        fd = (J(beam=up) - J(beam=dn)) / (2 * eps * base[idx])
+        print(f"  param {k}: fd {fd:.8e} adjoint {res.grad[k]:.8e}",
+              flush=True)
+        assert abs(fd) > 0 and rel(res.grad[k], fd) < 1e-3
+
+    def scaled(f):
+        q = copy.deepcopy(w.problem)
+        q.material = slp_tables(k_scale=f)
+        return q
+    fd = (J(prob=scaled(1 + eps)) - J(prob=scaled(1 - eps))) / (2 * eps)
+    print(f"  param 1: fd {fd:.8e} adjoint {res.grad[1]:.8e}", flush=True)
+    assert abs(fd) > 0 and rel(res.grad[1], fd) < 1e-3
