@@ -20,10 +20,10 @@ void launch_cell(const CellArgs &a, const double *src, double *dst,
 template <int P, int NS, int PK>
 void launch_face(const FaceArgs &a, const double *src, double *dst,
                  unsigned n_members, hipStream_t s) {
-  const int block = 256;
-  const int64_t grid = (a.n_faces + block - 1) / block;
+  const int64_t per = FaceLanes<P>::faces_per_block;
+  const int64_t grid = (a.n_faces + per - 1) / per;
   hipLaunchKernelGGL((face_kernel<P, NS, PK>), dim3((unsigned)grid, n_members),
-                     dim3(block), 0, s, a, src, dst);
+                     dim3(256), 0, s, a, src, dst);
 }
 
 template void launch_cell<ADM_P, ADM_NS, ADM_PK>(const CellArgs &,

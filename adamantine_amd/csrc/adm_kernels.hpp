@@ -96,19 +96,44 @@ __device__ __forceinline__ double face_value(const MatConst &m, int mid,
   return -irc * (conv_h * (T - conv_T) + rad_h * (T - rad_T));
 }
 
-// Face kernel (source/ThermalOperator.templates.hh:619-751): one lane per
-// boundary face of the active domain that carries a convective or radiative
-// term; face values from the face's (p+1)^2 nodes, per-face-point powder
-// state, JxW-weighted integration, atomic accumulation into dst.
+// LDS ordering between the lanes of one wavefront
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Face kernel (source/ThermalOperator.templates.hh:619-751) over the
+// boundary faces of the active domain that carry a convective or radiative
+// term: (p+1)^2 lanes per face, 64 / (p+1)^2 faces per wavefront. Lane q
+// loads node q of its face, evaluates the temperature at face point q (the
+// x- then y-sweep of S, in the oracle's order, from the face's nodes shared
+// through LDS), the point's powder state and boundary value; the weighted
+// values are shared again and lane j integrates node j (S^T along y, then
+// x) and adds it to dst atomically.
+template <int P>
+struct FaceLanes {
+  static constexpr int N = P + 1;
+  static constexpr int lanes_per_face = N * N;
+  static constexpr int faces_per_wave = 64 / (N * N);
+  static constexpr int faces_per_block = 4 * faces_per_wave;
+};
+
 template <int P, int NS, int PK>
 __global__ void __launch_bounds__(256)
     face_kernel(const FaceArgs a, const double *__restrict__ src,
                 double *__restrict__ dst) {
   constexpr int N = P + 1, NF = N * N;
-  const int64_t fi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr int FPW = FaceLanes<P>::faces_per_wave;
+  __shared__ double xf[4][FPW * NF];
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const int jf = lane / NF, q = lane % NF;
+  const int qa = q % N, qb = q / N;
   const ADM_CONSTANT MemberArgs *mb =
       a.members ? as_constant(a.members) + blockIdx.y : nullptr;
-  if (fi >= (mb ? mb->n_faces : a.n_faces)) return;
+  const int64_t fi = ((int64_t)blockIdx.x * 4 + wave) * FPW + jf;
+  // whole faces (all lanes of a face) agree
+  if (jf >= FPW || fi >= (mb ? mb->n_faces : a.n_faces)) return;
   const KConst &kc = *(const KConst *)as_global(mb ? mb->kc : a.kc);
   const MatConst &m = kc.mat;
   const ADM_GLOBAL FaceRef *frp = as_global(mb ? mb->faces : a.faces) + fi;
@@ -131,37 +156,55 @@ __global__ void __launch_bounds__(256)
   const int64_t base = (int64_t)cc[0] * P + ns[1] * cc[1] * P +
                        ns[2] * cc[2] * P + (side ? P : 0) * ns[d];
   const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
-  int64_t node[NF];
-  double fu[NF], ft[NF], fv[NF];
+  // this lane's node (face-local index q = a + N b)
+  const int64_t node = base + qa * ns[t1] + qb * ns[t2];
+  double *fx = &xf[wave][jf * NF];
+  fx[q] = src[node];
+  wave_lds_sync();
+  // T at point (qa, qb): ft[b] = sum_k S[qa][k] u[k + N b], then along b
+  double ft[N];
 #pragma unroll
-  for (int j = 0; j < NF; ++j) {
-    node[j] = base + (j % N) * ns[t1] + (j / N) * ns[t2];
-    fu[j] = src[node[j]];
+  for (int bb = 0; bb < N; ++bb) {
+    double acc = kc.S[qa][0] * fx[N * bb];
+#pragma unroll
+    for (int k = 1; k < N; ++k)
+      acc = __builtin_fma(kc.S[qa][k], fx[k + N * bb], acc);
+    ft[bb] = acc;
   }
-  sweep<N, 0, false, false, 2>(kc.S, fu, ft);
-  sweep<N, 1, false, false, 2>(kc.S, ft, fv);
-  ADM_GLOBAL double *fp = NS == 3 ? face_powder + fi * NF : nullptr;
+  double T = kc.S[qb][0] * ft[0];
 #pragma unroll
-  for (int q = 0; q < NF; ++q) {
-    const double T = fv[q];
-    double powder = fp ? fp[q] : 0.0;
-    double liquid;
-    double r[NS];
-    if (state_ratios<NS>(m, mid, T, powder, liquid, r)) fp[q] = powder;
-    double val;
-    if constexpr (PK == 2) {
-      const MergedEval ev(m.merged[mid].B, &m.merged[mid].coef[0][0][0][0], T);
-      val = face_value<NS>(m, mid, bt, T, powder, ev, r);
-    } else {
-      const typename EvalOf<PK>::type ev(m, mid, T);
-      val = face_value<NS>(m, mid, bt, T, powder, ev, r);
-    }
-    fu[q] = val * kc.fjxw[d][q];
+  for (int k = 1; k < N; ++k) T = __builtin_fma(kc.S[qb][k], ft[k], T);
+  ADM_GLOBAL double *fp = NS == 3 ? face_powder + fi * NF + q : nullptr;
+  double powder = fp ? *fp : 0.0;
+  double liquid;
+  double r[NS];
+  if (state_ratios<NS>(m, mid, T, powder, liquid, r)) *fp = powder;
+  double val;
+  if constexpr (PK == 2) {
+    const MergedEval ev(m.merged[mid].B, &m.merged[mid].coef[0][0][0][0], T);
+    val = face_value<NS>(m, mid, bt, T, powder, ev, r);
+  } else {
+    const typename EvalOf<PK>::type ev(m, mid, T);
+    val = face_value<NS>(m, mid, bt, T, powder, ev, r);
   }
-  sweep<N, 1, true, false, 2>(kc.S, fu, ft);
-  sweep<N, 0, true, false, 2>(kc.S, ft, fv);
+  const double w = val * kc.fjxw[d][q];
+  wave_lds_sync();
+  fx[q] = w;
+  wave_lds_sync();
+  // node (qa, qb): S^T along y (ft[i] = sum_k S[k][qb] w[i + N k]), then
+  // along x
 #pragma unroll
-  for (int j = 0; j < NF; ++j) unsafeAtomicAdd(dst + node[j], fv[j]);
+  for (int i = 0; i < N; ++i) {
+    double acc = kc.S[0][qb] * fx[i];
+#pragma unroll
+    for (int k = 1; k < N; ++k)
+      acc = __builtin_fma(kc.S[k][qb], fx[i + N * k], acc);
+    ft[i] = acc;
+  }
+  double out = kc.S[0][qa] * ft[0];
+#pragma unroll
+  for (int i = 1; i < N; ++i) out = __builtin_fma(kc.S[i][qa], ft[i], out);
+  unsafeAtomicAdd(dst + node, out);
 }
 
 // dot product of row r of M with an N-line of V (same chain as sweep())
@@ -205,13 +248,6 @@ struct CellLanes {
   // exchange buffers per cell, N^3 doubles each
   static constexpr int buffers = 2;
 };
-
-// LDS ordering between the lanes of one wavefront
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Block-shared copies of the small constant tables (read by lane-varying
 // indices from LDS, which global stores cannot alias).
