@@ -541,10 +541,15 @@ void Context::rebuild_cells() {
   meta.assign(n_cells, 0);
   powder_slot.assign(n_cells, -1);
   n_powder_cells = 0;
+  nz_active = 0;
   faces.clear();
   for (int64_t c = 0; c < n_cells; ++c) {
     uint32_t w = 0;
-    if (active[c]) w |= kActive;
+    if (active[c]) {
+      w |= kActive;
+      nz_active = std::max(nz_active,
+                           (int)(c / ((int64_t)nc[0] * nc[1])) + 1);
+    }
     w |= (uint32_t)material[c] << kMaterialShift;
     if (active[c]) {
       const uint32_t fm = face_mask(c);
@@ -858,13 +863,17 @@ FaceArgs Context::face_args() const {
 void Context::apply(const double *src, double *dst, int add, hipStream_t s) {
   CellArgs a = cell_args();
   a.add = add;
-  launch(a, face_args(), src, dst, 1, s);
+  launch(a, face_args(), src, dst, 1, s, nz_active);
 }
 
 // Cell and face kernels of the context's configuration; with n_members > 1
 // the arguments carry the member table and the grid's y index is the member.
 void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
-                     double *dst, unsigned n_members, hipStream_t s) const {
+                     double *dst, unsigned n_members, hipStream_t s,
+                     int nz_rows) const {
+  // cell layers above the highest active one hold no active cell
+  const int lim[3] = {nc[0], nc[1],
+                      nz_rows >= 0 ? std::min(nz_rows, nc[2]) : nc[2]};
   const int pk = property_kind();
   const size_t lds = pk == 2 ? sizeof(double) * kMergedLds * kc.mat.n_materials
                              : 0;
@@ -880,7 +889,8 @@ void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
     int64_t cells = 1;
     for (int d = 0; d < 3; ++d) {
       const int b = use_coloring ? (col >> d) & 1 : 0;
-      ac.lext[d] = use_coloring ? (nc[d] > b ? (nc[d] - b + 1) / 2 : 0) : nc[d];
+      ac.lext[d] =
+          use_coloring ? (lim[d] > b ? (lim[d] - b + 1) / 2 : 0) : lim[d];
       cells *= ac.lext[d];
     }
     if (cells == 0) continue;
@@ -1128,7 +1138,9 @@ void Ensemble::apply(const double *src, double *dst, int add, hipStream_t s) {
   fa.members = d_members;
   fa.member_stride = n_dofs;
   fa.n_faces = max_faces;
-  c0.launch(a, fa, src, dst, (unsigned)n_members, s);
+  int nz = 0;
+  for (const auto &m : members) nz = std::max(nz, m->nz_active);
+  c0.launch(a, fa, src, dst, (unsigned)n_members, s, nz);
   ADM_HIP(hipGetLastError());
 }
 

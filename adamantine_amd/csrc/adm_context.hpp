@@ -73,8 +73,11 @@ struct Context {
   void build_merged_tables();
   int property_kind() const;
   void apply(const double *src, double *dst, int add, hipStream_t s);
+  // nz_rows: cell layers (from z = 0) that can hold an active cell; the
+  // launches cover only those (-1: all)
   void launch(const CellArgs &a, const FaceArgs &fa, const double *src,
-              double *dst, unsigned n_members, hipStream_t s) const;
+              double *dst, unsigned n_members, hipStream_t s,
+              int nz_rows = -1) const;
   void fill(double *x, double v, int64_t n, hipStream_t s);
   void compute_inverse_mass(double *out, hipStream_t s);
   void ensure_step_buffers(int method);
@@ -95,6 +98,8 @@ struct Context {
   std::vector<uint32_t> meta;
   std::vector<int32_t> powder_slot;
   int64_t n_powder_cells = 0;
+  // 1 + the highest cell layer (z index) holding an active cell, 0 if none
+  int nz_active = 0;
   std::vector<FaceRef> faces;
 
   std::vector<BeamData> beams;

@@ -258,3 +258,73 @@ def adjoint_workload(n_cells=(64, 32, 16), extent=(4 * MM, 2 * MM, 1 * MM),
     return AdjointWorkload(p, truth, [0.32, b.max_power, 1.1 * b.radius,
                                       b.depth], obs_dof, noise, dt, n_steps,
                            checkpoint_every)
+
+
+@dataclass
+class DepositionWorkload:
+    base: Problem
+    members: list
+    activation_step: np.ndarray   # [n_members, n_layers] step of each layer
+    layer_boxes: np.ndarray       # [n_layers, 6] deposition box per layer
+    dt: float
+    n_steps: int
+
+
+def deposition_workload(n_members: int = 32, n_cells=(96, 96, 40),
+                        extent=(3 * MM, 3 * MM, 1.25 * MM), n_layers: int = 10,
+                        steps_per_layer: int = 200, dt: float = 2e-7,
+                        n_steps: int = 2000) -> DepositionWorkload:
+    """The deposition workload: ``n_members`` members of a 96x96x40 Q2 box
+    (31.25 um cells) built layer by layer, the material model of
+    ``plain_large``, an electron beam of 1500 W, depth 150 um, radius 150 um.
+
+    Member j draws beam_0.absorption_efficiency ~ N(0.85, 0.03) with the
+    reference's scheme. Layer k (k = 0 .. n_layers-1) holds cell layers
+    4k .. 4k+3; layer 0 is active at t = 0 and layer k >= 1 is deposited at
+    step steps_per_layer * k + delta_jk with delta ~ U{0..100}
+    (numpy RandomState(7), drawn member-major), by a deposition box covering
+    the layer. The beam rasters along x at 2 m/s at mid-y on the nominal top
+    of the last deposited layer (layer k during steps [200k, 200(k+1)))."""
+    from .ensemble import Member, draw_member_values
+    from .problem import SOURCE_ELECTRON_BEAM as EB
+    nz = n_cells[2]
+    hz = extent[2] / nz
+    cells_per_layer = nz // n_layers
+    m = slp_tables()
+    # raster: one track along x per layer window, z on that layer's top
+    speed = 2.0
+    x0, y0 = 0.25 * MM, 0.5 * extent[1]
+    pts = [{"mode": "point", "xyz": (x0, y0, cells_per_layer * hz),
+            "pmod": 0.0, "param": 1e-9}]
+    x = x0
+    for k in range(n_layers):
+        z = (k + 1) * cells_per_layer * hz
+        if k > 0:
+            pts.append({"mode": "point", "xyz": (x, y0, z), "pmod": 0.0,
+                        "param": 1e-9})
+        x = x + speed * steps_per_layer * dt
+        pts.append({"mode": "line", "xyz": (x, y0, z), "pmod": 1.0,
+                    "param": speed})
+    beam = Beam(type=EB, absorption_efficiency=0.85, max_power=1500.0,
+                radius=150 * UM, depth=150 * UM,
+                segments=segments_from_path(pts))
+    p = Problem(n_cells=tuple(n_cells), extent=tuple(extent), fe_degree=2,
+                quadrature=0, dim=3, material=m,
+                boundary_types=[BC_ADIABATIC] * 5 +
+                [BC_RADIATIVE | BC_CONVECTIVE],
+                beams=[beam], method=FORWARD_EULER, initial_temperature=300.0)
+    z = p.cell_centers()[:, -1]
+    active0 = (z < cells_per_layer * hz).astype(np.uint8)
+    vals = draw_member_values([("beam_0.absorption_efficiency", 0.85, 0.03)],
+                              n_members)
+    members = [Member(absorption_efficiency=[vals[
+        "beam_0.absorption_efficiency"][j]], max_power=[1500.0],
+        cell_active=active0.copy()) for j in range(n_members)]
+    delta = np.random.RandomState(7).randint(0, 101, (n_members, n_layers))
+    steps = steps_per_layer * np.arange(n_layers)[None, :] + delta
+    steps[:, 0] = 0
+    boxes = np.array([[0.0, 0.0, (k * cells_per_layer + 0.25) * hz,
+                       extent[0], extent[1],
+                       ((k + 1) * cells_per_layer - 0.25) * hz]
+                      for k in range(n_layers)])
+    return DepositionWorkload(p, members, steps, boxes, dt, n_steps)
