@@ -11,8 +11,7 @@ void launch_cell(const CellArgs &a, const double *src, double *dst,
                  hipStream_t s) {
   constexpr int cpb = 4 * CellLanes<P>::cells_per_wave;
   const unsigned grid = (unsigned)n_blocks;
-  const size_t xbytes = sizeof(double) * cpb * CellLanes<P>::buffers *
-                        (P + 1) * (P + 1) * (P + 1);
+  const size_t xbytes = sizeof(double) * cpb * CellLanes<P>::stride;
   hipLaunchKernelGGL((cell_kernel<P, NS, PK>), dim3(grid, n_members), dim3(256),
                      lds_bytes + xbytes, s, a, src, dst);
 }

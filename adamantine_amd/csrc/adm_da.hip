@@ -515,9 +515,20 @@ void ensemble_analysis(Ensemble &ens, double *T, int n_params, double *params,
   ADM_HIP_DA(hipFree(d_oc));
 }
 
+// T[idx[k]] = v[k] (distinct indices)
+__global__ void gather_set_kernel(double *__restrict__ T,
+                                  const int64_t *__restrict__ idx,
+                                  const double *__restrict__ v, int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) T[idx[k]] = v[k];
+}
+
 // get_expt_to_dof_mapping (source/experimental_data_utils.cc:70-110): the
 // nearest support point among the nodes of active cells. Equidistant nodes
-// resolve to the lowest node index.
+// resolve to the lowest node index. The search scans index boxes around the
+// point's nearest grid index, in increasing node order, and widens the box
+// until it holds every node closer than the best one found: the result is
+// the one of a scan over all nodes.
 void nearest_dofs(const Context &c, int64_t np, const double *pts,
                   int64_t *out) {
   const NodeGeometry geo(c);
@@ -534,19 +545,56 @@ void nearest_dofs(const Context &c, int64_t np, const double *pts,
                       (size_t)c.nn[0] * ((cy * c.p + j) +
                                          (size_t)c.nn[1] * (cz * c.p + k))] = 1;
   }
+  // smallest node spacing per axis
+  double gap = 1.0;
+  for (int k = 0; k < c.p; ++k) gap = std::min(gap, geo.gl[k + 1] - geo.gl[k]);
   for (int64_t q = 0; q < np; ++q) {
     const double *x = pts + 3 * q;
+    int64_t i0[3];
+    for (int d = 0; d < 3; ++d) {
+      const double f = (x[d] - geo.origin[d]) / geo.h[d] * c.p;
+      const double cl =
+          f == f ? std::min(std::max(f, 0.0), (double)(c.nn[d] - 1)) : 0.0;
+      i0[d] = (int64_t)std::llround(cl);
+    }
+    int64_t r = 1;
     double best = DBL_MAX;
     int64_t best_i = -1;
-    for (int64_t i = 0; i < c.n_dofs; ++i) {
-      if (!node_active[(size_t)i]) continue;
-      double xi[3];
-      geo.coords(i, xi);
-      const double d = dist3(x, xi);
-      if (d < best) {
-        best = d;
-        best_i = i;
+    for (;;) {
+      best = DBL_MAX;
+      best_i = -1;
+      int64_t lo[3], hi[3];
+      bool whole = true;
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = std::max<int64_t>(i0[d] - r, 0);
+        hi[d] = std::min<int64_t>(i0[d] + r, c.nn[d] - 1);
+        whole = whole && lo[d] == 0 && hi[d] == c.nn[d] - 1;
       }
+      for (int64_t iz = lo[2]; iz <= hi[2]; ++iz)
+        for (int64_t iy = lo[1]; iy <= hi[1]; ++iy)
+          for (int64_t ix = lo[0]; ix <= hi[0]; ++ix) {
+            const int64_t i = ix + c.nn[0] * (iy + (int64_t)c.nn[1] * iz);
+            if (!node_active[(size_t)i]) continue;
+            double xi[3];
+            geo.coords(i, xi);
+            const double dd = dist3(x, xi);
+            if (dd < best) {
+              best = dd;
+              best_i = i;
+            }
+          }
+      if (whole) break;
+      if (best_i < 0) {
+        r *= 2;
+        continue;
+      }
+      // every node within `best` of x lies within R index steps of i0
+      int64_t R = 0;
+      for (int d = 0; d < 3; ++d)
+        R = std::max<int64_t>(
+            R, (int64_t)std::ceil(best / (geo.h[d] * gap)) + 1);
+      if (R <= r) break;
+      r = R;
     }
     out[q] = best_i;
   }
@@ -640,6 +688,55 @@ int adm_ensemble_nearest_dofs(void *ens, int32_t member, int64_t n_points,
   if (member < 0 || member >= e.n_members)
     return adm::set_error(ADM_ERR_INVALID, "member index out of range");
   adm::nearest_dofs(*e.members[member], n_points, points, dofs);
+  ADM_DA_END
+}
+
+int adm_nearest_dofs(void *ctx, int64_t n_points, const double *points,
+                     int64_t *dofs) {
+  ADM_DA_BEGIN
+  if (!ctx || (!points && n_points) || (!dofs && n_points))
+    return adm::set_error(ADM_ERR_INVALID, "null argument");
+  adm::nearest_dofs(*static_cast<adm::Context *>(ctx), n_points, points, dofs);
+  ADM_DA_END
+}
+
+int adm_set_values(void *ctx, int64_t n, const int64_t *dofs,
+                   const double *values, double *T, void *stream) {
+  ADM_DA_BEGIN
+  if (!ctx || !T || (n && (!dofs || !values)))
+    return adm::set_error(ADM_ERR_INVALID, "null argument");
+  adm::Context &c = *static_cast<adm::Context *>(ctx);
+  // in input order, so a repeated DoF keeps its last value
+  std::vector<int64_t> last((size_t)c.n_dofs, -1);
+  for (int64_t k = 0; k < n; ++k) {
+    if (dofs[k] < 0 || dofs[k] >= c.n_dofs)
+      return adm::set_error(ADM_ERR_INVALID, "DoF index out of range");
+    last[(size_t)dofs[k]] = k;
+  }
+  std::vector<int64_t> idx;
+  std::vector<double> val;
+  for (int64_t k = 0; k < n; ++k)
+    if (last[(size_t)dofs[k]] == k) {
+      idx.push_back(dofs[k]);
+      val.push_back(values[k]);
+    }
+  if (idx.empty()) return ADM_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int64_t *d_idx = nullptr;
+  double *d_val = nullptr;
+  ADM_HIP_DA(hipMalloc(&d_idx, sizeof(int64_t) * idx.size()));
+  ADM_HIP_DA(hipMalloc(&d_val, sizeof(double) * idx.size()));
+  ADM_HIP_DA(hipMemcpyAsync(d_idx, idx.data(), sizeof(int64_t) * idx.size(),
+                            hipMemcpyHostToDevice, s));
+  ADM_HIP_DA(hipMemcpyAsync(d_val, val.data(), sizeof(double) * idx.size(),
+                            hipMemcpyHostToDevice, s));
+  const int64_t m = (int64_t)idx.size();
+  hipLaunchKernelGGL(adm::gather_set_kernel, dim3((unsigned)((m + 255) / 256)),
+                     dim3(256), 0, s, T, d_idx, d_val, m);
+  ADM_HIP_DA(hipGetLastError());
+  ADM_HIP_DA(hipStreamSynchronize(s));
+  ADM_HIP_DA(hipFree(d_idx));
+  ADM_HIP_DA(hipFree(d_val));
   ADM_DA_END
 }
 

@@ -245,8 +245,11 @@ struct CellLanes {
   static constexpr int N = P + 1;
   static constexpr int lanes_per_cell = N * N;
   static constexpr int cells_per_wave = 64 / (N * N);
-  // exchange buffers per cell, N^3 doubles each
+  // exchange buffers per cell, N^3 doubles each; consecutive cells'
+  // buffers are 5 doubles apart (fewer LDS bank conflicts: 3 % less
+  // operator time than no gap on the 128x64x32 Q2 workload)
   static constexpr int buffers = 2;
+  static constexpr int stride = buffers * N * N * N + 5;
 };
 
 // Block-shared copies of the small constant tables (read by lane-varying
@@ -368,7 +371,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
   const double xqa = sk.xq[la], xqb = sk.xq[lb];
   const int own = N * la + NL * lb;  // lane's slot in a [z][y][x] buffer
   double *xb = lds + (PK == 2 ? n_mat * kMergedLds : 0) +
-               (wave * CPW + (j < CPW ? j : 0)) * (CellLanes<P>::buffers * N3);
+               (wave * CPW + (j < CPW ? j : 0)) * CellLanes<P>::stride;
   double *xb2 = xb + N3;
   const int64_t sy = a.nn[0], sz = (int64_t)a.nn[0] * a.nn[1];
   const double *mBall = lds;

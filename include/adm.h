@@ -411,6 +411,19 @@ int adm_da_fill_noise(void *da, int32_t count, int32_t n_obs, const double *R,
 int adm_ensemble_nearest_dofs(void *ens, int32_t member, int64_t n_points,
                               const double *points, int64_t *dofs);
 
+/* get_expt_to_dof_mapping (source/experimental_data_utils.cc:70-110) for one
+ * operator: the nearest support point, among nodes of active cells, of each
+ * point (host points[n_points][3] -> host dofs[n_points], -1 if no cell is
+ * active). Equidistant candidates resolve to the lowest node index. */
+int adm_nearest_dofs(void *ctx, int64_t n_points, const double *points,
+                     int64_t *dofs);
+
+/* set_with_experimental_data (source/experimental_data_utils.cc:112-137):
+ * T[dofs[k]] = values[k] in input order (a repeated DoF keeps its last
+ * value). dofs, values: host, n entries; T: device. */
+int adm_set_values(void *ctx, int64_t n, const int64_t *dofs,
+                   const double *values, double *T, void *stream);
+
 /* DataAssimilator::update_ensemble (source/DataAssimilator.cc:95-189) on the
  * augmented state [T; parameters]: with d_j = noise_j + (obs - H x_j) and P
  * the localised sample covariance (calc_sample_covariance_sparse, :690-752),
