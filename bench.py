@@ -26,7 +26,7 @@ FP64_PEAK_TFLOPS = 78.6
 # HBM bytes of one operator application of this workload (8 color launches
 # of the cell kernel + the face kernel), from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs scaled by a 256 MiB copy (profiles/hbm_traffic_plain_large.md)
-MEASURED_TRAFFIC_BYTES = 290.5e6
+MEASURED_TRAFFIC_BYTES = 290.4e6
 
 
 def algorithmic_bytes(problem, n_powder_qp: int, n_active_cells: int) -> int:
