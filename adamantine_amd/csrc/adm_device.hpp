@@ -272,6 +272,10 @@ struct MergedEval {
     j = k;
     d = T - B[k > 0 ? k - 1 : 0];
   }
+  // interval already found (merged_interval)
+  __device__ __forceinline__ MergedEval(const double *B, const double *coef_,
+                                        double T, int k)
+      : coef(coef_), j(k), d(T - B[k > 0 ? k - 1 : 0]) {}
   template <int NS>
   __device__ __forceinline__ double operator()(int prop,
                                                const double (&r)[NS]) const {
@@ -284,6 +288,20 @@ struct MergedEval {
     return value;
   }
 };
+
+// The interval index of MergedEval for M temperatures at once: the M
+// searches advance together, so their table reads overlap.
+template <int M>
+__device__ __forceinline__ void merged_interval(const double *B,
+                                                const double (&T)[M],
+                                                int (&k)[M]) {
+#pragma unroll
+  for (int i = 0; i < M; ++i) k[i] = 0;
+#pragma unroll
+  for (int step = kMergedMax / 2; step >= 1; step >>= 1)
+#pragma unroll
+    for (int i = 0; i < M; ++i) k[i] += (B[k[i] + step - 1] <= T[i]) ? step : 0;
+}
 
 // update_state_ratios for one quadrature point. Returns true if the powder
 // ratio changed.

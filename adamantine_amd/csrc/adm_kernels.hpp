@@ -491,6 +491,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
       }
       const double py = org[1] + (rc.cc[1] + xqa) * hh[1];
       const double pz = org[2] + (rc.cc[2] + xqb) * hh[2];
+      int kint[N];
+      if constexpr (PK == 2) merged_interval<N>(mB, V, kint);
       double w0[N], w1[N], w2[N], val[N];
       bool line_liquid = false;
 #pragma unroll
@@ -542,7 +544,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
           }
         };
         if constexpr (PK == 2)
-          props(MergedEval(mB, mC, T));
+          props(MergedEval(mB, mC, T, kint[q]));
         else
           props(typename EvalOf<PK>::type(m, mid, T));
         double fx, fy;
