@@ -692,8 +692,7 @@ inline void launch_vjp(Context &c, const double *Y, const double *pw,
   std::memset(&aa, 0, sizeof(aa));
   aa.c = c.cell_args();
   aa.c.color = -1;
-  const int cpw = c.p == 1 ? CellLanes<1>::cells_per_wave
-                           : CellLanes<2>::cells_per_wave;
+  const int cpw = cells_per_wave(c.p);
   for (int d = 0; d < 3; ++d) aa.c.lext[d] = c.nc[d];
   aa.c.nseg = (c.nc[0] + cpw - 1) / cpw;
   aa.c.n_rows = aa.c.nseg * c.nc[1] * c.nc[2];
@@ -707,8 +706,7 @@ inline void launch_vjp(Context &c, const double *Y, const double *pw,
   const unsigned blocks = (unsigned)((aa.c.n_rows + 3) / 4);
   const size_t lds =
       (pk == 2 ? sizeof(double) * kMergedLds * c.kc.mat.n_materials : 0) +
-      sizeof(double) * 4 * (c.p == 1 ? CellLanes<1>::cells_per_wave
-                                     : CellLanes<2>::cells_per_wave) *
+      sizeof(double) * 4 * cpw *
           4 * (c.p + 1) * (c.p + 1) * (c.p + 1);
   FaceArgs fa = c.face_args();
   const unsigned face_blocks = (unsigned)((fa.n_faces + 255) / 256);
@@ -735,6 +733,12 @@ inline void launch_vjp(Context &c, const double *Y, const double *pw,
   ADM_ADJ(2, 1, 2)
   ADM_ADJ(2, 2, 2)
   ADM_ADJ(2, 3, 2)
+  ADM_ADJ(3, 1, 0)
+  ADM_ADJ(3, 2, 0)
+  ADM_ADJ(3, 3, 0)
+  ADM_ADJ(3, 1, 2)
+  ADM_ADJ(3, 2, 2)
+  ADM_ADJ(3, 3, 2)
 #undef ADM_ADJ
   if (!launched)
     throw AdmError(ADM_ERR_UNSUPPORTED,

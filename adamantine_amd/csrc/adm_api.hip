@@ -287,9 +287,9 @@ Context::Context(const adm_problem_desc &d) {
   const adm_mesh_desc &m = d.mesh;
   if (m.dim != 3)
     throw AdmError(ADM_ERR_UNSUPPORTED, "the device path is three-dimensional");
-  if (m.fe_degree < 1 || m.fe_degree > 2)
+  if (m.fe_degree < 1 || m.fe_degree > 3)
     throw AdmError(ADM_ERR_UNSUPPORTED,
-                   "fe_degree must be 1 or 2 on the device path");
+                   "fe_degree must be 1, 2 or 3 on the device path");
   if (m.quadrature != 0 && m.quadrature != 1)
     throw AdmError(ADM_ERR_INVALID, "quadrature must be 0 (Gauss) or 1 "
                                     "(Gauss-Lobatto)");
@@ -881,8 +881,7 @@ void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
   // the cell kernel runs once per parity color (cells of one color share no
   // node: plain read-add-write of dst), or once with atomics
   const int n_colors = use_coloring ? 8 : 1;
-  const int cpw = p == 1 ? CellLanes<1>::cells_per_wave
-                         : CellLanes<2>::cells_per_wave;
+  const int cpw = cells_per_wave(p);
   for (int col = 0; col < n_colors; ++col) {
     CellArgs ac = a;
     ac.color = use_coloring ? col : -1;
@@ -913,6 +912,9 @@ void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
     ADM_DISPATCH(2, 1)
     ADM_DISPATCH(2, 2)
     ADM_DISPATCH(2, 3)
+    ADM_DISPATCH(3, 1)
+    ADM_DISPATCH(3, 2)
+    ADM_DISPATCH(3, 3)
 #undef ADM_DISPATCH
   }
   if (fa.n_faces) {
@@ -931,6 +933,9 @@ void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
     ADM_DISPATCH(2, 1)
     ADM_DISPATCH(2, 2)
     ADM_DISPATCH(2, 3)
+    ADM_DISPATCH(3, 1)
+    ADM_DISPATCH(3, 2)
+    ADM_DISPATCH(3, 3)
 #undef ADM_DISPATCH
   }
   ADM_HIP(hipGetLastError());
@@ -949,8 +954,11 @@ void Context::compute_inverse_mass(double *out, hipStream_t s) {
   if (p == 1)
     hipLaunchKernelGGL(inverse_mass_kernel<1>, dim3(grid), dim3(block), 0, s,
                        d_meta, d_wgl, det, nc[0], nc[1], nc[2], n_dofs, out);
-  else
+  else if (p == 2)
     hipLaunchKernelGGL(inverse_mass_kernel<2>, dim3(grid), dim3(block), 0, s,
+                       d_meta, d_wgl, det, nc[0], nc[1], nc[2], n_dofs, out);
+  else
+    hipLaunchKernelGGL(inverse_mass_kernel<3>, dim3(grid), dim3(block), 0, s,
                        d_meta, d_wgl, det, nc[0], nc[1], nc[2], n_dofs, out);
   ADM_HIP(hipGetLastError());
 }
@@ -1468,6 +1476,23 @@ int adm_ensemble_get_qp_state(void *ptr, int32_t member, double *liquid,
     return adm::set_error(ADM_ERR_INVALID, "member index out of range");
   ADM_HIP(hipDeviceSynchronize());
   ens.members[member]->get_qp_state(liquid, powder);
+  ADM_API_END
+}
+
+int adm_ensemble_set_beam_gain(void *ptr, int32_t member, int32_t beam,
+                               double absorption_efficiency,
+                               double max_power) {
+  ADM_API_BEGIN
+  ADM_ENS(ptr)
+  if (member < 0 || member >= ens.n_members)
+    return adm::set_error(ADM_ERR_INVALID, "member index out of range");
+  adm::Context &c = *ens.members[member];
+  if (beam < 0 || beam >= (int)c.beams.size())
+    return adm::set_error(ADM_ERR_INVALID, "beam index out of range");
+  ADM_HIP(hipDeviceSynchronize());
+  c.beams[beam].desc.absorption_efficiency = absorption_efficiency;
+  c.beams[beam].desc.max_power = max_power;
+  ens.refresh_members();
   ADM_API_END
 }
 

@@ -96,6 +96,11 @@ __device__ __forceinline__ double face_value(const MatConst &m, int mid,
   return -irc * (conv_h * (T - conv_T) + rad_h * (T - rad_T));
 }
 
+// cells per wavefront of the cell kernel for degree p
+inline int cells_per_wave(int p) {
+  return 64 / ((p + 1) * (p + 1));
+}
+
 // LDS ordering between the lanes of one wavefront
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -285,7 +290,8 @@ __device__ __forceinline__ RowCell row_cell(const CellArgs &a, int r, int j,
 }
 
 template <int P, int NS, int PK>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(P == 3 ? 2 : 3)))
     cell_kernel(const CellArgs a, const double *__restrict__ src,
                 double *__restrict__ dst) {
   constexpr int N = P + 1, NL = N * N, N3 = N * N * N;

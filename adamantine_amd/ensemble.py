@@ -78,6 +78,8 @@ def _bind(L):
                                                  _dp, _ip32]
     L.adm_ensemble_add_material.argtypes = [_vp, C.c_int32, _ip32, _dp, _dp,
                                             C.c_double, _vp, _vp]
+    L.adm_ensemble_set_beam_gain.argtypes = [_vp, C.c_int32, C.c_int32,
+                                             C.c_double, C.c_double]
     L._ens_bound = True
     return L
 
@@ -283,6 +285,15 @@ class EnsembleDevice:
             None if solution is None else _dev_ptr(solution),
             _stream_ptr(stream)))
         self._inverse_mass = None
+
+    def set_beam_parameters(self, member: int, beam: int,
+                            absorption_efficiency: float, max_power: float):
+        """update_physics_parameters (source/ThermalPhysics.templates.hh:
+        827-852) of one member: absorption efficiency and max power of a
+        beam (radius, depth and path are shared by the members)."""
+        check(_lib().adm_ensemble_set_beam_gain(
+            self._h, int(member), int(beam), float(absorption_efficiency),
+            float(max_power)))
 
     def qp_state(self, member: int):
         liq = np.zeros((self.n_cells, self.n_q))

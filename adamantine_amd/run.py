@@ -1,0 +1,265 @@
+"""Ensemble simulation with data assimilation from an adamantine input file.
+
+`run_ensemble` is the time loop of the reference's run_ensemble
+(application/adamantine.hh:1961-2433) for the thermal physics on one
+structured box, on the device:
+  - the members are drawn by create_database_ensemble
+    (source/ensemble_management.cc:166-265);
+  - the whole ensemble is stepped by one batched call per time step (the
+    reference steps member after member, :2137-2142);
+  - experimental frames are read at their time stamps and assimilated with
+    the EnKF, with the reference's frame cadence (:2150-2361).
+Mesh refinement, material deposition, checkpoints, output and mechanics of
+the reference's loop are not part of this path.
+"""
+from __future__ import annotations
+
+import math
+import os
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from .problem import (FORWARD_EULER, RK_FOURTH_ORDER, RK_THIRD_ORDER,
+                      STATE_NAMES, STATE_PROPERTY_NAMES, Problem, _get,
+                      beams_from_database, boundary_from_database,
+                      material_from_database, read_info)
+
+_METHODS = {"forward_euler": FORWARD_EULER, "rk_third_order": RK_THIRD_ORDER,
+            "rk_fourth_order": RK_FOURTH_ORDER}
+
+
+def material_template_parameters(materials: Dict):
+    """get_material_template_parameters (application/adamantine.cc:
+    155-241): number of material states (highest state with a property, plus
+    one) and polynomial order (most coefficients minus one; 0 for tables)."""
+    use_table = _get(materials, "property_format") == "table"
+    n_states = 0
+    p_order = 0
+    for mid in range(int(_get(materials, "n_materials"))):
+        mdb = materials.get("material_%d" % mid)
+        if not isinstance(mdb, dict):
+            continue
+        for s, sname in enumerate(STATE_NAMES):
+            sdb = mdb.get(sname)
+            if not isinstance(sdb, dict):
+                continue
+            for pname in STATE_PROPERTY_NAMES:
+                if pname in sdb:
+                    n_states = max(n_states, s + 1)
+                    if not use_table:
+                        n = len("".join(str(sdb[pname]).split()).split(","))
+                        p_order = max(p_order, n - 1)
+    if not (0 < n_states < 4) or p_order > 4:
+        raise ValueError("Error when computing the material parameters")
+    return n_states, p_order
+
+
+def problem_from_database(db: Dict, base_dir: str = ".",
+                          fe_degree: Optional[int] = None) -> Problem:
+    """The structured-box problem that ThermalPhysics::setup builds from an
+    input database (geometry source/Geometry.cc:186-352, boundary
+    source/Boundary.cc:12-70, materials, sources, active cells
+    source/ThermalPhysics.templates.hh:472-493)."""
+    g = db["geometry"]
+    if str(_get(g, "import_mesh", "false")) == "true":
+        raise NotImplementedError("imported meshes are not on the device path")
+    dim = int(_get(g, "dim", 3))
+    if dim != 3:
+        raise NotImplementedError("the device path is three-dimensional")
+    if str(_get(g, "material_deposition", "false")) == "true":
+        raise NotImplementedError("deposition from the input file: use "
+                                  "cells_to_activate / add_material")
+    # axis x = length, y = width, z = height (Geometry.cc:188-243)
+    n_cells = (int(_get(g, "length_divisions", 10)),
+               int(_get(g, "width_divisions", 10)),
+               int(_get(g, "height_divisions", 10)))
+    origin = (float(_get(g, "length_origin", 0.0)),
+              float(_get(g, "width_origin", 0.0)),
+              float(_get(g, "height_origin", 0.0)))
+    extent = (float(g["length"]), float(g["width"]), float(g["height"]))
+    mats = db["materials"]
+    n_states, p_order = material_template_parameters(mats)
+    m = material_from_database(mats, n_states, p_order)
+    disc = _get(db, "discretization.thermal", {})
+    deg = int(fe_degree if fe_degree is not None else disc.get("fe_degree", 1))
+    quad = 1 if str(disc.get("quadrature", "gauss")) == "lobatto" else 0
+    ts = db["time_stepping"]
+    p = Problem(n_cells=n_cells, extent=extent, origin=origin, fe_degree=deg,
+                quadrature=quad, dim=3, material=m,
+                boundary_types=boundary_from_database(db.get("boundary", {}),
+                                                      6),
+                beams=beams_from_database(db["sources"], base_dir,
+                                          db.get("units")),
+                method=_METHODS[str(ts["method"])],
+                initial_temperature=float(_get(mats, "initial_temperature",
+                                               300.0)))
+    height = float(_get(g, "material_height", 1e9))
+    if height < origin[2] + extent[2]:
+        p.set_material_height(height)
+    if str(_get(g, "use_powder", "false")) == "true":
+        p.set_powder_layer(height, float(g["powder_layer"]))
+    elif n_states == 3 or n_states == 2:
+        st = np.zeros((p.n_cells_total, n_states))
+        st[:, 0] = 1.0
+        p.cell_state = st
+    return p
+
+
+def _stddev_keys(tree: Dict, path: str = "") -> List[str]:
+    """Leaves of the ensemble database in input order
+    (source/ensemble_management.cc:166-211), skipping the two switches."""
+    out = []
+    for k, v in tree.items():
+        full = k if not path else path + "." + k
+        if isinstance(v, dict) and v:
+            out += _stddev_keys(v, full)
+        elif full not in ("ensemble_simulation", "ensemble_size"):
+            out.append(full)
+    return out
+
+
+@dataclass
+class EnsembleRun:
+    solution: object                    # [n_members, n_dofs] device tensor
+    params: np.ndarray                  # [n_members, n_augmented]
+    param_names: List[str]
+    member_values: Dict[str, np.ndarray]
+    times: List[float] = field(default_factory=list)  # DA times
+    n_steps: int = 0
+    time: float = 0.0
+
+
+def run_ensemble(db: Dict, base_dir: str = ".",
+                 fe_degree: Optional[int] = None,
+                 verbose: bool = False) -> EnsembleRun:
+    """run_ensemble for the thermal physics on the device."""
+    import torch
+    from .ensemble import (DataAssimilator, EnsembleDevice, Member,
+                           draw_member_values)
+    from .experimental_data import PointCloud, read_frame_timestamps
+    p = problem_from_database(db, base_dir, fe_degree)
+    ens_db = db.get("ensemble", {})
+    n = int(ens_db.get("ensemble_size", 5))
+    keys = []
+    for k in _stddev_keys(ens_db):
+        if not k.endswith("_stddev"):
+            raise ValueError("ensemble key %s is not a standard deviation" % k)
+        base = k[:-len("_stddev")]
+        keys.append((base, float(_get(db, base)), float(_get(ens_db, k))))
+    vals = draw_member_values(keys, n)
+    n_beams = len(p.beams)
+    T0 = np.full(n, p.initial_temperature)
+    members = []
+    for j in range(n):
+        eta = [b.absorption_efficiency for b in p.beams]
+        power = [b.max_power for b in p.beams]
+        for key, v in vals.items():
+            parts = key.split(".")
+            if key == "materials.initial_temperature":
+                T0[j] = v[j]
+            elif (len(parts) == 3 and parts[0] == "sources" and
+                  parts[1].startswith("beam_") and
+                  parts[2] in ("absorption_efficiency", "max_power")):
+                b = int(parts[1][5:])
+                if b >= n_beams:
+                    raise ValueError("no heat source %s" % parts[1])
+                (eta if parts[2] == "absorption_efficiency" else power)[b] = \
+                    float(v[j])
+            else:
+                raise NotImplementedError(
+                    "perturbation of %s is not on the device path" % key)
+        members.append(Member(absorption_efficiency=eta, max_power=power))
+    ens = EnsembleDevice(p, members)
+    ens.compute_inverse_mass_matrix()
+    sol = ens.initialize_dof_vector(0.0)
+    sol += torch.tensor(T0, dtype=torch.float64, device=sol.device)[:, None]
+
+    # augmented state (application/adamantine.hh:1600-1700)
+    da_db = db.get("data_assimilation", {})
+    names = []
+    if str(da_db.get("augment_with_beam_0_absorption", "false")) == "true":
+        names.append("beam_0_absorption")
+    if str(da_db.get("augment_with_beam_0_max_power", "false")) == "true":
+        names.append("beam_0_max_power")
+    params = np.zeros((n, len(names)))
+    for j, m in enumerate(members):
+        for i, nm in enumerate(names):
+            params[j, i] = (m.absorption_efficiency[0]
+                            if nm == "beam_0_absorption" else m.max_power[0])
+
+    # experimental data
+    ex = db.get("experiment", {})
+    stamps, cloud = None, None
+    if str(ex.get("read_in_experimental_data", "false")) == "true":
+        if str(ex.get("format", "point_cloud")).lower() != "point_cloud":
+            raise NotImplementedError("ray-tracing experimental data")
+        stamps = read_frame_timestamps(
+            os.path.join(base_dir, str(ex["log_filename"])),
+            int(ex["first_camera_id"]), int(ex["last_camera_id"]),
+            int(ex["last_frame"]), int(ex.get("first_frame", 0)),
+            float(ex.get("first_frame_temporal_offset", 0.0)))
+        if not stamps or not stamps[0]:
+            raise ValueError("the experimental log shows no frame")
+        cloud = PointCloud(os.path.join(base_dir, str(ex["file"])),
+                           int(ex["first_camera_id"]),
+                           int(ex["last_camera_id"]),
+                           int(ex.get("first_frame", 0)))
+    assimilate = (str(da_db.get("assimilate_data", "false")) == "true" and
+                  cloud is not None)
+    da = None
+    if assimilate:
+        da = DataAssimilator(
+            str(da_db.get("localization_cutoff_function", "none")),
+            float(da_db.get("localization_cutoff_distance", 1e100)))
+    sigma = float(ex.get("estimated_uncertainty", 0.0))
+
+    ts = db["time_stepping"]
+    dt = float(ts["time_step"])
+    duration = float(ts["duration"])
+    half_window = 1.01 * dt
+    time, da_time = 0.0, -1.0
+    frame_index = -1
+    n_step = 1
+    run = EnsembleRun(sol, params, names, vals)
+    while time < duration:
+        old_time = time
+        time = ens.evolve(old_time, dt, 1, sol)
+        if assimilate:
+            frame_time = math.inf
+            if frame_index + 1 < len(stamps[0]):
+                if time > da_time + half_window:
+                    da_time = stamps[0][frame_index + 1]
+                if stamps[0][frame_index + 1] <= time:
+                    frame_index = cloud.read_next_frame()
+                    frame_time = stamps[0][frame_index]
+            if frame_time <= time:
+                if not (frame_time > old_time or n_step == 1):
+                    raise RuntimeError("Unexpectedly missed a data "
+                                       "assimilation frame.")
+                pv = cloud.get_points_values()
+                if len(pv.values) > 0:
+                    da.update_dof_mapping(ens, np.array(pv.points, float))
+                    R = np.eye(len(pv.values)) * sigma * sigma
+                    da.update_ensemble(ens, sol, pv.values, R,
+                                       params=params if names else None)
+                    run.times.append(time)
+                    if verbose:
+                        print("data assimilation at t = %g" % time,
+                              flush=True)
+                    # new parameters into the members' sources
+                    for j in range(n):
+                        eta = members[j].absorption_efficiency
+                        power = members[j].max_power
+                        for i, nm in enumerate(names):
+                            if nm == "beam_0_absorption":
+                                eta[0] = float(params[j, i])
+                            else:
+                                power[0] = float(params[j, i])
+                        if names:
+                            ens.set_beam_parameters(j, 0, eta[0], power[0])
+        n_step += 1
+    run.n_steps = n_step - 1
+    run.time = time
+    return run

@@ -369,6 +369,13 @@ int adm_ensemble_evolve(void *ens, int method, double t, double dt,
 int adm_ensemble_get_qp_state(void *ens, int32_t member, double *liquid,
                               double *powder);
 
+/* update_physics_parameters of one member (source/ThermalPhysics.templates.hh:
+ * 827-852) for beam `beam`'s absorption efficiency and max power, e.g. after
+ * an analysis of the augmented state (application/adamantine.hh:2306-2361).
+ * Radius, depth and scan path are shared by the members of an ensemble. */
+int adm_ensemble_set_beam_gain(void *ens, int32_t member, int32_t beam,
+                               double absorption_efficiency, double max_power);
+
 /* Material deposition for one member (see adm_cells_to_activate and
  * adm_add_material; T is the whole ensemble vector, or NULL). Replaces the
  * per-member activation of run_ensemble (application/adamantine.hh:

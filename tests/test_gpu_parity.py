@@ -267,3 +267,36 @@ def test_invalid_arguments():
     bad = torch.zeros(op.m(), dtype=torch.float32, device="cuda")
     with pytest.raises(AdmError):
         op.vmult(bad, bad)
+
+
+def test_degree_three():
+    """FE degree 3 (16 lanes per cell, 4 cells per wavefront): lumped mass
+    bit-exact, vmult and phase state against the oracle on a SLP table
+    problem with powder, rotation and radiative/convective faces, and an RK4
+    step."""
+    torch = dev()
+    from adamantine_amd.thermal import ThermalOperatorDevice, ThermalPhysicsDevice
+    p = plain_large(n_cells=(9, 5, 6))
+    p.fe_degree = 3
+    ang = np.random.RandomState(5).uniform(0, np.pi, p.n_cells_total)
+    p.deposition_cos, p.deposition_sin = np.cos(ang), np.sin(ang)
+    op = ThermalOperatorDevice(p)
+    np.testing.assert_array_equal(op.get_inverse_mass_matrix().cpu().numpy(),
+                                  Oracle(p).inverse_mass())
+    o = Oracle(p)
+    for k, t in enumerate((0.0, 1e-4)):
+        T = random_temperature(p, seed=21 + k, low=1200.0, high=2000.0)
+        o.set_time(t)
+        assert rel(gpu_vmult(op, T, t), o.vmult(T)) < VMULT_TOL
+        lq, pw = op.qp_state()
+        olq, opw = o.qp_state()
+        np.testing.assert_array_equal(pw, opw)
+        np.testing.assert_array_equal(lq, olq)
+    ph = ThermalPhysicsDevice(p, RK_FOURTH_ORDER)
+    ph.setup()
+    T0 = smooth_temperature(p)
+    sol = torch.tensor(T0, device="cuda")
+    ph.evolve(0.0, 1e-7, 3, sol)
+    want, _ = Oracle(p).evolve(RK_FOURTH_ORDER, 0.0, 1e-7, 3, T0.copy())
+    torch.cuda.synchronize()
+    assert rel(sol.cpu().numpy(), want) < STEP_TOL

@@ -1,0 +1,58 @@
+"""The reference's data-assimilation integration tests on the device path,
+with their own input files (tests/golden): an ensemble of three members run
+by run_ensemble with point-cloud frames assimilated at the logged times.
+
+integration_data_assimilation (tests/test_integration_da.cc:19-84): the
+members' initial temperatures are drawn around 300 K; one observation of
+200 K per frame pulls the ensemble, and the average of the members' minimum
+temperatures lies in [200, 300).
+
+integration_3D_data_assimilation_augmented
+(tests/test_integration_da_augmented.cc:19-61): beam 0's absorption
+efficiency is in the augmented state, drawn around 0.4 while the data were
+made with 0.3; the analysed average is compared with 0.3 at the reference's
+tolerance (5, relative)."""
+import os
+
+import numpy as np
+import pytest
+
+from adamantine_amd.problem import read_info
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_data_assimilation_bare_plate():
+    from adamantine_amd.run import run_ensemble
+    print("run_ensemble: bare_plate_L_da.info (4x4x1 Q3, 3 members)",
+          flush=True)
+    db = read_info(os.path.join(GOLD, "bare_plate_L_da.info"))
+    run = run_ensemble(db, GOLD)
+    assert run.solution.shape[0] == 3
+    assert len(run.times) == 2            # both frames assimilated
+    assert abs(run.time - 5e-2) < 1e-12 and run.n_steps == 500
+    T = run.solution.cpu().numpy()
+    avg_min = T.min(axis=1).mean()
+    print("  initial temperatures", run.member_values[
+        "materials.initial_temperature"], "average minimum", avg_min,
+        flush=True)
+    assert 200.0 <= avg_min < 300.0
+
+
+def test_data_assimilation_augmented_absorption():
+    from adamantine_amd.run import run_ensemble
+    print("run_ensemble: bare_plate_L_da_augmented.info (8x8x1 Q3)",
+          flush=True)
+    db = read_info(os.path.join(GOLD, "bare_plate_L_da_augmented.info"))
+    run = run_ensemble(db, GOLD)
+    assert run.param_names == ["beam_0_absorption"]
+    prior = run.member_values["sources.beam_0.absorption_efficiency"]
+    post = run.params[:, 0]
+    print("  prior", prior, "analysed", post, flush=True)
+    # frames at 0.02 + 0.02 and 0.04 + 0.02 s: only the first falls in the
+    # 0.05 s run
+    assert len(run.times) == 1
+    assert abs(post.mean() - 0.3) <= 5.0 * 0.3
+    # the analysis moves every member away from its prior towards the data
+    assert (post < prior).all()
