@@ -9,8 +9,12 @@ structured box, on the device:
     reference steps member after member, :2137-2142);
   - experimental frames are read at their time stamps and assimilated with
     the EnKF, with the reference's frame cadence (:2150-2361).
-Mesh refinement, material deposition, checkpoints, output and mechanics of
-the reference's loop are not part of this path.
+`run` is the single-simulation loop of the reference's run
+(application/adamantine.hh:1063-1400) with material deposition: the boxes
+of create_material_deposition_boxes are activated window by window
+(:1134-1207) before each step.
+Mesh refinement, checkpoints, output and mechanics of the reference's loops
+are not part of this path.
 """
 from __future__ import annotations
 
@@ -68,9 +72,6 @@ def problem_from_database(db: Dict, base_dir: str = ".",
     dim = int(_get(g, "dim", 3))
     if dim != 3:
         raise NotImplementedError("the device path is three-dimensional")
-    if str(_get(g, "material_deposition", "false")) == "true":
-        raise NotImplementedError("deposition from the input file: use "
-                                  "cells_to_activate / add_material")
     # axis x = length, y = width, z = height (Geometry.cc:188-243)
     n_cells = (int(_get(g, "length_divisions", 10)),
                int(_get(g, "width_divisions", 10)),
@@ -117,6 +118,60 @@ def _stddev_keys(tree: Dict, path: str = "") -> List[str]:
             out += _stddev_keys(v, full)
         elif full not in ("ensemble_simulation", "ensemble_size"):
             out.append(full)
+    return out
+
+
+@dataclass
+class Run:
+    temperature: object                 # [n_dofs] device tensor
+    physics: object                     # ThermalPhysicsDevice
+    n_steps: int = 0
+    time: float = 0.0
+    n_activations: int = 0
+
+
+def run(db: Dict, base_dir: str = ".", fe_degree: Optional[int] = None,
+        verbose: bool = False) -> Run:
+    """run for the thermal physics on the device, with material deposition
+    from the scan paths or a deposition file."""
+    from bisect import bisect_left
+    from .deposition import create_material_deposition_boxes
+    from .thermal import ThermalPhysicsDevice
+    p = problem_from_database(db, base_dir, fe_degree)
+    ph = ThermalPhysicsDevice(p)
+    ph.setup()
+    T = ph.initialize_dof_vector(p.initial_temperature)
+    g = db["geometry"]
+    boxes, dep_t, dep_cos, dep_sin = create_material_deposition_boxes(
+        g, p.beams, base_dir)
+    dep_t = list(dep_t)
+    ts = db["time_stepping"]
+    dt = float(ts["time_step"])
+    duration = float(ts["duration"])
+    activation_time = float(_get(g, "deposition_time", 0.0))
+    new_temperature = float(_get(db, "materials.new_material_temperature",
+                                 300.0))
+    out = Run(T, ph)
+    time, activation_time_end = 0.0, -1.0
+    tol = dt * 1e-6
+    while time < duration:
+        if dep_t and time - activation_time_end > -tol:
+            start = bisect_left(dep_t, time - tol)
+            activation_time_end = min(time + max(activation_time, dt),
+                                      duration)
+            end = bisect_left(dep_t, activation_time_end)
+            if start < end:
+                b = ph.elements_to_activate(boxes[start:end])
+                ph.add_material(b, dep_cos[start:end], dep_sin[start:end],
+                                new_temperature, T)
+                out.n_activations += 1
+                if verbose:
+                    print("t = %g: boxes %d..%d, %d cells" %
+                          (time, start, end - 1, int((b >= 0).sum())),
+                          flush=True)
+        time = ph.evolve(time, dt, 1, T)
+        out.n_steps += 1
+    out.time = time
     return out
 
 
