@@ -8,7 +8,9 @@ structured box, on the device:
   - the whole ensemble is stepped by one batched call per time step (the
     reference steps member after member, :2137-2142);
   - experimental frames are read at their time stamps and assimilated with
-    the EnKF, with the reference's frame cadence (:2150-2361).
+    the EnKF, with the reference's frame cadence (:2150-2361);
+  - material is deposited in every member with the activation windows of
+    the loop (:2054-2128).
 `run` is the single-simulation loop of the reference's run
 (application/adamantine.hh:1063-1400) with material deposition: the boxes
 of create_material_deposition_boxes are activated window by window
@@ -184,6 +186,8 @@ class EnsembleRun:
     times: List[float] = field(default_factory=list)  # DA times
     n_steps: int = 0
     time: float = 0.0
+    n_activations: int = 0
+    ensemble: object = None             # EnsembleDevice
 
 
 def run_ensemble(db: Dict, base_dir: str = ".",
@@ -228,6 +232,17 @@ def run_ensemble(db: Dict, base_dir: str = ".",
         members.append(Member(absorption_efficiency=eta, max_power=power))
     ens = EnsembleDevice(p, members)
     ens.compute_inverse_mass_matrix()
+    # deposition boxes of member 0's sources, shared by all members
+    # (application/adamantine.hh:1917-1925)
+    from bisect import bisect_left
+    from .deposition import create_material_deposition_boxes
+    g = db["geometry"]
+    boxes, dep_t, dep_cos, dep_sin = create_material_deposition_boxes(
+        g, p.beams, base_dir)
+    dep_t = list(dep_t)
+    activation_time = float(_get(g, "deposition_time", 0.0))
+    new_temperature = float(_get(db, "materials.new_material_temperature",
+                                 300.0))
     sol = ens.initialize_dof_vector(0.0)
     sol += torch.tensor(T0, dtype=torch.float64, device=sol.device)[:, None]
 
@@ -277,8 +292,22 @@ def run_ensemble(db: Dict, base_dir: str = ".",
     time, da_time = 0.0, -1.0
     frame_index = -1
     n_step = 1
-    run = EnsembleRun(sol, params, names, vals)
+    activation_time_end = -1.0
+    tol = dt * 1e-6
+    run = EnsembleRun(sol, params, names, vals, ensemble=ens)
     while time < duration:
+        # material deposition, every member (:2054-2128)
+        if dep_t and time - activation_time_end > -tol:
+            start = bisect_left(dep_t, time - tol)
+            activation_time_end = min(time + max(activation_time, dt),
+                                      duration)
+            end = bisect_left(dep_t, activation_time_end)
+            if start < end:
+                for j in range(n):
+                    b = ens.elements_to_activate(j, boxes[start:end])
+                    ens.add_material(j, b, dep_cos[start:end],
+                                     dep_sin[start:end], new_temperature, sol)
+                run.n_activations += 1
         old_time = time
         time = ens.evolve(old_time, dt, 1, sol)
         if assimilate:

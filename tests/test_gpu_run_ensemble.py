@@ -56,3 +56,24 @@ def test_data_assimilation_augmented_absorption():
     assert abs(post.mean() - 0.3) <= 5.0 * 0.3
     # the analysis moves every member away from its prior towards the data
     assert (post < prior).all()
+
+
+def test_data_assimilation_while_adding_material():
+    """integration_3D_da_point_cloud_add_material
+    (tests/test_integration_da.cc:89-161): material deposited along the scan
+    path in every member while a point-cloud frame is assimilated; the
+    average member minimum lies in [200, 300)."""
+    from adamantine_amd.run import run_ensemble
+    print("run_ensemble: integration_da_add_material.info (3x3x3 Q2)",
+          flush=True)
+    db = read_info(os.path.join(GOLD, "integration_da_add_material.info"))
+    run = run_ensemble(db, GOLD)
+    assert run.solution.shape[0] == 3
+    assert run.n_activations > 0 and len(run.times) == 1
+    T = run.solution.cpu().numpy()
+    # over each member's active nodes, as the reference's vectors hold
+    active = run.ensemble.get_inverse_mass_matrix().cpu().numpy() > 0
+    avg_min = np.mean([T[j][active[j]].min() for j in range(T.shape[0])])
+    print("  activations", run.n_activations, "average minimum", avg_min,
+          flush=True)
+    assert 200.0 <= avg_min < 300.0
