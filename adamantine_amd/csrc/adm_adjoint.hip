@@ -185,8 +185,9 @@ __device__ __forceinline__ void inv_rho_cp_lin(const MatConst &m, int mid,
 // are held fixed)
 __device__ __forceinline__ double beam_value_dR(const adm_beam_state &s,
                                                 double q, double x, double y,
-                                                double log_01) {
+                                                double z, double log_01) {
   if (s.type == ADM_SOURCE_CUBE || q == 0.0) return 0.0;
+  source_point(s, x, y, z);
   const double dx = x - s.center[0], dy = y - s.center[1];
   const double r2 = dx * dx + dy * dy;
   const double R = sqrt(s.radius_squared);
@@ -416,7 +417,7 @@ __global__ void __launch_bounds__(256)
         c_T += mu * ircT * qs * jxw;
         if constexpr (NS == 3) c_p += mu * ircp * qs * jxw;
         pe += mu * irc * q0 * aa.inv_eta0 * jxw;
-        pr += mu * irc * beam_value_dR(sbeams[0], q0, px, py, kc.log_01) * jxw;
+        pr += mu * irc * beam_value_dR(sbeams[0], q0, px, py, pz, kc.log_01) * jxw;
       }
       if constexpr (NS == 3) {
         if (pw) {

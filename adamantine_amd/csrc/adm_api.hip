@@ -139,6 +139,72 @@ static int scan_segment(const BeamData &b, double t, int &old_segment,
   return cur;
 }
 
+// Unit quaternion (r, i, j, k) of a five-axis path (source/Quaternion.cc).
+struct Quat {
+  double r, i, j, k;
+  static Quat of(const double (&q)[4]) { return {q[0], q[1], q[2], q[3]}; }
+  bool operator==(const Quat &o) const {
+    return r == o.r && i == o.i && j == o.j && k == o.k;
+  }
+  // Hamilton product, this * o (source/Quaternion.cc:62-83)
+  Quat operator*(const Quat &o) const {
+    return {r * o.r - i * o.i - j * o.j - k * o.k,
+            r * o.i + i * o.r + j * o.k - k * o.j,
+            r * o.j - i * o.k + j * o.r + k * o.i,
+            r * o.k + i * o.j - j * o.i + k * o.r};
+  }
+  Quat conj() const { return {r, -i, -j, -k}; }
+  double dot(const Quat &o) const {
+    double d = 0.0;
+    d += r * o.r;
+    d += i * o.i;
+    d += j * o.j;
+    d += k * o.k;
+    return d;
+  }
+  // q^e through the axis-angle form (source/Quaternion.cc:101-118)
+  Quat pow(double e) const {
+    const double n = std::sqrt(i * i + j * j + k * k);
+    const double ux = i / n, uy = j / n, uz = k / n;
+    const double th = std::atan2(n, r);
+    const double s_ = std::sin(e * th);
+    return {std::cos(e * th), ux * s_, uy * s_, uz * s_};
+  }
+  // row-major rotation matrix (source/Quaternion.cc:120-134)
+  void matrix(double (&R)[9]) const {
+    R[0] = 1.0 - 2.0 * (j * j + k * k);
+    R[1] = 2.0 * (i * j - k * r);
+    R[2] = 2.0 * (i * k + j * r);
+    R[3] = 2.0 * (i * j + k * r);
+    R[4] = 1.0 - 2.0 * (i * i + k * k);
+    R[5] = 2.0 * (j * k - i * r);
+    R[6] = 2.0 * (i * k - j * r);
+    R[7] = 2.0 * (j * k + i * r);
+    R[8] = 1.0 - 2.0 * (i * i + j * j);
+  }
+};
+
+static bool five_axis_path(const BeamData &b) {
+  const double *q = b.segments[0].end_rotation;
+  return q[0] != 0.0 || q[1] != 0.0 || q[2] != 0.0 || q[3] != 0.0;
+}
+
+// ScanPath::get_quaternion (source/ScanPath.cc:290-341): the slerp from the
+// segment's start rotation to its end rotation on the shorter arc; `cur` and
+// `st` come from the segment search at the same time.
+static Quat path_rotation(const BeamData &b, double t, int cur, double st) {
+  const adm_segment &last = b.segments.back();
+  if (t > last.end_time) return Quat::of(last.end_rotation);
+  const Quat q1 = Quat::of(b.segments[cur].end_rotation);
+  const Quat q0 = Quat::of(b.segments[cur > 0 ? cur - 1 : 0].end_rotation);
+  if (q0 == q1) return q0;
+  const double duration = b.segments[cur].end_time - st;
+  if (duration < 1e-12) return q1;
+  Quat q = q1;
+  if (q0.dot(q) < 0.0) q = {-q.r, -q.i, -q.j, -q.k};
+  return (q * q0.conj()).pow((t - st) / duration) * q0;
+}
+
 void beam_update(const BeamData &b, double t, int &old_segment,
                  adm_beam_state &s, double *pmod_out, double *den_out) {
   std::memset(&s, 0, sizeof(s));
@@ -165,8 +231,16 @@ void beam_update(const BeamData &b, double t, int &old_segment,
     for (int d = 0; d < 3; ++d)
       s.center[d] =
           sp[d] + ((b.segments[cur].end_point[d] - sp[d]) * inv) * (t - st);
+    if (five_axis_path(b)) {
+      path_rotation(b, t, cur, st).matrix(s.rotation);
+      s.rotated = 1;
+    }
   } else {
     for (int d = 0; d < 3; ++d) s.center[d] = -DBL_MAX;
+    if (five_axis_path(b)) {
+      Quat::of(last.end_rotation).matrix(s.rotation);
+      s.rotated = 1;
+    }
   }
   s.on = pmod > 0.0;
   const double r2 = std::pow(b.desc.radius, 2);
@@ -462,6 +536,15 @@ Context::Context(const adm_problem_desc &d) {
         throw AdmError(ADM_ERR_INVALID, "beam without scan path segments");
       bd.segments.assign(bd.desc.segments,
                          bd.desc.segments + bd.desc.n_segments);
+      if (five_axis_path(bd))
+        for (const adm_segment &sg : bd.segments) {
+          const double *q = sg.end_rotation;
+          // Quaternion::reinit (source/Quaternion.cc:44-53)
+          if (!(std::abs(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] +
+                         q[3] * q[3] - 1.0) < 1e-10))
+            throw AdmError(ADM_ERR_INVALID,
+                           "five-axis scan path: the quaternion is not normed");
+        }
     } else if (bd.desc.type < 0 || bd.desc.type > 2) {
       throw AdmError(ADM_ERR_INVALID, "unknown heat source type");
     }

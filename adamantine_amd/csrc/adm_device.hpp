@@ -405,6 +405,27 @@ __device__ __forceinline__ double member_alpha(const adm_beam_state &s,
   return s.alpha;
 }
 
+// The point a beam is evaluated at: rotation * (x, y, z) on a five-axis path
+// (source/GoldakHeatSource.cc:95-103; Tensor<2,3> * Point<3> sums over the
+// columns in order), the point itself otherwise.
+__host__ __device__ __forceinline__ void source_point(const adm_beam_state &s,
+                                                     double &x, double &y,
+                                                     double &z) {
+  if (!s.rotated) return;
+  const double *R = s.rotation;
+  double r[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double acc = R[3 * a] * x;
+    acc += R[3 * a + 1] * y;
+    acc += R[3 * a + 2] * z;
+    r[a] = acc;
+  }
+  x = r[0];
+  y = r[1];
+  z = r[2];
+}
+
 __device__ __forceinline__ double beam_value(const adm_beam_state &s, double x,
                                              double y, double z_,
                                              double log_01) {
@@ -415,6 +436,7 @@ __device__ __forceinline__ double beam_value(const adm_beam_state &s, double x,
     if (!(z_ >= s.cube_min[2] && z_ <= s.cube_max[2])) return 0.0;
     return 1.0 * s.cube_value;
   }
+  source_point(s, x, y, z_);
   const double z = z_ - s.center[2];
   if (z + s.depth < 0.0) return 0.0;
   const double dx = x - s.center[0];
@@ -448,12 +470,35 @@ __device__ __forceinline__ double beam_value(const adm_beam_state &s, double x,
 // True if beam s can be non-zero somewhere in the box [lo, hi] (the cut-offs
 // z + depth < 0 and |xy|^2 > 5 R^2 of the reference, or the cube's box).
 __host__ __device__ __forceinline__ bool beam_may_touch(
-    const adm_beam_state &s, const double (&lo)[3], const double (&hi)[3]) {
+    const adm_beam_state &s, const double (&lo_)[3], const double (&hi_)[3]) {
   if (s.type == ADM_SOURCE_CUBE) {
     if (!s.on) return false;
     for (int d = 0; d < 3; ++d)
-      if (hi[d] < s.cube_min[d] || lo[d] > s.cube_max[d]) return false;
+      if (hi_[d] < s.cube_min[d] || lo_[d] > s.cube_max[d]) return false;
     return true;
+  }
+  double lo[3], hi[3];
+  if (s.rotated) {
+    // the axis-aligned hull of the rotated box (centre R c, half-widths
+    // |R| h), widened so that rounding never culls a touched cell
+    double c[3], h[3];
+    for (int d = 0; d < 3; ++d) {
+      c[d] = 0.5 * (lo_[d] + hi_[d]);
+      h[d] = 0.5 * (hi_[d] - lo_[d]);
+    }
+    source_point(s, c[0], c[1], c[2]);
+    for (int a = 0; a < 3; ++a) {
+      double w = 0.0;
+      for (int b = 0; b < 3; ++b) w += fabs(s.rotation[3 * a + b]) * h[b];
+      w = w * (1.0 + 1e-6) + 1e-6 * fabs(c[a]);
+      lo[a] = c[a] - w;
+      hi[a] = c[a] + w;
+    }
+  } else {
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = lo_[d];
+      hi[d] = hi_[d];
+    }
   }
   if (hi[2] - s.center[2] + s.depth < 0.0) return false;
   double d2 = 0.0;

@@ -49,44 +49,96 @@ def read_material_deposition(filename: str, dim: int = 3) -> Deposition:
             np.array(times), np.array(cos), np.array(sin))
 
 
+def quaternion_matrix(q) -> np.ndarray:
+    """Rotation matrix of the unit quaternion (r, i, j, k)
+    (Quaternion::build_rotation_matrices, source/Quaternion.cc:120-134)."""
+    r, i, j, k = (float(v) for v in q)
+    return np.array([
+        [1.0 - 2.0 * (j * j + k * k), 2.0 * (i * j - k * r),
+         2.0 * (i * k + j * r)],
+        [2.0 * (i * j + k * r), 1.0 - 2.0 * (i * i + k * k),
+         2.0 * (j * k - i * r)],
+        [2.0 * (i * k - j * r), 2.0 * (j * k + i * r),
+         1.0 - 2.0 * (i * i + j * j)]])
+
+
+def _apply(R: np.ndarray, p) -> np.ndarray:
+    """R p with the sum over columns in order (Tensor<2,3> * Point<3>)."""
+    out = np.empty(3)
+    for a in range(3):
+        acc = R[a, 0] * p[0]
+        acc += R[a, 1] * p[1]
+        acc += R[a, 2] * p[2]
+        out[a] = acc
+    return out
+
+
 def deposition_along_scan_path(geometry: Dict, segments: Sequence[tuple],
                                dim: int = 3) -> Deposition:
-    """deposition_along_scan_path (source/material_deposition.cc:130-300)
-    for a three-axis path: boxes of deposition_length along every powered
-    segment (the last one shortened to the segment end), deposition_width
-    across it and deposition_height below the path, each deposited
-    deposition_lead_time before the beam reaches its centre.
-    segments: (end_time, power_modifier, end_point) as read_scan_path gives
-    them."""
+    """deposition_along_scan_path (source/material_deposition.cc:130-300):
+    boxes of deposition_length along every powered segment (the last one
+    shortened to the segment end), deposition_width across it and
+    deposition_height below the path, each deposited deposition_lead_time
+    before the beam reaches its centre.
+
+    segments: (end_time, power_modifier, end_point[, end_rotation]) as
+    read_scan_path gives them. On a five-axis path (segments carrying a
+    quaternion) the box centres walk from the inverse-rotated start point to
+    the inverse-rotated end point, and each box is the axis-aligned hull of
+    the segment's end rotation applied to the box's two corners around the
+    centre, as the reference does."""
     length = float(geometry["deposition_length"])
     height = float(geometry["deposition_height"])
     width = float(_get(geometry, "deposition_width", 0.0))
     lead = float(geometry["deposition_lead_time"])
     eps, eps_time = 1.0e-10, 1.0e-10
     boxes, times, coss, sins = [], [], [], []
+
+    def rot(seg):
+        return quaternion_matrix(seg[3]) if len(seg) > 3 else None
+
+    def build_ref(R, point):
+        # inv_rotate: the transpose applied to the point
+        return _apply(R.T, point) if R is not None else point.copy()
+
     start_time = 0.0
     start = np.array(segments[0][2], dtype=float)
-    for end_time, pmod, end_point in segments:
-        end = np.array(end_point, dtype=float)
+    ref_start = build_ref(rot(segments[0]), start)
+    for seg in segments:
+        end_time, pmod = seg[0], seg[1]
+        end = np.array(seg[2], dtype=float)
+        R = rot(seg)
         if pmod > eps:
+            ref_end = build_ref(R, end)
             seg_len = float(np.linalg.norm(end - start))
-            center = start.copy()
+            center = ref_start.copy()
             velocity = seg_len / (end_time - start_time)
             c = (end[0] - start[0]) / seg_len if seg_len != 0.0 else 1.0
             s = (end[1] - start[1]) / seg_len if seg_len != 0.0 else 0.0
             box_len = length
             while True:
-                dist = float(np.linalg.norm(center - start))
+                dist = float(np.linalg.norm(center - ref_start))
                 t_center = dist / velocity if velocity != 0.0 else 0.0
-                if dim == 2:
-                    size = [box_len, height]
+                if R is not None:
+                    if dim != 3:
+                        raise NotImplementedError(
+                            "five-axis deposition needs dim 3")
+                    xl = abs(c) * box_len + abs(s) * width
+                    yl = abs(c) * width + abs(s) * box_len
+                    a = _apply(R, [xl / 2.0, yl / 2.0, 0.0])
+                    b = _apply(R, [-xl / 2.0, -yl / 2.0, -height])
+                    lo = list(center + np.minimum(a, b))
+                    hi = list(center + np.maximum(a, b))
                 else:
-                    size = [abs(c) * box_len + abs(s) * width,
-                            abs(c) * width + abs(s) * box_len, height]
-                lo = [center[d] - 0.5 * size[d] for d in range(dim - 1)]
-                hi = [center[d] + 0.5 * size[d] for d in range(dim - 1)]
-                lo.append(center[dim - 1] - size[dim - 1])
-                hi.append(center[dim - 1])
+                    if dim == 2:
+                        size = [box_len, height]
+                    else:
+                        size = [abs(c) * box_len + abs(s) * width,
+                                abs(c) * width + abs(s) * box_len, height]
+                    lo = [center[d] - 0.5 * size[d] for d in range(dim - 1)]
+                    hi = [center[d] + 0.5 * size[d] for d in range(dim - 1)]
+                    lo.append(center[dim - 1] - size[dim - 1])
+                    hi.append(center[dim - 1])
                 boxes.append(lo + hi)
                 times.append(max(start_time + t_center - lead, eps_time))
                 coss.append(c)
@@ -97,10 +149,18 @@ def deposition_along_scan_path(geometry: Dict, segments: Sequence[tuple],
                 if dist + length > seg_len:
                     incr = seg_len - dist
                     box_len = seg_len - dist
-                center[0] += c * incr
-                center[1] += s * incr
+                if R is not None:
+                    # deal.II's Point /= scalar multiplies by the reciprocal
+                    direction = ref_end - center
+                    direction = direction * (1.0 / np.sqrt(np.sum(
+                        direction * direction)))
+                    center = center + incr * direction
+                else:
+                    center[0] += c * incr
+                    center[1] += s * incr
         start = end
         start_time = end_time
+        ref_start = build_ref(R, start)
     if not boxes:
         return _empty(dim)
     return (np.array(boxes, dtype=float), np.array(times), np.array(coss),

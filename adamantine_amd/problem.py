@@ -82,7 +82,8 @@ class MeshDesc(C.Structure):
 
 class Segment(C.Structure):
     _fields_ = [("end_time", C.c_double), ("power_modifier", C.c_double),
-                ("end_point", C.c_double * 3)]
+                ("end_point", C.c_double * 3),
+                ("end_rotation", C.c_double * 4)]
 
 
 class BeamDesc(C.Structure):
@@ -105,6 +106,8 @@ class BeamState(C.Structure):
         ("inv_depth_squared", C.c_double), ("radius_squared", C.c_double),
         ("inv_radius_squared", C.c_double), ("cube_value", C.c_double),
         ("cube_min", C.c_double * 3), ("cube_max", C.c_double * 3),
+        ("rotated", C.c_int32), ("pad_", C.c_int32),
+        ("rotation", C.c_double * 9),
     ]
 
 
@@ -197,7 +200,11 @@ class Beam:
     max_power: float = 0.0
     radius: float = 0.0          # diameter / 2
     depth: float = 0.0
-    segments: List[tuple] = field(default_factory=list)  # (end_time, pmod, (x,y,z))
+    # (end_time, pmod, (x,y,z)), plus the unit quaternion (r,i,j,k) on a
+    # five-axis path
+    segments: List[tuple] = field(default_factory=list)
+    # the file ends with SCAN_PATH_END (ScanPath::is_finished)
+    scan_path_end: bool = False
     # cube
     start_time: float = 0.0
     end_time: float = 0.0
@@ -256,6 +263,8 @@ def read_scan_path(path: str, file_format: str = "segment",
                                          for i in (1, 2, 3))})
         return segments_from_path(entries)
     if file_format == "event_series":
+        # "time x y z [r i j k] power", space or comma separated; a line's
+        # power applies to the next segment (source/ScanPath.cc:143-195)
         segs = []
         last_power = 0.0
         for line in lines:
@@ -265,10 +274,31 @@ def read_scan_path(path: str, file_format: str = "segment",
                 break
             tok = [t for t in re.split(r"[ ,]+", line.strip()) if t]
             xyz = tuple(float(tok[i]) * distance_scaling for i in (1, 2, 3))
-            segs.append((float(tok[0]), last_power, xyz))
+            seg = (float(tok[0]), last_power, xyz)
+            if len(tok) == 9:
+                q = tuple(float(tok[i]) for i in (4, 5, 6, 7))
+                if abs(sum(v * v for v in q) - 1.0) >= 1e-10:
+                    raise ValueError("The quaternion is not normed.")
+                seg = seg + (q,)
+            segs.append(seg)
             last_power = float(tok[-1])
+        if any(len(s) > 3 for s in segs) and not all(len(s) > 3 for s in segs):
+            raise ValueError("Five-axis scan path %s has lines without a "
+                             "rotation." % path)
         return segs
     raise ValueError("Error: Format of scan path file not recognized.")
+
+
+def scan_path_finished(path: str, file_format: str = "segment") -> bool:
+    """ScanPath::is_finished (source/ScanPath.cc:58-195): SCAN_PATH_END was
+    read. A segment file stops reading after its declared segment count, so
+    there the marker counts only if it comes before that many segments."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    if file_format == "segment":
+        n_seg = int(lines[1].strip())
+        lines = lines[3:3 + n_seg]
+    return any("SCAN_PATH_END" in line for line in lines)
 
 
 # --------------------------------------------------------------------------
@@ -394,11 +424,14 @@ class Problem:
                 bd.min_point[k] = b.min_point[k]
                 bd.max_point[k] = b.max_point[k]
             segs = (Segment * max(1, len(b.segments)))()
-            for k, (et, pm, xyz) in enumerate(b.segments):
-                segs[k].end_time = et
-                segs[k].power_modifier = pm
+            for k, seg in enumerate(b.segments):
+                segs[k].end_time = seg[0]
+                segs[k].power_modifier = seg[1]
                 for j in range(3):
-                    segs[k].end_point[j] = xyz[j]
+                    segs[k].end_point[j] = seg[2][j]
+                if len(seg) > 3:
+                    for j in range(4):
+                        segs[k].end_rotation[j] = seg[3][j]
             bd.n_segments = len(b.segments)
             bd.segments = C.cast(segs, C.POINTER(Segment))
             keep.append(segs)
@@ -567,10 +600,12 @@ def beams_from_database(db: Dict, base_dir: str = ".",
         if kind is None:
             raise ValueError("Beam type '%s' not recognized." % t)
         path = os.path.join(base_dir, b["scan_path_file"])
-        segs = read_scan_path(path, b["scan_path_file_format"], sp_s, vel_s)
+        fmt = b["scan_path_file_format"]
+        segs = read_scan_path(path, fmt, sp_s, vel_s)
         beams.append(Beam(
             type=kind, absorption_efficiency=float(b["absorption_efficiency"]),
             max_power=float(b["max_power"]) * pow_s,
             radius=float(b["diameter"]) * dim_s / 2.0,
-            depth=float(b["depth"]) * dim_s, segments=segs))
+            depth=float(b["depth"]) * dim_s, segments=segs,
+            scan_path_end=scan_path_finished(path, fmt)))
     return beams

@@ -123,6 +123,21 @@ def _stddev_keys(tree: Dict, path: str = "") -> List[str]:
     return out
 
 
+def simulation_duration(ts: Dict, beams) -> float:
+    """time_stepping.duration, or with scan_path_for_duration the earliest
+    end of the heat sources' scan paths once every path file is finished
+    (application/adamantine.hh:998-1004, 1041-1058). A path that is still
+    being written (no SCAN_PATH_END) would make the reference wait for the
+    file; that live mode is not on the device path."""
+    if str(_get(ts, "scan_path_for_duration", "false")) != "true":
+        return float(ts["duration"])
+    paths = [b for b in beams if b.segments]
+    if not all(b.scan_path_end for b in paths):
+        raise NotImplementedError(
+            "scan_path_for_duration with a scan path that is not finished")
+    return min((b.segments[-1][0] for b in paths), default=math.inf)
+
+
 @dataclass
 class Run:
     temperature: object                 # [n_dofs] device tensor
@@ -149,7 +164,7 @@ def run(db: Dict, base_dir: str = ".", fe_degree: Optional[int] = None,
     dep_t = list(dep_t)
     ts = db["time_stepping"]
     dt = float(ts["time_step"])
-    duration = float(ts["duration"])
+    duration = simulation_duration(ts, p.beams)
     activation_time = float(_get(g, "deposition_time", 0.0))
     new_temperature = float(_get(db, "materials.new_material_temperature",
                                  300.0))
@@ -157,6 +172,8 @@ def run(db: Dict, base_dir: str = ".", fe_degree: Optional[int] = None,
     time, activation_time_end = 0.0, -1.0
     tol = dt * 1e-6
     while time < duration:
+        if time + dt > duration:
+            dt = duration - time     # application/adamantine.hh:1111-1112
         if dep_t and time - activation_time_end > -tol:
             start = bisect_left(dep_t, time - tol)
             activation_time_end = min(time + max(activation_time, dt),
@@ -287,7 +304,7 @@ def run_ensemble(db: Dict, base_dir: str = ".",
 
     ts = db["time_stepping"]
     dt = float(ts["time_step"])
-    duration = float(ts["duration"])
+    duration = simulation_duration(ts, p.beams)
     half_window = 1.01 * dt
     time, da_time = 0.0, -1.0
     frame_index = -1
@@ -296,6 +313,8 @@ def run_ensemble(db: Dict, base_dir: str = ".",
     tol = dt * 1e-6
     run = EnsembleRun(sol, params, names, vals, ensemble=ens)
     while time < duration:
+        if time + dt > duration:
+            dt = duration - time     # application/adamantine.hh:2016-2017
         # material deposition, every member (:2054-2128)
         if dep_t and time - activation_time_end > -tol:
             start = bisect_left(dep_t, time - tol)

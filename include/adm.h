@@ -128,11 +128,17 @@ typedef struct adm_mesh_desc {
   double extent[3];      /* length, width, height */
 } adm_mesh_desc;
 
-/* One scan-path segment (source/ScanPath.hh:40-48); end_time is absolute. */
+/* One scan-path segment (source/ScanPath.hh:40-48); end_time is absolute.
+ * end_rotation is the unit quaternion (r, i, j, k) of a five-axis event
+ * series (source/ScanPath.cc:180-187), all zero on a three-axis path. A path
+ * is five-axis when its first segment carries a rotation; then every segment
+ * must carry one with |q|^2 within 1e-10 of 1 (Quaternion::reinit,
+ * source/Quaternion.cc:44-53). */
 typedef struct adm_segment {
   double end_time;
   double power_modifier;
   double end_point[3];
+  double end_rotation[4];
 } adm_segment;
 
 /* One heat source (source/BeamHeatSourceProperties.hh:55-66,
@@ -165,6 +171,13 @@ typedef struct adm_beam_state {
   double cube_value;
   double cube_min[3];
   double cube_max[3];
+  /* five-axis path: the source is evaluated at rotation * point
+   * (source/GoldakHeatSource.cc:46-57), rotation being the row-major matrix
+   * of ScanPath::get_quaternion(t) (source/ScanPath.cc:290-341,
+   * source/Quaternion.cc:133-150); rotated = 0 on a three-axis path */
+  int32_t rotated;
+  int32_t pad_;
+  double rotation[9];
 } adm_beam_state;
 
 /* Everything the reference's ThermalPhysics/ThermalOperator constructors and

@@ -307,6 +307,106 @@ double oracle_scan_path_power_modifier(const adm_beam_desc *b, double t,
   return b->segments[cur].power_modifier;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Quaternions of five-axis scan paths (source/Quaternion.cc)                 */
+/* ------------------------------------------------------------------------ */
+
+/* Quaternion::operator*=(Quaternion) (source/Quaternion.cc:62-83): the
+ * Hamilton product a <- a b. */
+void oracle_quaternion_mul(double a[4], const double b[4]) {
+  double r = a[0], i = a[1], j = a[2], k = a[3];
+  double br = b[0], bi = b[1], bj = b[2], bk = b[3];  /* b may alias a */
+  a[0] = r * br - i * bi - j * bj - k * bk;
+  a[1] = r * bi + i * br + j * bk - k * bj;
+  a[2] = r * bj - i * bk + j * br + k * bi;
+  a[3] = r * bk + i * bj - j * bi + k * br;
+}
+
+/* Quaternion::operator/=(Quaternion) (source/Quaternion.cc:92-99): a <- a
+ * conj(b), b being of unit norm. */
+void oracle_quaternion_div(double a[4], const double b[4]) {
+  double c[4] = {b[0], -b[1], -b[2], -b[3]};
+  oracle_quaternion_mul(a, c);
+}
+
+/* Quaternion::pow (source/Quaternion.cc:101-118): the axis-angle power,
+ * theta = atan2(|v|, r). */
+void oracle_quaternion_pow(double q[4], double e) {
+  double pn = sqrt(q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+  double u[3] = {q[1] / pn, q[2] / pn, q[3] / pn};
+  double theta = atan2(pn, q[0]);
+  q[0] = cos(e * theta);
+  for (int d = 0; d < 3; ++d) q[d + 1] = u[d] * sin(e * theta);
+}
+
+/* Quaternion::build_rotation_matrices (source/Quaternion.cc:120-150):
+ * the rotation matrix, row-major; the inverse is its transpose. */
+void oracle_quaternion_matrix(const double q[4], double R[9]) {
+  double r = q[0], i = q[1], j = q[2], k = q[3];
+  R[0] = 1.0 - 2.0 * (j * j + k * k);
+  R[1] = 2.0 * (i * j - k * r);
+  R[2] = 2.0 * (i * k + j * r);
+  R[3] = 2.0 * (i * j + k * r);
+  R[4] = 1.0 - 2.0 * (i * i + k * k);
+  R[5] = 2.0 * (j * k - i * r);
+  R[6] = 2.0 * (i * k - j * r);
+  R[7] = 2.0 * (j * k + i * r);
+  R[8] = 1.0 - 2.0 * (i * i + j * j);
+}
+
+/* Tensor<2,3> * Point<3> (Quaternion::rotate, source/Quaternion.hh:
+ * 135-139); inverse = 1 multiplies by the transpose (inv_rotate). */
+void oracle_rotate(const double R[9], int inverse, const double p[3],
+                   double out[3]) {
+  double r[3];
+  for (int a = 0; a < 3; ++a) {
+    double acc = 0.0;
+    for (int b = 0; b < 3; ++b)
+      acc += (inverse ? R[3 * b + a] : R[3 * a + b]) * p[b];
+    r[a] = acc;
+  }
+  for (int a = 0; a < 3; ++a) out[a] = r[a];
+}
+
+static int five_axis(const adm_beam_desc *b) {
+  const double *q = b->segments[0].end_rotation;
+  return q[0] != 0.0 || q[1] != 0.0 || q[2] != 0.0 || q[3] != 0.0;
+}
+
+/* ScanPath::get_quaternion (source/ScanPath.cc:290-341): identity on a
+ * three-axis path, the last rotation after the end, else the slerp between
+ * the segment's start and end rotations on the shorter arc. */
+void oracle_scan_path_quaternion(const adm_beam_desc *b, double t,
+                                 int *old_segment, double q[4]) {
+  q[0] = 1.0;
+  q[1] = q[2] = q[3] = 0.0;
+  if (!five_axis(b)) return;
+  const adm_segment *last = &b->segments[b->n_segments - 1];
+  if (t > last->end_time) {
+    for (int d = 0; d < 4; ++d) q[d] = last->end_rotation[d];
+    return;
+  }
+  double sp[3], st;
+  int cur = current_segment(b, t, old_segment, 0, sp, &st);
+  const double *q0 = cur > 0 ? b->segments[cur - 1].end_rotation
+                             : b->segments[cur].end_rotation;
+  const double *q1 = b->segments[cur].end_rotation;
+  if (q0[0] == q1[0] && q0[1] == q1[1] && q0[2] == q1[2] && q0[3] == q1[3]) {
+    for (int d = 0; d < 4; ++d) q[d] = q0[d];
+    return;
+  }
+  double duration = b->segments[cur].end_time - st;
+  for (int d = 0; d < 4; ++d) q[d] = q1[d];
+  if (duration < 1e-12) return;
+  double dot = 0.0;
+  for (int d = 0; d < 4; ++d) dot += q0[d] * q[d];
+  if (dot < 0.0)
+    for (int d = 0; d < 4; ++d) q[d] *= -1.0;
+  oracle_quaternion_div(q, q0);
+  oracle_quaternion_pow(q, (t - st) / duration);
+  oracle_quaternion_mul(q, q0);
+}
+
 /* HeatSource::update_time for the three source kinds */
 void oracle_beam_update(const adm_beam_desc *b, double t, int *old_segment,
                         adm_beam_state *s) {
@@ -325,6 +425,13 @@ void oracle_beam_update(const adm_beam_desc *b, double t, int *old_segment,
   double pmod = oracle_scan_path_power_modifier(b, t, old_segment);
   s->on = pmod > 0.0;
   oracle_scan_path_position(b, t, old_segment, s->center);
+  if (five_axis(b)) {
+    /* source/GoldakHeatSource.cc:31, ElectronBeamHeatSource.cc:31 */
+    double q[4];
+    oracle_scan_path_quaternion(b, t, old_segment, q);
+    oracle_quaternion_matrix(q, s->rotation);
+    s->rotated = 1;
+  }
   double r2 = pow(b->radius, 2);
   double d = b->depth;
   s->depth = d;
@@ -356,6 +463,12 @@ double oracle_beam_value(const adm_beam_state *s, int dim, const double *pt) {
     for (int d = 0; d < dim; ++d)
       if (!(pt[d] >= s->cube_min[d] && pt[d] <= s->cube_max[d])) return 0.0;
     return 1.0 * s->cube_value;
+  }
+  double rp[3];
+  if (dim == 3 && s->rotated) {
+    /* the five-axis point (source/GoldakHeatSource.cc:95-103) */
+    oracle_rotate(s->rotation, 0, pt, rp);
+    pt = rp;
   }
   int zi = dim - 1;
   double z = pt[zi] - s->center[zi];

@@ -60,6 +60,15 @@ void oracle_scan_path_position(const adm_beam_desc *b, double t,
 double oracle_scan_path_power_modifier(const adm_beam_desc *b, double t,
                                        int *old_segment);
 double oracle_beam_value(const adm_beam_state *s, int dim, const double *pt);
+/* Quaternion (source/Quaternion.cc) and ScanPath::get_quaternion */
+void oracle_quaternion_mul(double a[4], const double b[4]);
+void oracle_quaternion_div(double a[4], const double b[4]);
+void oracle_quaternion_pow(double q[4], double e);
+void oracle_quaternion_matrix(const double q[4], double R[9]);
+void oracle_rotate(const double R[9], int inverse, const double p[3],
+                   double out[3]);
+void oracle_scan_path_quaternion(const adm_beam_desc *b, double t,
+                                 int *old_segment, double q[4]);
 /* MaterialProperty::compute_material_property (vectorized host form) */
 double oracle_material_property(const adm_material_desc *m, int prop,
                                 int material_id, const double *state_ratios,

@@ -67,6 +67,13 @@ def lib():
             C.POINTER(BeamDesc), C.c_double, C.POINTER(C.c_int)]
         L.oracle_beam_value.restype = C.c_double
         L.oracle_beam_value.argtypes = [C.POINTER(BeamState), C.c_int, _dp]
+        L.oracle_quaternion_mul.argtypes = [_dp, _dp]
+        L.oracle_quaternion_div.argtypes = [_dp, _dp]
+        L.oracle_quaternion_pow.argtypes = [_dp, C.c_double]
+        L.oracle_quaternion_matrix.argtypes = [_dp, _dp]
+        L.oracle_rotate.argtypes = [_dp, C.c_int, _dp, _dp]
+        L.oracle_scan_path_quaternion.argtypes = [
+            C.POINTER(BeamDesc), C.c_double, C.POINTER(C.c_int), _dp]
         L.oracle_material_property.restype = C.c_double
         L.oracle_material_property.argtypes = [C.POINTER(MaterialDesc), C.c_int,
                                                C.c_int, _dp, C.c_double]
@@ -207,6 +214,41 @@ def beam_state(beam_desc: BeamDesc, t: float, old_segment=None) -> BeamState:
     lib().oracle_beam_update(C.byref(beam_desc), float(t), C.byref(seg),
                              C.byref(s))
     return s
+
+
+def scan_path_quaternion(beam_desc: BeamDesc, t: float,
+                         old_segment=None) -> np.ndarray:
+    q = np.zeros(4)
+    seg = C.c_int(0 if old_segment is None else old_segment)
+    lib().oracle_scan_path_quaternion(C.byref(beam_desc), float(t),
+                                      C.byref(seg), _ptr(q))
+    return q
+
+
+def quaternion_matrix(q) -> np.ndarray:
+    q = np.ascontiguousarray(q, np.float64)
+    R = np.zeros(9)
+    lib().oracle_quaternion_matrix(_ptr(q), _ptr(R))
+    return R
+
+
+def rotate(R, p, inverse: bool = False) -> np.ndarray:
+    R = np.ascontiguousarray(R, np.float64)
+    p = np.ascontiguousarray(p, np.float64)
+    out = np.zeros(3)
+    lib().oracle_rotate(_ptr(R), int(inverse), _ptr(p), _ptr(out))
+    return out
+
+
+def quaternion_op(op: str, a, b=None, e: float = 0.0) -> np.ndarray:
+    """a * b ("mul"), a / b ("div") or a^e ("pow") as Quaternion does."""
+    a = np.array(a, np.float64)
+    if op == "pow":
+        lib().oracle_quaternion_pow(_ptr(a), float(e))
+    else:
+        b = np.ascontiguousarray(b, np.float64)
+        getattr(lib(), "oracle_quaternion_" + op)(_ptr(a), _ptr(b))
+    return a
 
 
 def beam_value(state: BeamState, dim: int, point) -> float:

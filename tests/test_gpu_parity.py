@@ -300,3 +300,71 @@ def test_degree_three():
     want, _ = Oracle(p).evolve(RK_FOURTH_ORDER, 0.0, 1e-7, 3, T0.copy())
     torch.cuda.synchronize()
     assert rel(sol.cpu().numpy(), want) < STEP_TOL
+
+
+def _five_axis_problem(golden, fe_degree=2):
+    """plain_small's material on [-1, 1]^3 (m) with a Goldak beam along the
+    reference's scan_path_arcs.txt, whose rotation changes along the path;
+    the source is evaluated at R(t) x, so it sits at R(t)^T c(t)."""
+    import os
+    from adamantine_amd.problem import read_scan_path
+    p = plain_small(n_cells=(14, 14, 14))
+    p.extent = (2.0, 2.0, 2.0)
+    p.origin = (-1.0, -1.0, -1.0)
+    p.fe_degree = fe_degree
+    b = p.beams[0]
+    b.segments = read_scan_path(golden("scan_path_arcs.txt"), "event_series")
+    b.radius, b.depth, b.max_power = 0.15, 0.3, 2e7
+    return p
+
+
+def test_five_axis_beam_state(golden):
+    """the library's host scan path (position, power, slerped rotation)
+    equals the oracle's bit for bit along a path whose rotation changes"""
+    from adamantine_amd.thermal import ThermalOperatorDevice
+    from oracle import oracle as orc
+    p = _five_axis_problem(golden)
+    op = ThermalOperatorDevice(p)
+    d, keep = p.to_c()
+    seg = 0
+    for t in np.linspace(0.0, 3.2, 67):
+        got = op.beam_state_at(0, t)
+        want = orc.beam_state(d.beams[0], t, seg)
+        for name, _ in type(got)._fields_:
+            a, b = getattr(got, name), getattr(want, name)
+            if hasattr(a, "__len__"):
+                assert list(a) == list(b), (t, name, list(a), list(b))
+            else:
+                assert a == b, (t, name, a, b)
+        if t <= 3.0:
+            assert got.rotated == 1
+
+
+def test_five_axis_vmult_and_evolve(golden):
+    """vmult with the rotated source (and the culling of cells around the
+    rotated support) and RK4 steps against the oracle"""
+    torch = dev()
+    from adamantine_amd.thermal import ThermalOperatorDevice, ThermalPhysicsDevice
+    p = _five_axis_problem(golden)
+    op = ThermalOperatorDevice(p)
+    o = Oracle(p)
+    T = random_temperature(p, seed=4, low=300.0, high=900.0)
+    for t in (0.5, 1.5, 2.2, 2.5, 2.9):
+        o.set_time(t)
+        got = gpu_vmult(op, T, t)
+        want = o.vmult(T)
+        assert rel(got, want) < VMULT_TOL, (t, rel(got, want))
+        # the source contributes: the result moves with the beam
+        o2 = Oracle(p)
+        o2.set_time(5.0)   # after the path: no source
+        assert rel(want, o2.vmult(T)) > 1e-6
+    ph = ThermalPhysicsDevice(p, RK_FOURTH_ORDER)
+    ph.setup()
+    T0 = np.full(p.n_dofs, 300.0)
+    sol = torch.tensor(T0, device="cuda")
+    ph.evolve(1.4, 1e-2, 5, sol)
+    want, _ = Oracle(p).evolve(RK_FOURTH_ORDER, 1.4, 1e-2, 5, T0.copy())
+    torch.cuda.synchronize()
+    got = sol.cpu().numpy()
+    assert rel(got, want) < STEP_TOL
+    assert got.max() > 301.0
