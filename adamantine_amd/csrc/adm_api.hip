@@ -1377,6 +1377,18 @@ int adm_get_cell_state(void *ptr, double *cs) {
   ADM_API_END
 }
 
+int adm_get_cell_data(void *ptr, uint8_t *active, double *cos_,
+                      double *sin_) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  for (int64_t c = 0; c < ctx.n_cells; ++c) {
+    if (active) active[c] = ctx.active[c] ? 1 : 0;
+    if (cos_) cos_[c] = ctx.dep_cos[c];
+    if (sin_) sin_[c] = ctx.dep_sin[c];
+  }
+  ADM_API_END
+}
+
 int adm_get_qp_state(void *ptr, double *liquid, double *powder) {
   ADM_API_BEGIN
   ADM_CTX(ptr)

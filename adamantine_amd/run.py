@@ -150,14 +150,27 @@ class Run:
 def run(db: Dict, base_dir: str = ".", fe_degree: Optional[int] = None,
         verbose: bool = False) -> Run:
     """run for the thermal physics on the device, with material deposition
-    from the scan paths or a deposition file."""
+    from the scan paths or a deposition file, and the checkpoint and restart
+    sections of the input (files <prefix>.npz and <prefix>_time.txt)."""
     from bisect import bisect_left
     from .deposition import create_material_deposition_boxes
     from .thermal import ThermalPhysicsDevice
     p = problem_from_database(db, base_dir, fe_degree)
     ph = ThermalPhysicsDevice(p)
     ph.setup()
-    T = ph.initialize_dof_vector(p.initial_temperature)
+    # checkpoint / restart (application/adamantine.hh:879-911, 964-973)
+    ck = db.get("checkpoint")
+    every = int(ck["time_steps_between_checkpoint"]) if ck else 0
+    time, n_time_step = 0.0, 0
+    if db.get("restart"):
+        prefix = str(db["restart"]["filename_prefix"])
+        T = ph.load_checkpoint(prefix)
+        with open(prefix + "_time.txt") as f:
+            tok = f.read().split()
+        time, n_time_step = float(tok[0]), int(tok[1])
+    else:
+        T = ph.initialize_dof_vector(p.initial_temperature)
+    n_time_step += 1
     g = db["geometry"]
     boxes, dep_t, dep_cos, dep_sin = create_material_deposition_boxes(
         g, p.beams, base_dir)
@@ -169,7 +182,7 @@ def run(db: Dict, base_dir: str = ".", fe_degree: Optional[int] = None,
     new_temperature = float(_get(db, "materials.new_material_temperature",
                                  300.0))
     out = Run(T, ph)
-    time, activation_time_end = 0.0, -1.0
+    activation_time_end = -1.0
     tol = dt * 1e-6
     while time < duration:
         if time + dt > duration:
@@ -190,6 +203,15 @@ def run(db: Dict, base_dir: str = ".", fe_degree: Optional[int] = None,
                           flush=True)
         time = ph.evolve(time, dt, 1, T)
         out.n_steps += 1
+        if every and n_time_step % every == 0:
+            # application/adamantine.hh:1349-1372
+            prefix = str(ck["filename_prefix"])
+            if str(ck.get("overwrite_files", "true")) != "true":
+                prefix += "_%d" % n_time_step
+            ph.save_checkpoint(prefix, T)
+            with open(prefix + "_time.txt", "w") as f:
+                f.write("%r %d\n" % (time, n_time_step))
+        n_time_step += 1
     out.time = time
     return out
 

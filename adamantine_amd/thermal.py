@@ -62,6 +62,7 @@ def load_library():
     L.adm_get_qp_state.argtypes = [_vp, _dp, _dp]
     L.adm_set_qp_state.argtypes = [_vp, _dp, _dp]
     L.adm_set_deposition.argtypes = [_vp, _dp, _dp]
+    L.adm_get_cell_data.argtypes = [_vp, C.POINTER(C.c_uint8), _dp, _dp]
     L.adm_set_beam_parameters.argtypes = [_vp, C.c_int, C.c_double, C.c_double,
                                           C.c_double, C.c_double]
     L.adm_beam_state_at.argtypes = [_vp, C.c_int, C.c_double,
@@ -180,6 +181,16 @@ class ThermalOperatorDevice:
                                                 s.ctypes.data_as(_dp)))
 
     # quadrature-point state --------------------------------------------------
+    def cell_data(self):
+        """(active flags, deposition cos, deposition sin) per cell as they
+        stand after the activations so far."""
+        active = np.zeros(self.n_cells, np.uint8)
+        cos_, sin_ = np.zeros(self.n_cells), np.zeros(self.n_cells)
+        check(load_library().adm_get_cell_data(
+            self._h, active.ctypes.data_as(C.POINTER(C.c_uint8)),
+            cos_.ctypes.data_as(_dp), sin_.ctypes.data_as(_dp)))
+        return active, cos_, sin_
+
     def qp_state(self):
         liq = np.zeros((self.n_cells, self.n_q))
         pw = np.zeros((self.n_cells, self.n_q))
@@ -277,3 +288,47 @@ class ThermalPhysicsDevice:
 
     def set_state_to_material_properties(self):
         return self.operator.set_state_to_material_properties()
+
+    def save_checkpoint(self, filename: str, solution):
+        """save_checkpoint (source/ThermalPhysics.templates.hh:1087-1185):
+        per cell the state ratios, the deposition cos/sin (10 on inactive
+        cells), has_melted and the active flag, plus the temperature, into
+        <filename>.npz. The mesh is the input file's box, so it is not
+        saved. The state is the cell average of the quadrature-point state
+        at this moment."""
+        active, cos_, sin_ = self.operator.cell_data()
+        state = self.set_state_to_material_properties()
+        on = active.astype(bool)
+        T = solution.detach().cpu().numpy() if hasattr(solution, "cpu") \
+            else np.asarray(solution, np.float64)
+        np.savez(filename + ".npz", temperature=T,
+                 cell_state=state, active=active,
+                 deposition_cos=np.where(on, cos_, 10.0),
+                 deposition_sin=np.where(on, sin_, 10.0),
+                 has_melted=np.zeros(self.operator.n_cells, np.uint8))
+
+    def load_checkpoint(self, filename: str):
+        """load_checkpoint (source/ThermalPhysics.templates.hh:974-1085):
+        rebuilds the operator with the saved active cells, orientations and
+        cell states (every quadrature point takes its cell's state), redoes
+        setup(), and returns the saved temperature on the device."""
+        import dataclasses
+        import torch
+        with np.load(filename + ".npz", allow_pickle=False) as f:
+            data = {k: f[k] for k in f.files}
+        n = self.operator.n_cells
+        if data["active"].shape != (n,) or data["temperature"].shape != (
+                self.operator.m(),):
+            raise ValueError("checkpoint %s does not match the problem" %
+                             filename)
+        on = data["active"].astype(bool)
+        p = dataclasses.replace(
+            self.problem, cell_active=data["active"].astype(np.uint8),
+            cell_state=data["cell_state"],
+            deposition_cos=np.where(on, data["deposition_cos"], 1.0),
+            deposition_sin=np.where(on, data["deposition_sin"], 0.0))
+        self.problem = p
+        self.operator = ThermalOperatorDevice(p)
+        self.setup()
+        return torch.tensor(data["temperature"], dtype=torch.float64,
+                            device="cuda")

@@ -271,7 +271,8 @@ int adm_set_cell_state(void *ctx, const double *cell_state);
 int adm_get_cell_state(void *ctx, double *cell_state);
 
 /* Per-quadrature-point liquid and powder ratios (host, [n_cells][n_q];
- * either pointer may be NULL). Inactive cells read as zero. */
+ * either pointer may be NULL). An inactive cell keeps the state it was given
+ * until it is activated; powder reads as zero in cells that hold none. */
 int adm_get_qp_state(void *ctx, double *liquid, double *powder);
 int adm_set_qp_state(void *ctx, const double *liquid, const double *powder);
 
@@ -283,6 +284,12 @@ int adm_get_face_state(void *ctx, double *face_powder);
 /* set_material_deposition_orientation (source/ThermalOperator.templates.hh:
  * 897-940): per-cell cos/sin (host, n_cells each). */
 int adm_set_deposition(void *ctx, const double *cos_, const double *sin_);
+
+/* The per-cell data ThermalPhysics::save_checkpoint serialises besides the
+ * state (source/ThermalPhysics.templates.hh:1093-1147): the active flag
+ * (fe index 0) and the deposition cos/sin, as they stand after every
+ * activation so far (host out, n_cells each; any pointer may be NULL). */
+int adm_get_cell_data(void *ctx, uint8_t *active, double *cos_, double *sin_);
 
 /* Beam parameter update (ThermalPhysics::update_physics_parameters,
  * source/ThermalPhysics.templates.hh:827-852): new absorption efficiency,

@@ -48,3 +48,31 @@ def test_integration_3d(name, gold, lo, hi, bar):
     print("  largest relative gap of the sorted values %.3e" %
           (d / np.abs(ref)).max(), flush=True)
     assert (d <= bar * np.abs(ref)).all() and (d <= bar * np.abs(got)).all()
+
+
+def test_checkpoint_restart(tmp_path):
+    """integration_3D_checkpoint_restart (tests/test_integration_3d.cc:
+    117-165, tolerance 1e-10): a run that checkpoints every 80 steps, then
+    the same input restarted from its last checkpoint, end on the same
+    temperatures."""
+    from adamantine_amd.run import run
+    db = read_info(os.path.join(GOLD, "demo_316_short_anisotropic.info"))
+    prefix = str(tmp_path / "test_checkpoint")
+    db["checkpoint"] = {"filename_prefix": prefix, "overwrite_files": "true",
+                        "time_steps_between_checkpoint": "80"}
+    r1 = run(db, GOLD)
+    T1 = r1.temperature.cpu().numpy()
+    assert os.path.exists(prefix + ".npz")
+    with open(prefix + "_time.txt") as f:
+        t_ck, n_ck = f.read().split()
+    assert int(n_ck) == 80
+    db["restart"] = {"filename_prefix": prefix}
+    r2 = run(db, GOLD)
+    T2 = r2.temperature.cpu().numpy()
+    print("  first run %d steps to t = %r; restart from step %s (t = %s), "
+          "%d steps" % (r1.n_steps, r1.time, n_ck, t_ck, r2.n_steps),
+          flush=True)
+    assert r2.n_steps == r1.n_steps - 80 and r2.time == r1.time
+    d = np.abs(T1 - T2)
+    print("  largest difference %.3e K" % d.max(), flush=True)
+    assert (d <= 1e-10 * np.abs(T1)).all()
