@@ -148,10 +148,13 @@ class Run:
 
 
 def run(db: Dict, base_dir: str = ".", fe_degree: Optional[int] = None,
-        verbose: bool = False) -> Run:
+        verbose: bool = False, output: bool = False) -> Run:
     """run for the thermal physics on the device, with material deposition
     from the scan paths or a deposition file, and the checkpoint and restart
-    sections of the input (files <prefix>.npz and <prefix>_time.txt)."""
+    sections of the input (files <prefix>.npz and <prefix>_time.txt).
+    With output=True the post_processor section's VTU/PVTU/PVD files are
+    written at the reference's cadence (the initial state, then every
+    time_steps_between_output steps)."""
     from bisect import bisect_left
     from .deposition import create_material_deposition_boxes
     from .thermal import ThermalPhysicsDevice
@@ -170,6 +173,20 @@ def run(db: Dict, base_dir: str = ".", fe_degree: Optional[int] = None,
         time, n_time_step = float(tok[0]), int(tok[1])
     else:
         T = ph.initialize_dof_vector(p.initial_temperature)
+    pp = None
+    if output:
+        # output_pvtu (application/adamantine.hh:69-129, 978-981)
+        from .output import PostProcessor
+        post = db.get("post_processor", {})
+        every_out = int(post.get("time_steps_between_output", 1))
+        pp = PostProcessor(post, ph.problem)
+
+        def write(step, t):
+            active, _, _ = ph.operator.cell_data()
+            pp.write_thermal_output(step, t, T,
+                                    ph.set_state_to_material_properties(),
+                                    active)
+        write(n_time_step, time)
     n_time_step += 1
     g = db["geometry"]
     boxes, dep_t, dep_cos, dep_sin = create_material_deposition_boxes(
@@ -211,7 +228,11 @@ def run(db: Dict, base_dir: str = ".", fe_degree: Optional[int] = None,
             ph.save_checkpoint(prefix, T)
             with open(prefix + "_time.txt", "w") as f:
                 f.write("%r %d\n" % (time, n_time_step))
+        if pp is not None and n_time_step % every_out == 0:
+            write(n_time_step, time)
         n_time_step += 1
+    if pp is not None:
+        pp.write_pvd()
     out.time = time
     return out
 
