@@ -123,6 +123,189 @@ class PointCloud:
         return self._current
 
 
+class RayTracing:
+    """RayTracing (source/RayTracing.cc:111-416) on the structured box.
+    Each line of a frame file is "x0,y0,z0,x1,y1,z1,value": the ray starts
+    at the first point and goes through the second. A ray's observed point
+    is its nearest intersection (distance d >= 0 along the direction) with a
+    face of an active cell; rays that hit nothing are dropped.
+
+    The reference finds candidate cells with a bounding-volume search and
+    then tests every face of each candidate: the ray-plane distance by
+    Cramer's rule and the point in either of the face's two triangles by
+    barycentric coordinates. On the box the candidate faces are those of
+    the grid planes the ray crosses, taken in the order the ray meets them;
+    each is tested with the reference's own face arithmetic, so the points
+    are those the reference computes. `active` returns the current active
+    flags per cell (cell order x fastest)."""
+
+    def __init__(self, file: str, first_camera_id: int, last_camera_id: int,
+                 problem, active, first_frame: int = 0,
+                 timeout: Optional[float] = None):
+        self._data_filename = file
+        self._next_frame = first_frame
+        self._first_camera_id = first_camera_id
+        self._last_camera_id = last_camera_id
+        self._problem = problem
+        self._active = active
+        self._timeout = timeout
+        self._origins = np.zeros((0, 3))
+        self._directions = np.zeros((0, 3))
+        self._values = np.zeros(0)
+
+    def read_next_frame(self) -> int:
+        """read_next_frame (source/RayTracing.cc:126-187)"""
+        o, d, v = [], [], []
+        for cam in range(self._first_camera_id, self._last_camera_id + 1):
+            name = self._data_filename.replace("#camera", str(cam)).replace(
+                "#frame", str(self._next_frame))
+            wait_for_file(name, "Waiting for the next frame: " + name,
+                          timeout=self._timeout)
+            with open(name) as f:
+                lines = f.read().split("\n")
+            if lines and lines[-1] == "":
+                lines.pop()
+            for line in lines[1:]:
+                point, direction, value = [0.0] * 3, [0.0] * 3, 0.0
+                i = 0
+                for fld in line.split(","):
+                    if fld == "":
+                        continue
+                    x = _strtod_prefix(fld)
+                    if i < 3:
+                        point[i] = x
+                    elif i < 6:
+                        direction[i - 3] = x - point[i - 3]
+                    else:
+                        value = x
+                    i += 1
+                o.append(point)
+                d.append(direction)
+                v.append(value)
+        self._origins = np.array(o, dtype=float).reshape(-1, 3)
+        self._directions = np.array(d, dtype=float).reshape(-1, 3)
+        self._values = np.array(v, dtype=float)
+        frame = self._next_frame
+        self._next_frame += 1
+        return frame
+
+    @staticmethod
+    def _face_hit(o, r, p0, p1, p2, p3):
+        """The reference's face test (source/RayTracing.cc:290-356) for rays
+        o + d r against the faces (p0, p1, p2, p3), elementwise: returns
+        (d, hit)."""
+        def dot(a, b):
+            return (a[..., 0] * b[..., 0] + a[..., 1] * b[..., 1]) + \
+                a[..., 2] * b[..., 2]
+
+        e01, e02 = p1 - p0, p2 - p0
+        m0 = -r
+        # determinant of the rows (-r, e01, e02)
+        c0 = e01[..., 1] * e02[..., 2] - e01[..., 2] * e02[..., 1]
+        c1 = e01[..., 2] * e02[..., 0] - e01[..., 0] * e02[..., 2]
+        c2 = e01[..., 0] * e02[..., 1] - e01[..., 1] * e02[..., 0]
+        det = (m0[..., 0] * c0 + m0[..., 1] * c1) + m0[..., 2] * c2
+        area = np.zeros(det.shape)
+        for e in range(3):
+            for k in range(3):
+                area = area + np.abs(e01[..., e] * e02[..., k])
+        ok = ~(np.abs(det) < 1e-10 * area)
+        cross = np.stack([c0, c1, c2], -1)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            d = dot(cross, o - p0) / det
+        ok &= d >= 0
+        x = o + d[..., None] * r
+
+        def in_triangle(a, b, c):
+            ab, ac, ap = b - a, c - a, x - a
+            ab_ab, ab_ac, ac_ac = dot(ab, ab), dot(ab, ac), dot(ac, ac)
+            ap_ab, ap_ac = dot(ap, ab), dot(ap, ac)
+            denom = ab_ab * ac_ac - ab_ac * ab_ac
+            with np.errstate(divide="ignore", invalid="ignore"):
+                v = (ac_ac * ap_ab - ab_ac * ap_ac) / denom
+                w = (ab_ab * ap_ac - ab_ac * ap_ab) / denom
+            u = 1.0 - v - w
+            return (u >= 0.0) & (v >= 0.0) & (w >= 0.0)
+
+        ok &= in_triangle(p0, p1, p2) | in_triangle(p1, p2, p3)
+        return d, ok
+
+    def get_points_values(self) -> PointsValues:
+        """get_points_values (source/RayTracing.cc:189-416)"""
+        p = self._problem
+        n = np.array(p.n_cells)
+        org = np.array(p.origin, dtype=float)
+        h = np.array(p.extent, dtype=float) / n
+        act = np.asarray(self._active(), bool).reshape(n[2], n[1], n[0])
+        O, R = self._origins, self._directions
+        nr = len(O)
+        best = np.full(nr, np.inf)
+        point = np.zeros((nr, 3))
+        # deal.II's face vertex order: the two tangential axes, lower first
+        tang = {0: (1, 2), 1: (0, 2), 2: (0, 1)}
+        for a in (2, 1, 0):
+            b, c = tang[a]
+            go = R[:, a] != 0.0
+            with np.errstate(divide="ignore", invalid="ignore"):
+                # plane index of the first crossing at t >= 0, ray order
+                f = (O[:, a] - org[a]) / h[a]
+                step = np.where(R[:, a] > 0, 1, -1)
+                k = np.where(step > 0, np.ceil(f), np.floor(f))
+            k = np.nan_to_num(k, nan=-1.0, posinf=-1.0, neginf=-1.0)
+            # a ray that starts before the first plane it meets enters at it
+            k = np.where(step > 0, np.maximum(k, 0), np.minimum(k, n[a]))
+            go &= (k >= 0) & (k <= n[a])
+            idx = np.nonzero(go)[0]
+            k, step = k[idx].astype(int), step[idx]
+            while idx.size:
+                inside = (k >= 0) & (k <= n[a])
+                P = org[a] + k * h[a]
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    t = (P - O[idx, a]) / R[idx, a]
+                keep = inside & (t <= best[idx] * (1 + 1e-12) + 1e-300)
+                idx, k, step, P, t = (idx[keep], k[keep], step[keep],
+                                      P[keep], t[keep])
+                if not idx.size:
+                    break
+                q = O[idx] + t[:, None] * R[idx]
+                jb0 = np.floor((q[:, b] - org[b]) / h[b]).astype(int)
+                jc0 = np.floor((q[:, c] - org[c]) / h[c]).astype(int)
+                for db in (-1, 0, 1):
+                    for dc in (-1, 0, 1):
+                        jb, jc = jb0 + db, jc0 + dc
+                        ok = (jb >= 0) & (jb < n[b]) & (jc >= 0) & (jc < n[c])
+                        cell = np.zeros((idx.size, 3), int)
+                        cell[:, b], cell[:, c] = jb, jc
+                        on = np.zeros(idx.size, bool)
+                        for side in (k - 1, k):
+                            cs = cell.copy()
+                            cs[:, a] = side
+                            ins = ok & (side >= 0) & (side < n[a])
+                            cz = np.clip(cs, 0, n - 1)
+                            on |= ins & act[cz[:, 2], cz[:, 1], cz[:, 0]]
+                        if not on.any():
+                            continue
+                        sel = np.nonzero(on)[0]
+                        v = np.zeros((sel.size, 4, 3))
+                        for m, (ub, uc) in enumerate(((0, 0), (1, 0), (0, 1),
+                                                      (1, 1))):
+                            v[:, m, a] = P[sel]
+                            v[:, m, b] = org[b] + (jb[sel] + ub) * h[b]
+                            v[:, m, c] = org[c] + (jc[sel] + uc) * h[c]
+                        d, hit = self._face_hit(O[idx[sel]], R[idx[sel]],
+                                                v[:, 0], v[:, 1], v[:, 2],
+                                                v[:, 3])
+                        rows = idx[sel]
+                        better = hit & (d < best[rows])
+                        r2 = rows[better]
+                        best[r2] = d[better]
+                        point[r2] = O[r2] + d[better][:, None] * R[r2]
+                k = k + step
+        found = np.isfinite(best)
+        return PointsValues([tuple(x) for x in point[found]],
+                            list(self._values[found]))
+
+
 def read_frame_timestamps(log_filename: str, first_camera_id: int,
                           last_camera_id: int, last_frame: int,
                           first_frame: int = 0,

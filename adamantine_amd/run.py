@@ -257,7 +257,8 @@ def run_ensemble(db: Dict, base_dir: str = ".",
     import torch
     from .ensemble import (DataAssimilator, EnsembleDevice, Member,
                            draw_member_values)
-    from .experimental_data import PointCloud, read_frame_timestamps
+    from .experimental_data import (PointCloud, RayTracing,
+                                    read_frame_timestamps)
     p = problem_from_database(db, base_dir, fe_degree)
     ens_db = db.get("ensemble", {})
     n = int(ens_db.get("ensemble_size", 5))
@@ -322,9 +323,11 @@ def run_ensemble(db: Dict, base_dir: str = ".",
     # experimental data
     ex = db.get("experiment", {})
     stamps, cloud = None, None
+    # member 0's active cells, which the ray tracing sees
+    # (application/adamantine.hh:1834-1851)
+    active0 = (np.ones(p.n_cells_total, np.uint8) if p.cell_active is None
+               else np.asarray(p.cell_active, np.uint8).copy())
     if str(ex.get("read_in_experimental_data", "false")) == "true":
-        if str(ex.get("format", "point_cloud")).lower() != "point_cloud":
-            raise NotImplementedError("ray-tracing experimental data")
         stamps = read_frame_timestamps(
             os.path.join(base_dir, str(ex["log_filename"])),
             int(ex["first_camera_id"]), int(ex["last_camera_id"]),
@@ -332,10 +335,16 @@ def run_ensemble(db: Dict, base_dir: str = ".",
             float(ex.get("first_frame_temporal_offset", 0.0)))
         if not stamps or not stamps[0]:
             raise ValueError("the experimental log shows no frame")
-        cloud = PointCloud(os.path.join(base_dir, str(ex["file"])),
-                           int(ex["first_camera_id"]),
-                           int(ex["last_camera_id"]),
-                           int(ex.get("first_frame", 0)))
+        if str(ex.get("format", "point_cloud")).lower() == "point_cloud":
+            cloud = PointCloud(os.path.join(base_dir, str(ex["file"])),
+                               int(ex["first_camera_id"]),
+                               int(ex["last_camera_id"]),
+                               int(ex.get("first_frame", 0)))
+        else:
+            cloud = RayTracing(os.path.join(base_dir, str(ex["file"])),
+                               int(ex["first_camera_id"]),
+                               int(ex["last_camera_id"]), p,
+                               lambda: active0, int(ex.get("first_frame", 0)))
     assimilate = (str(da_db.get("assimilate_data", "false")) == "true" and
                   cloud is not None)
     da = None
@@ -369,6 +378,8 @@ def run_ensemble(db: Dict, base_dir: str = ".",
                     b = ens.elements_to_activate(j, boxes[start:end])
                     ens.add_material(j, b, dep_cos[start:end],
                                      dep_sin[start:end], new_temperature, sol)
+                    if j == 0:
+                        active0[b >= 0] = 1
                 run.n_activations += 1
         old_time = time
         time = ens.evolve(old_time, dt, 1, sol)

@@ -77,3 +77,26 @@ def test_data_assimilation_while_adding_material():
     print("  activations", run.n_activations, "average minimum", avg_min,
           flush=True)
     assert 200.0 <= avg_min < 300.0
+
+
+def test_data_assimilation_ray_while_adding_material():
+    """integration_3D_da_ray_add_material (tests/test_integration_da.cc:
+    164-240): the same input with the frame given as a ray
+    ("experiment.format ray"); the ray's hit on member 0's active cells is
+    assimilated, and the average member minimum lies in [200, 300)."""
+    from adamantine_amd.run import run_ensemble
+    print("run_ensemble: integration_da_add_material.info, ray frames",
+          flush=True)
+    db = read_info(os.path.join(GOLD, "integration_da_add_material.info"))
+    db["experiment"]["file"] = \
+        "integration_da_add_material_expt_ray_#camera_#frame.csv"
+    db["experiment"]["format"] = "ray"
+    run = run_ensemble(db, GOLD)
+    assert run.solution.shape[0] == 3
+    assert run.n_activations > 0 and len(run.times) == 1
+    T = run.solution.cpu().numpy()
+    active = run.ensemble.get_inverse_mass_matrix().cpu().numpy() > 0
+    avg_min = np.mean([T[j][active[j]].min() for j in range(T.shape[0])])
+    print("  activations", run.n_activations, "average minimum", avg_min,
+          flush=True)
+    assert 200.0 <= avg_min < 300.0
