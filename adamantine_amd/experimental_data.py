@@ -4,6 +4,7 @@ temperature vector.
 
 Host readers follow the reference's file formats:
   PointCloud             source/PointCloud.cc:14-105
+  RayTracing             source/RayTracing.cc:111-416
   read_frame_timestamps  source/experimental_data_utils.cc:139-207
   wait_for_file          source/utils.hh:24-36
 The node lookup and the vector write are libadm.so calls
