@@ -15,7 +15,8 @@ LIB = os.path.join(HERE, "libadm.so")
 
 # translation units: the API plus one per cell-kernel configuration
 # (fe_degree, number of material states, property evaluator)
-CELL_CONFIGS = [(p, ns, k) for p in (1, 2, 3) for ns in (1, 2, 3) for k in (0, 1, 2)]
+CELL_CONFIGS = [(p, ns, k) for p in (1, 2, 3, 4, 5) for ns in (1, 2, 3)
+                for k in (0, 1, 2)]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -361,9 +361,9 @@ Context::Context(const adm_problem_desc &d) {
   const adm_mesh_desc &m = d.mesh;
   if (m.dim != 3)
     throw AdmError(ADM_ERR_UNSUPPORTED, "the device path is three-dimensional");
-  if (m.fe_degree < 1 || m.fe_degree > 3)
+  if (m.fe_degree < 1 || m.fe_degree > 5)
     throw AdmError(ADM_ERR_UNSUPPORTED,
-                   "fe_degree must be 1, 2 or 3 on the device path");
+                   "fe_degree must be 1 to 5 on the device path");
   if (m.quadrature != 0 && m.quadrature != 1)
     throw AdmError(ADM_ERR_INVALID, "quadrature must be 0 (Gauss) or 1 "
                                     "(Gauss-Lobatto)");
@@ -998,6 +998,12 @@ void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
     ADM_DISPATCH(3, 1)
     ADM_DISPATCH(3, 2)
     ADM_DISPATCH(3, 3)
+    ADM_DISPATCH(4, 1)
+    ADM_DISPATCH(4, 2)
+    ADM_DISPATCH(4, 3)
+    ADM_DISPATCH(5, 1)
+    ADM_DISPATCH(5, 2)
+    ADM_DISPATCH(5, 3)
 #undef ADM_DISPATCH
   }
   if (fa.n_faces) {
@@ -1019,6 +1025,12 @@ void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
     ADM_DISPATCH(3, 1)
     ADM_DISPATCH(3, 2)
     ADM_DISPATCH(3, 3)
+    ADM_DISPATCH(4, 1)
+    ADM_DISPATCH(4, 2)
+    ADM_DISPATCH(4, 3)
+    ADM_DISPATCH(5, 1)
+    ADM_DISPATCH(5, 2)
+    ADM_DISPATCH(5, 3)
 #undef ADM_DISPATCH
   }
   ADM_HIP(hipGetLastError());
@@ -1040,8 +1052,14 @@ void Context::compute_inverse_mass(double *out, hipStream_t s) {
   else if (p == 2)
     hipLaunchKernelGGL(inverse_mass_kernel<2>, dim3(grid), dim3(block), 0, s,
                        d_meta, d_wgl, det, nc[0], nc[1], nc[2], n_dofs, out);
-  else
+  else if (p == 3)
     hipLaunchKernelGGL(inverse_mass_kernel<3>, dim3(grid), dim3(block), 0, s,
+                       d_meta, d_wgl, det, nc[0], nc[1], nc[2], n_dofs, out);
+  else if (p == 4)
+    hipLaunchKernelGGL(inverse_mass_kernel<4>, dim3(grid), dim3(block), 0, s,
+                       d_meta, d_wgl, det, nc[0], nc[1], nc[2], n_dofs, out);
+  else
+    hipLaunchKernelGGL(inverse_mass_kernel<5>, dim3(grid), dim3(block), 0, s,
                        d_meta, d_wgl, det, nc[0], nc[1], nc[2], n_dofs, out);
   ADM_HIP(hipGetLastError());
 }

@@ -19,9 +19,9 @@
 
 namespace adm {
 
-constexpr int kMaxN = 4;          // fe_degree + 1 <= 4 on the device path
-constexpr int kMaxQ = 64;
-constexpr int kMaxFQ = 16;
+constexpr int kMaxN = 6;          // fe_degree + 1 <= 6 on the device path
+constexpr int kMaxQ = 216;
+constexpr int kMaxFQ = 36;
 constexpr double kStefanBoltzmann = 5.670374419e-8;  // source/types.hh:158
 
 // cell meta word bits

@@ -291,7 +291,7 @@ __device__ __forceinline__ RowCell row_cell(const CellArgs &a, int r, int j,
 
 template <int P, int NS, int PK>
 __global__ void __launch_bounds__(256)
-    __attribute__((amdgpu_waves_per_eu(P == 3 ? 2 : 3)))
+    __attribute__((amdgpu_waves_per_eu(P >= 3 ? 2 : 3)))
     cell_kernel(const CellArgs a, const double *__restrict__ src,
                 double *__restrict__ dst) {
   constexpr int N = P + 1, NL = N * N, N3 = N * N * N;

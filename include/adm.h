@@ -121,7 +121,7 @@ typedef struct adm_material_desc {
  * (source/ThermalPhysics.templates.hh:364-369). */
 typedef struct adm_mesh_desc {
   int32_t dim;           /* 3 (the CPU oracle also accepts 2: axes x, z) */
-  int32_t fe_degree;     /* 1..4 */
+  int32_t fe_degree;     /* 1..5 */
   int32_t quadrature;    /* 0 = QGauss(p+1), 1 = QGaussLobatto(p+1) */
   int32_t n_cells[3];    /* length_divisions, width_divisions, height_divisions */
   double origin[3];

@@ -35,9 +35,9 @@
 #include <omp.h>
 #endif
 
-#define MAXN 5     /* fe_degree + 1 <= 5 */
-#define MAXQ 125   /* (fe_degree + 1)^3 */
-#define MAXFQ 25
+#define MAXN 6     /* fe_degree + 1 <= 6 */
+#define MAXQ 216   /* (fe_degree + 1)^3 */
+#define MAXFQ 36
 
 static const double STEFAN_BOLTZMANN = 5.670374419e-8; /* types.hh:158 */
 
@@ -550,8 +550,11 @@ static void compute_inverse_mass(oracle_t *o) {
 }
 
 oracle_t *oracle_create(const adm_problem_desc *desc) {
-  oracle_t *o = (oracle_t *)calloc(1, sizeof(oracle_t));
   const adm_mesh_desc *m = &desc->mesh;
+  if (m->fe_degree < 1 || m->fe_degree > MAXN - 1 || m->dim < 2 ||
+      m->dim > 3)
+    return NULL;
+  oracle_t *o = (oracle_t *)calloc(1, sizeof(oracle_t));
   o->dim = m->dim;
   o->p = m->fe_degree;
   o->n1 = o->p + 1;

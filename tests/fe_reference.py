@@ -1,16 +1,30 @@
 """Brute-force finite-element assembly used to pin the oracle.
 
 Independent of the oracle and of the library: the quadrature comes from
-numpy's Gauss-Legendre rule, the Lagrange basis on the exact Gauss-Lobatto
-support points of degrees 1 and 2 ({0,1}, {0,1/2,1}) is evaluated directly
-(no sum factorisation), and element matrices are assembled into dense
-global matrices on the lexicographic node numbering.
+numpy's Gauss-Legendre rule, the Lagrange basis on the Gauss-Lobatto support
+points, written in closed form (the end points and the roots of P_p' on
+[-1, 1]: 0; +-1/sqrt(5); 0, +-sqrt(3/7); +-sqrt(1/3 -+ 2 sqrt(7)/21)), is
+evaluated directly (no sum factorisation), and element matrices are
+assembled into dense global matrices on the lexicographic node numbering.
 """
 import itertools
 
 import numpy as np
 
-SUPPORT = {1: np.array([0.0, 1.0]), 2: np.array([0.0, 0.5, 1.0])}
+
+def _gl01(interior):
+    r = np.sort(np.concatenate([[-1.0, 1.0], interior]))
+    return 0.5 * (r + 1.0)
+
+
+SUPPORT = {
+    1: np.array([0.0, 1.0]),
+    2: np.array([0.0, 0.5, 1.0]),
+    3: _gl01([-1.0 / np.sqrt(5.0), 1.0 / np.sqrt(5.0)]),
+    4: _gl01([-np.sqrt(3.0 / 7.0), 0.0, np.sqrt(3.0 / 7.0)]),
+    5: _gl01([s * np.sqrt(1.0 / 3.0 + t * 2.0 * np.sqrt(7.0) / 21.0)
+              for s in (-1.0, 1.0) for t in (-1.0, 1.0)]),
+}
 
 
 def gauss01(n):

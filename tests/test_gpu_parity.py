@@ -368,3 +368,37 @@ def test_five_axis_vmult_and_evolve(golden):
     got = sol.cpu().numpy()
     assert rel(got, want) < STEP_TOL
     assert got.max() > 301.0
+
+
+@pytest.mark.parametrize("p,nc", [(4, (5, 4, 3)), (5, (4, 3, 3))])
+def test_degree_four_and_five(p, nc):
+    """FE degrees 4 and 5 (25 and 36 lanes per cell, 2 and 1 cells per
+    wavefront): lumped mass bit-exact, vmult and phase state against the
+    oracle on the SLP table problem with powder, rotation and
+    radiative/convective faces, and RK4 steps."""
+    torch = dev()
+    from adamantine_amd.thermal import ThermalOperatorDevice, ThermalPhysicsDevice
+    prob = plain_large(n_cells=nc)
+    prob.fe_degree = p
+    ang = np.random.RandomState(p).uniform(0, np.pi, prob.n_cells_total)
+    prob.deposition_cos, prob.deposition_sin = np.cos(ang), np.sin(ang)
+    op = ThermalOperatorDevice(prob)
+    np.testing.assert_array_equal(op.get_inverse_mass_matrix().cpu().numpy(),
+                                  Oracle(prob).inverse_mass())
+    o = Oracle(prob)
+    for k, t in enumerate((0.0, 1e-4)):
+        T = random_temperature(prob, seed=31 + k, low=1200.0, high=2000.0)
+        o.set_time(t)
+        assert rel(gpu_vmult(op, T, t), o.vmult(T)) < VMULT_TOL
+        lq, pw = op.qp_state()
+        olq, opw = o.qp_state()
+        np.testing.assert_array_equal(pw, opw)
+        np.testing.assert_array_equal(lq, olq)
+    ph = ThermalPhysicsDevice(prob, RK_FOURTH_ORDER)
+    ph.setup()
+    T0 = smooth_temperature(prob)
+    sol = torch.tensor(T0, device="cuda")
+    ph.evolve(0.0, 1e-7, 3, sol)
+    want, _ = Oracle(prob).evolve(RK_FOURTH_ORDER, 0.0, 1e-7, 3, T0.copy())
+    torch.cuda.synchronize()
+    assert rel(sol.cpu().numpy(), want) < STEP_TOL

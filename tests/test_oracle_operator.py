@@ -72,11 +72,11 @@ def test_spmv_anisotropic_2d():
     np.testing.assert_allclose(A, -K, rtol=0, atol=1e-12 * np.abs(K).max())
 
 
-@pytest.mark.parametrize("p", [1, 2])
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5])
 def test_spmv_rotated_anisotropic_3d(p):
     # tests/test_thermal_operator.cc:415-578: angle pi/6, k = (1, 0.8, 0.8)
     m = const_material(kx=1.0, ky=0.8, kz=0.8)
-    nc = (3, 2, 2)
+    nc = (3, 2, 2) if p <= 3 else (2, 1, 2)
     ext = (3.0, 2.0, 1.5)
     prob = Problem(n_cells=nc, extent=ext, fe_degree=p, dim=3, material=m)
     o = Oracle(prob)
@@ -172,7 +172,8 @@ def test_nonlinear_cell_term_3d(p):
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * np.abs(want).max())
 
 
-@pytest.mark.parametrize("p,dim", [(1, 3), (2, 3), (2, 2)])
+@pytest.mark.parametrize("p,dim", [(1, 3), (2, 3), (3, 3), (4, 3), (5, 3),
+                                   (2, 2)])
 def test_inverse_mass(p, dim):
     """Lumped Gauss-Lobatto mass: sums to the active volume, equals the row
     sums of the Gauss-Lobatto mass matrix, and is zero off the active cells."""
