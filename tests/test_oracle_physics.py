@@ -119,3 +119,39 @@ def test_convective_gold(golden):
     T, t = run_until(o, FORWARD_EULER, np.full(o.n_dofs, 400.0), 2.5e-4, 1.0)
     assert T.size == gold.size
     np.testing.assert_allclose(np.sort(T), gold, rtol=1e-6)
+
+
+def test_thermal_2d_explicit(golden):
+    """thermal_2d_explicit_host (tests/test_thermal_physics.cc:15-26,
+    tests/test_thermal_physics.hh:77-155): 2-D Q2 on 12 x 6 mm (4 x 5
+    cells), an electron beam with infinite radius and depth, forward Euler
+    with dt = 0.05 to t = 0.1; l2 norm 0.291705 at 1e-3."""
+    segs = read_scan_path(golden("scan_path_test_thermal_physics.txt"))
+    eb = Beam(type=SOURCE_ELECTRON_BEAM, absorption_efficiency=0.1,
+              max_power=1e300, radius=1e100 / 2.0, depth=1e100, segments=segs)
+    prob = Problem(n_cells=(4, 1, 5), extent=(12e-3, 1.0, 6e-3), fe_degree=2,
+                   dim=2, material=material(1.0, 1.0, 1.0, p_order=1),
+                   boundary_types=[BC_ADIABATIC] * 4, beams=[eb])
+    o = Oracle(prob)
+    T, t = run_until(o, FORWARD_EULER, np.zeros(o.n_dofs), 0.05, 0.1)
+    assert abs(t - 0.1) <= 1e-3 * 0.1
+    assert abs(np.linalg.norm(T) - 0.291705) <= 1e-3 * 0.291705
+
+
+def test_radiation_bounds_2d():
+    """radiation_bcs_host (tests/test_thermal_physics.hh:395-510): 5 x 5
+    Q2 cells, radiative faces with emissivity 1 towards 20 from 10, forward
+    Euler with dt = 0.05 to t = 100; every value stays in [10, 20] and some
+    rise above 10."""
+    from adamantine_amd.problem import BC_RADIATIVE
+    m = material(1.0, 1.0, 1.0, extra={"emissivity": 1.0,
+                                       "convection_heat_transfer_coef": 1.0})
+    m.set_property(0, "radiation_temperature_infty", 20.0)
+    m.set_property(0, "convection_temperature_infty", 0.0)
+    prob = Problem(n_cells=(5, 1, 5), extent=(5.0, 1.0, 5.0), fe_degree=2,
+                   dim=2, material=m, boundary_types=[BC_RADIATIVE] * 4,
+                   method=FORWARD_EULER)
+    o = Oracle(prob)
+    T, t = run_until(o, FORWARD_EULER, np.full(o.n_dofs, 10.0), 0.05, 100.0)
+    assert T.min() >= 10.0 and T.min() <= 20.0
+    assert T.max() > 10.0 and T.max() <= 20.0
