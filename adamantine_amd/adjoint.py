@@ -73,3 +73,91 @@ def operator_vjp(op, t: float, T, mu, stream=None):
         op.handle, float(t), _dev_ptr(T), _dev_ptr(mu), _dev_ptr(out),
         g.ctypes.data_as(_dp), _stream_ptr(stream)))
     return out, g
+
+
+class Calibration:
+    """Calibration harness over theta = (absorption efficiency of beam 0,
+    conductivity scale s, radius of beam 0): the misfit of a forward run of
+    ``n_steps`` from ``T0`` against point observations, differentiable in
+    torch through ``misfit(theta, T0)``.
+
+    The conductivity scale multiplies every thermal-conductivity table
+    value or polynomial coefficient of ``problem``; a new s rebuilds the
+    device operator (the adjoint gives the derivative around the operator
+    as built, so dJ/ds = (dJ/ds')|_{s'=1} / s). Absorption efficiency and
+    radius go through update_physics_parameters without a rebuild."""
+
+    def __init__(self, problem, t0: float, dt: float, n_steps: int, obs_dof,
+                 obs, obs_weight=None, checkpoint_every: int = 0,
+                 method=None):
+        self.problem = problem
+        self.t0, self.dt, self.n_steps = float(t0), float(dt), int(n_steps)
+        self.obs_dof, self.obs, self.obs_weight = obs_dof, obs, obs_weight
+        self.checkpoint_every = int(checkpoint_every)
+        self.method = problem.method if method is None else method
+        self._physics = None
+        self._scale = None
+
+    def physics_for(self, theta) -> ThermalPhysicsDevice:
+        import copy
+        eta, s, radius = (float(v) for v in theta)
+        if self._physics is None or s != self._scale:
+            p = copy.deepcopy(self.problem)
+            if s != 1.0:
+                p.material.scale_conductivity(s)
+            self._physics = ThermalPhysicsDevice(p, self.method)
+            self._physics.setup()
+            self._scale = s
+        b = self.problem.beams[0]
+        self._physics.update_physics_parameters(0, eta, b.max_power, radius,
+                                                b.depth)
+        return self._physics
+
+    def run(self, theta, T0) -> AdjointResult:
+        return adjoint_gradient(self.physics_for(theta), self.t0, self.dt,
+                                self.n_steps, T0, self.obs_dof, self.obs,
+                                self.obs_weight, self.checkpoint_every,
+                                self.method)
+
+    def misfit(self, theta, T0):
+        """J(theta, T0) as a torch scalar with gradients for theta (CPU,
+        float64, 3 entries) and T0 (device vector)."""
+        return _Misfit.apply(theta, T0, self)
+
+
+def _misfit_function():
+    import torch
+
+    class Misfit(torch.autograd.Function):
+        """torch.autograd.Function over adm_adjoint_gradient: forward runs
+        the misfit and its adjoint once; backward scales the stored
+        gradients by the incoming gradient."""
+
+        @staticmethod
+        def forward(ctx, theta, T0, cal):
+            r = cal.run(theta.detach().cpu().numpy(), T0.detach())
+            g = r.grad.copy()
+            g[1] /= float(theta[1])   # derivative around the built operator
+            ctx.save_for_backward(torch.tensor(g, dtype=theta.dtype),
+                                  r.lambda0)
+            return torch.tensor(r.J, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, gJ):
+            g, lam = ctx.saved_tensors
+            return (gJ * g).to(gJ.dtype), gJ.to(lam.device) * lam, None
+
+    return Misfit
+
+
+class _MisfitProxy:
+    _fn = None
+
+    @classmethod
+    def apply(cls, *args):
+        if cls._fn is None:
+            cls._fn = _misfit_function()
+        return cls._fn.apply(*args)
+
+
+_Misfit = _MisfitProxy

@@ -136,12 +136,7 @@ def member_problem(base: Problem, m: Member) -> Problem:
         if m.max_power is not None:
             beam.max_power = float(m.max_power[b])
     if m.conductivity_scale != 1.0:
-        s = float(m.conductivity_scale)
-        for name in ("thermal_conductivity_x", "thermal_conductivity_y",
-                     "thermal_conductivity_z"):
-            k = STATE_PROPERTY_NAMES.index(name)
-            p.material.table[:, k, :, :, 1] *= s
-            p.material.poly[:, k, :, :] *= s
+        p.material.scale_conductivity(float(m.conductivity_scale))
     if m.cell_active is not None:
         p.cell_active = np.asarray(m.cell_active, dtype=np.uint8).copy()
     if m.cell_state is not None:

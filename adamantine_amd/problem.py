@@ -173,6 +173,15 @@ class Material:
     def set_property(self, material: int, name: str, value: float):
         self.prop[material, PROPERTY_NAMES.index(name)] = value
 
+    def scale_conductivity(self, s: float):
+        """Multiply every thermal conductivity (table values and polynomial
+        coefficients, all materials and states) by s."""
+        for name in ("thermal_conductivity_x", "thermal_conductivity_y",
+                     "thermal_conductivity_z"):
+            k = STATE_PROPERTY_NAMES.index(name)
+            self.table[:, k, :, :, 1] *= s
+            self.poly[:, k, :, :] *= s
+
     def to_c(self) -> MaterialDesc:
         d = MaterialDesc()
         d.n_materials = self.n_materials

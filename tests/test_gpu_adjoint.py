@@ -296,3 +296,58 @@ def test_adjoint_workload_against_fd():
     fd = (J(prob=scaled(1 + eps)) - J(prob=scaled(1 - eps))) / (2 * eps)
     print(f"  param 1: fd {fd:.8e} adjoint {res.grad[1]:.8e}", flush=True)
     assert abs(fd) > 0 and rel(res.grad[1], fd) < 1e-3
+
+
+def test_calibration_autograd():
+    """torch.autograd through Calibration.misfit: the gradients that
+    backward() leaves on theta = (absorption, conductivity scale, radius)
+    and on T0 are the adjoint's, and at a rebuilt operator (s = 1.02) the
+    conductivity-scale gradient matches central differences of the misfit
+    in s (relative step 1e-5)."""
+    import torch
+    from adamantine_amd.adjoint import Calibration, adjoint_gradient
+    from adamantine_amd.thermal import ThermalPhysicsDevice
+    print("calibration: 10x6x5 Q2 SLP, 7 RK4 steps, autograd", flush=True)
+    p = small_problem("q2_slp", powder=False, method=RK_FOURTH_ORDER)
+    t0, dt, n = 0.0, 1e-6, 7
+    T0h = hot_field(p, 1500.0, 1690.0)
+    X = p.node_coordinates()
+    dof = np.nonzero(np.isclose(X[:, 2], p.extent[2]))[0][::7]
+    obs = np.random.RandomState(5).uniform(1550.0, 1650.0, dof.size)
+    b = p.beams[0]
+    cal = Calibration(p, t0, dt, n, dof, obs, checkpoint_every=3)
+    theta = torch.tensor([b.absorption_efficiency, 1.0, b.radius],
+                         dtype=torch.float64, requires_grad=True)
+    T0 = torch.tensor(T0h, device="cuda", requires_grad=True)
+    J = cal.misfit(theta, T0)
+    J.backward()
+    ph = ThermalPhysicsDevice(p, RK_FOURTH_ORDER)
+    ph.setup()
+    res = adjoint_gradient(ph, t0, dt, n, torch.tensor(T0h, device="cuda"),
+                           dof, obs, checkpoint_every=3)
+    torch.cuda.synchronize()
+    assert float(J.detach()) == res.J
+    for k in range(3):
+        assert rel(float(theta.grad[k]), res.grad[k]) < 1e-10
+    # the adjoint accumulates with atomics: equal up to summation order
+    lam = res.lambda0.cpu().numpy()
+    np.testing.assert_allclose(T0.grad.cpu().numpy(), lam, rtol=0,
+                               atol=1e-12 * np.abs(lam).max())
+    # at s = 1.02 the operator is rebuilt; d J / d s against differences
+    s = 1.02
+    th = torch.tensor([b.absorption_efficiency, s, b.radius],
+                      dtype=torch.float64, requires_grad=True)
+    J2 = cal.misfit(th, T0.detach())
+    J2.backward()
+    e = 1e-5 * s
+    with torch.no_grad():
+        Jp = float(cal.misfit(torch.tensor([b.absorption_efficiency, s + e,
+                                            b.radius], dtype=torch.float64),
+                              T0.detach()))
+        Jm = float(cal.misfit(torch.tensor([b.absorption_efficiency, s - e,
+                                            b.radius], dtype=torch.float64),
+                              T0.detach()))
+    fd = (Jp - Jm) / (2 * e)
+    print(f"  s = {s}: fd {fd:.10e} autograd {float(th.grad[1]):.10e}",
+          flush=True)
+    assert abs(fd) > 0 and rel(float(th.grad[1]), fd) < 1e-5
