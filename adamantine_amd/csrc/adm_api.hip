@@ -978,16 +978,19 @@ void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
     if (cells == 0) continue;
     ac.nseg = (ac.lext[0] + cpw - 1) / cpw;
     ac.n_rows = ac.nseg * ac.lext[1] * ac.lext[2];
-    // one wavefront per row segment, four per block
-    const int64_t nlaunch = (ac.n_rows + 3) / 4;
-#define ADM_DISPATCH(P_, NS_)                                           \
-  if (p == P_ && n_states == NS_) {                                      \
-    if (pk == 0)                                                         \
-      launch_cell<P_, NS_, 0>(ac, src, dst, nlaunch, lds, M, s);         \
-    else if (pk == 1)                                                    \
-      launch_cell<P_, NS_, 1>(ac, src, dst, nlaunch, lds, M, s);         \
-    else                                                                 \
-      launch_cell<P_, NS_, 2>(ac, src, dst, nlaunch, lds, M, s);         \
+    // one wavefront per row segment, cell_waves per block
+    const int64_t nlaunch = (ac.n_rows + cell_waves - 1) / cell_waves;
+#define ADM_DISPATCH(P_, NS_)                                          \
+  if (p == P_ && n_states == NS_) {                                     \
+    if (pk == 0)                                                        \
+      launch_cell<P_, NS_, 0>(ac, src, dst, nlaunch, lds, M, cell_waves, \
+                              s);                                       \
+    else if (pk == 1)                                                   \
+      launch_cell<P_, NS_, 1>(ac, src, dst, nlaunch, lds, M, cell_waves, \
+                              s);                                       \
+    else                                                                \
+      launch_cell<P_, NS_, 2>(ac, src, dst, nlaunch, lds, M, cell_waves, \
+                              s);                                       \
   }
     ADM_DISPATCH(1, 1)
     ADM_DISPATCH(1, 2)
@@ -1356,6 +1359,93 @@ int adm_vmult(void *ptr, const double *src, double *dst, int add,
   // nodes on brick faces are accumulated atomically: start from zero
   if (!add) ctx.fill(dst, 0.0, ctx.n_dofs, s);
   ctx.apply(src, dst, add, s);
+  ADM_API_END
+}
+
+int adm_set_launch_config(void *ptr, int32_t use_coloring,
+                          int32_t waves_per_block) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  if (waves_per_block < 2 || waves_per_block > 4)
+    return adm::set_error(ADM_ERR_INVALID, "waves_per_block must be 2..4");
+  ADM_HIP(hipDeviceSynchronize());
+  ctx.use_coloring = use_coloring != 0;
+  ctx.cell_waves = waves_per_block;
+  ADM_API_END
+}
+
+// Times vmult(src) for every launch configuration and keeps the fastest.
+// The phase state that the evaluations advance is put back afterwards, so
+// the call has no effect on later results beyond the configuration.
+int adm_autotune(void *ptr, const double *src, double *dst, int32_t reps,
+                 void *stream, int32_t *use_coloring_out,
+                 int32_t *waves_out, double *ms_out) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  if (!src || !dst || reps < 1)
+    return adm::set_error(ADM_ERR_INVALID, "null vector or reps < 1");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  ADM_HIP(hipStreamSynchronize(s));
+  // snapshot of the evaluation state
+  struct Saved {
+    void *dev;
+    void *copy;
+    size_t bytes;
+  };
+  std::vector<Saved> saved;
+  auto keep = [&](void *p, size_t bytes) {
+    if (!p || !bytes) return;
+    void *c = nullptr;
+    ADM_HIP(hipMalloc(&c, bytes));
+    ADM_HIP(hipMemcpy(c, p, bytes, hipMemcpyDeviceToDevice));
+    saved.push_back({p, c, bytes});
+  };
+  keep(ctx.d_liquid, sizeof(double) * ctx.n_cells * ctx.nq);
+  keep(ctx.d_liquid_dirty, (size_t)ctx.n_cells);
+  keep(ctx.d_powder, sizeof(double) * ctx.n_powder_cells * ctx.nq);
+  keep(ctx.d_face_powder, sizeof(double) * ctx.faces.size() * ctx.nfq);
+  auto restore = [&]() {
+    for (const Saved &v : saved)
+      ADM_HIP(hipMemcpy(v.dev, v.copy, v.bytes, hipMemcpyDeviceToDevice));
+  };
+  const bool col0 = ctx.use_coloring;
+  const int w0 = ctx.cell_waves;
+  hipEvent_t e0, e1;
+  ADM_HIP(hipEventCreate(&e0));
+  ADM_HIP(hipEventCreate(&e1));
+  double best = 1e300;
+  int best_col = col0 ? 1 : 0, best_w = w0, k = 0;
+  for (int col = 1; col >= 0; --col)
+    for (int w = 2; w <= 4; ++w, ++k) {
+      ctx.use_coloring = col != 0;
+      ctx.cell_waves = w;
+      ctx.fill(dst, 0.0, ctx.n_dofs, s);
+      ctx.apply(src, dst, 0, s);  // warm-up
+      ADM_HIP(hipEventRecord(e0, s));
+      for (int r = 0; r < reps; ++r) {
+        ctx.fill(dst, 0.0, ctx.n_dofs, s);
+        ctx.apply(src, dst, 0, s);
+      }
+      ADM_HIP(hipEventRecord(e1, s));
+      ADM_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      ADM_HIP(hipEventElapsedTime(&ms, e0, e1));
+      const double per = (double)ms / reps;
+      if (ms_out) ms_out[k] = per;
+      if (per < best) {
+        best = per;
+        best_col = col;
+        best_w = w;
+      }
+      restore();
+    }
+  ADM_HIP(hipEventDestroy(e0));
+  ADM_HIP(hipEventDestroy(e1));
+  for (const Saved &v : saved) ADM_HIP(hipFree(v.copy));
+  ctx.use_coloring = best_col != 0;
+  ctx.cell_waves = best_w;
+  if (use_coloring_out) *use_coloring_out = best_col;
+  if (waves_out) *waves_out = best_w;
   ADM_API_END
 }
 

@@ -8,12 +8,12 @@ namespace adm {
 template <int P, int NS, int PK>
 void launch_cell(const CellArgs &a, const double *src, double *dst,
                  int64_t n_blocks, size_t lds_bytes, unsigned n_members,
-                 hipStream_t s) {
-  constexpr int cpb = 4 * CellLanes<P>::cells_per_wave;
+                 int waves, hipStream_t s) {
+  const int cpb = waves * CellLanes<P>::cells_per_wave;
   const unsigned grid = (unsigned)n_blocks;
   const size_t xbytes = sizeof(double) * cpb * CellLanes<P>::stride;
-  hipLaunchKernelGGL((cell_kernel<P, NS, PK>), dim3(grid, n_members), dim3(256),
-                     lds_bytes + xbytes, s, a, src, dst);
+  hipLaunchKernelGGL((cell_kernel<P, NS, PK>), dim3(grid, n_members),
+                     dim3(64 * waves), lds_bytes + xbytes, s, a, src, dst);
 }
 
 template <int P, int NS, int PK>
@@ -28,7 +28,7 @@ void launch_face(const FaceArgs &a, const double *src, double *dst,
 template void launch_cell<ADM_P, ADM_NS, ADM_PK>(const CellArgs &,
                                                  const double *, double *,
                                                  int64_t, size_t, unsigned,
-                                                 hipStream_t);
+                                                 int, hipStream_t);
 template void launch_face<ADM_P, ADM_NS, ADM_PK>(const FaceArgs &,
                                                  const double *, double *,
                                                  unsigned, hipStream_t);

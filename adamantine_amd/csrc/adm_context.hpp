@@ -109,6 +109,8 @@ struct Context {
   // cell kernel in 8 parity colors without atomics (default), or one
   // atomic launch (ADM_CELL_ATOMICS=1 in the environment at creation)
   bool use_coloring = true;
+  // wavefronts (row segments) per block of the cell kernel, 2..4
+  int cell_waves = 4;
   int n_cu = 256;  // compute units of the device (grid sizing)
 
   // device buffers

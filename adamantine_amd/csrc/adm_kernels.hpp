@@ -383,7 +383,7 @@ __global__ void __launch_bounds__(256)
   const double *mBall = lds;
 
   // this wavefront's row segment
-  const int r = blockIdx.x * 4 + wave;
+  const int r = blockIdx.x * (blockDim.x >> 6) + wave;
   if (r >= a.n_rows) return;
   const RowCell rc = row_cell(a, r, j, CPW);
   if (!rc.ok) return;  // whole cells (all lanes) leave together

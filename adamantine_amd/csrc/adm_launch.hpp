@@ -10,10 +10,11 @@ namespace adm {
 
 // PK: 0 polynomial properties, 1 general tables, 2 merged tables.
 // n_members > 1 only with CellArgs/FaceArgs::members set (grid y = member).
+// waves: wavefronts (row segments) per block of the cell kernel, 2..4.
 template <int P, int NS, int PK>
 void launch_cell(const CellArgs &a, const double *src, double *dst,
                  int64_t n_blocks, size_t lds_bytes, unsigned n_members,
-                 hipStream_t s);
+                 int waves, hipStream_t s);
 
 template <int P, int NS, int PK>
 void launch_face(const FaceArgs &a, const double *src, double *dst,

@@ -62,6 +62,10 @@ def load_library():
     L.adm_get_qp_state.argtypes = [_vp, _dp, _dp]
     L.adm_set_qp_state.argtypes = [_vp, _dp, _dp]
     L.adm_set_deposition.argtypes = [_vp, _dp, _dp]
+    L.adm_set_launch_config.argtypes = [_vp, C.c_int32, C.c_int32]
+    L.adm_autotune.argtypes = [_vp, _vp, _vp, C.c_int32, _vp,
+                               C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                               _dp]
     L.adm_get_cell_data.argtypes = [_vp, C.POINTER(C.c_uint8), _dp, _dp]
     L.adm_set_beam_parameters.argtypes = [_vp, C.c_int, C.c_double, C.c_double,
                                           C.c_double, C.c_double]
@@ -181,6 +185,25 @@ class ThermalOperatorDevice:
                                                 s.ctypes.data_as(_dp)))
 
     # quadrature-point state --------------------------------------------------
+    def set_launch_config(self, use_coloring: bool, waves_per_block: int):
+        check(load_library().adm_set_launch_config(
+            self._h, int(bool(use_coloring)), int(waves_per_block)))
+
+    def autotune(self, src, reps: int = 20, stream=None) -> dict:
+        """Time vmult(src) under the 6 launch configurations, keep the
+        fastest; the phase state is left as it was."""
+        dst = self.initialize_dof_vector()
+        col, w = C.c_int32(), C.c_int32()
+        ms = np.zeros(6)
+        check(load_library().adm_autotune(
+            self._h, _dev_ptr(src), _dev_ptr(dst), int(reps),
+            _stream_ptr(stream), C.byref(col), C.byref(w),
+            ms.ctypes.data_as(_dp)))
+        configs = [(c, k) for c in (True, False) for k in (2, 3, 4)]
+        return {"use_coloring": bool(col.value),
+                "waves_per_block": int(w.value),
+                "ms": dict(zip(configs, ms.tolist()))}
+
     def cell_data(self):
         """(active flags, deposition cos, deposition sin) per cell as they
         stand after the activations so far."""

@@ -248,6 +248,24 @@ int adm_set_time(void *ctx, double t);
 int adm_vmult(void *ctx, const double *src, double *dst, int add,
               void *stream);
 
+/* Launch configuration of the cell kernel (no reference counterpart; the
+ * reference's matrix-free loop has no such choice): 8 parity-colour
+ * launches with plain read-add-write (use_coloring = 1, the default) or one
+ * launch with FP64 atomics (0), and 2..4 wavefronts (row segments) per
+ * block (default 4). */
+int adm_set_launch_config(void *ctx, int32_t use_coloring,
+                          int32_t waves_per_block);
+
+/* Autotuner (no reference counterpart): times `reps` vmults of src into dst
+ * (device vectors) under each of the 6 launch configurations (colouring
+ * 1 then 0, each with 2, 3, 4 wavefronts per block; ms_out, host, may be
+ * NULL, receives the 6 per-vmult times in that order), keeps the fastest
+ * and returns it. The phase state is restored afterwards, so later results
+ * do not depend on the tuning. */
+int adm_autotune(void *ctx, const double *src, double *dst, int32_t reps,
+                 void *stream, int32_t *use_coloring_out, int32_t *waves_out,
+                 double *ms_out);
+
 /* Repeated evolve_one_time_step (source/ThermalPhysics.templates.hh:854-888):
  * n_steps steps of `method` from time t with step dt on the device solution
  * vector T (n_dofs doubles, updated in place). use_graph is reserved (the

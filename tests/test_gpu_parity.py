@@ -402,3 +402,39 @@ def test_degree_four_and_five(p, nc):
     want, _ = Oracle(prob).evolve(RK_FOURTH_ORDER, 0.0, 1e-7, 3, T0.copy())
     torch.cuda.synchronize()
     assert rel(sol.cpu().numpy(), want) < STEP_TOL
+
+
+def test_launch_configs_and_autotune():
+    """Every launch configuration of the cell kernel (8 colour launches or
+    one atomic launch; 2, 3 or 4 wavefronts per block) meets the vmult and
+    phase-state bars against the oracle; the autotuner picks one of them and
+    leaves the phase state as it found it."""
+    torch = dev()
+    from adamantine_amd.thermal import ThermalOperatorDevice
+    p = plain_large(n_cells=(24, 12, 8))
+    T = random_temperature(p, seed=8, low=1200.0, high=2000.0)
+    t = 1e-4
+    o = Oracle(p)
+    o.set_time(t)
+    want = o.vmult(T)
+    olq, opw = o.qp_state()
+    for col in (True, False):
+        for w in (2, 3, 4):
+            op = ThermalOperatorDevice(p)
+            op.set_launch_config(col, w)
+            got = gpu_vmult(op, T, t)
+            assert rel(got, want) < VMULT_TOL, (col, w, rel(got, want))
+            lq, pw = op.qp_state()
+            np.testing.assert_array_equal(pw, opw)
+            np.testing.assert_array_equal(lq, olq)
+    op = ThermalOperatorDevice(p)
+    op.set_time(t)
+    lq0, pw0 = op.qp_state()
+    res = op.autotune(torch.tensor(T, device="cuda"), reps=3)
+    print("  autotune:", res, flush=True)
+    assert res["waves_per_block"] in (2, 3, 4)
+    assert all(v > 0 for v in res["ms"].values())
+    lq1, pw1 = op.qp_state()
+    np.testing.assert_array_equal(lq0, lq1)
+    np.testing.assert_array_equal(pw0, pw1)
+    assert rel(gpu_vmult(op, T, t), want) < VMULT_TOL
