@@ -1374,19 +1374,18 @@ int adm_set_launch_config(void *ptr, int32_t use_coloring,
   ADM_API_END
 }
 
-// Times vmult(src) for every launch configuration and keeps the fastest.
-// The phase state that the evaluations advance is put back afterwards, so
-// the call has no effect on later results beyond the configuration.
-int adm_autotune(void *ptr, const double *src, double *dst, int32_t reps,
-                 void *stream, int32_t *use_coloring_out,
-                 int32_t *waves_out, double *ms_out) {
-  ADM_API_BEGIN
-  ADM_CTX(ptr)
-  if (!src || !dst || reps < 1)
-    return adm::set_error(ADM_ERR_INVALID, "null vector or reps < 1");
-  hipStream_t s = static_cast<hipStream_t>(stream);
+// Times `apply` (one operator application into a cleared dst) under every
+// launch configuration of `cfg`, keeps the fastest in cfg, and puts back
+// the evaluation state of every context in `holders` after each trial.
+extern "C++" {
+template <class Apply>
+static void tune_launch(adm::Context &cfg,
+                        const std::vector<adm::Context *> &holders,
+                        Apply apply, int reps, hipStream_t s,
+                        int32_t *use_coloring_out, int32_t *waves_out,
+                        double *ms_out) {
+  using namespace adm;
   ADM_HIP(hipStreamSynchronize(s));
-  // snapshot of the evaluation state
   struct Saved {
     void *dev;
     void *copy;
@@ -1400,32 +1399,28 @@ int adm_autotune(void *ptr, const double *src, double *dst, int32_t reps,
     ADM_HIP(hipMemcpy(c, p, bytes, hipMemcpyDeviceToDevice));
     saved.push_back({p, c, bytes});
   };
-  keep(ctx.d_liquid, sizeof(double) * ctx.n_cells * ctx.nq);
-  keep(ctx.d_liquid_dirty, (size_t)ctx.n_cells);
-  keep(ctx.d_powder, sizeof(double) * ctx.n_powder_cells * ctx.nq);
-  keep(ctx.d_face_powder, sizeof(double) * ctx.faces.size() * ctx.nfq);
+  for (Context *h : holders) {
+    keep(h->d_liquid, sizeof(double) * h->n_cells * h->nq);
+    keep(h->d_liquid_dirty, (size_t)h->n_cells);
+    keep(h->d_powder, sizeof(double) * h->n_powder_cells * h->nq);
+    keep(h->d_face_powder, sizeof(double) * h->faces.size() * h->nfq);
+  }
   auto restore = [&]() {
     for (const Saved &v : saved)
       ADM_HIP(hipMemcpy(v.dev, v.copy, v.bytes, hipMemcpyDeviceToDevice));
   };
-  const bool col0 = ctx.use_coloring;
-  const int w0 = ctx.cell_waves;
   hipEvent_t e0, e1;
   ADM_HIP(hipEventCreate(&e0));
   ADM_HIP(hipEventCreate(&e1));
   double best = 1e300;
-  int best_col = col0 ? 1 : 0, best_w = w0, k = 0;
+  int best_col = cfg.use_coloring ? 1 : 0, best_w = cfg.cell_waves, k = 0;
   for (int col = 1; col >= 0; --col)
     for (int w = 2; w <= 4; ++w, ++k) {
-      ctx.use_coloring = col != 0;
-      ctx.cell_waves = w;
-      ctx.fill(dst, 0.0, ctx.n_dofs, s);
-      ctx.apply(src, dst, 0, s);  // warm-up
+      cfg.use_coloring = col != 0;
+      cfg.cell_waves = w;
+      apply();  // warm-up
       ADM_HIP(hipEventRecord(e0, s));
-      for (int r = 0; r < reps; ++r) {
-        ctx.fill(dst, 0.0, ctx.n_dofs, s);
-        ctx.apply(src, dst, 0, s);
-      }
+      for (int r = 0; r < reps; ++r) apply();
       ADM_HIP(hipEventRecord(e1, s));
       ADM_HIP(hipEventSynchronize(e1));
       float ms = 0.f;
@@ -1442,10 +1437,28 @@ int adm_autotune(void *ptr, const double *src, double *dst, int32_t reps,
   ADM_HIP(hipEventDestroy(e0));
   ADM_HIP(hipEventDestroy(e1));
   for (const Saved &v : saved) ADM_HIP(hipFree(v.copy));
-  ctx.use_coloring = best_col != 0;
-  ctx.cell_waves = best_w;
+  cfg.use_coloring = best_col != 0;
+  cfg.cell_waves = best_w;
   if (use_coloring_out) *use_coloring_out = best_col;
   if (waves_out) *waves_out = best_w;
+}
+}  // extern "C++"
+
+int adm_autotune(void *ptr, const double *src, double *dst, int32_t reps,
+                 void *stream, int32_t *use_coloring_out,
+                 int32_t *waves_out, double *ms_out) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  if (!src || !dst || reps < 1)
+    return adm::set_error(ADM_ERR_INVALID, "null vector or reps < 1");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  tune_launch(
+      ctx, {&ctx},
+      [&]() {
+        ctx.fill(dst, 0.0, ctx.n_dofs, s);
+        ctx.apply(src, dst, 0, s);
+      },
+      reps, s, use_coloring_out, waves_out, ms_out);
   ADM_API_END
 }
 
@@ -1650,6 +1663,40 @@ int adm_ensemble_vmult(void *ptr, const double *src, double *dst, int add,
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (!add) ens.members[0]->fill(dst, 0.0, ens.n_dofs * ens.n_members, s);
   ens.apply(src, dst, add, s);
+  ADM_API_END
+}
+
+int adm_ensemble_set_launch_config(void *ptr, int32_t use_coloring,
+                                   int32_t waves_per_block) {
+  ADM_API_BEGIN
+  ADM_ENS(ptr)
+  if (waves_per_block < 2 || waves_per_block > 4)
+    return adm::set_error(ADM_ERR_INVALID, "waves_per_block must be 2..4");
+  ADM_HIP(hipDeviceSynchronize());
+  ens.members[0]->use_coloring = use_coloring != 0;
+  ens.members[0]->cell_waves = waves_per_block;
+  ADM_API_END
+}
+
+int adm_ensemble_autotune(void *ptr, const double *src, double *dst,
+                          int32_t reps, void *stream,
+                          int32_t *use_coloring_out, int32_t *waves_out,
+                          double *ms_out) {
+  ADM_API_BEGIN
+  ADM_ENS(ptr)
+  if (!src || !dst || reps < 1)
+    return adm::set_error(ADM_ERR_INVALID, "null vector or reps < 1");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  std::vector<adm::Context *> holders;
+  for (auto &m : ens.members) holders.push_back(m.get());
+  adm::Context &c0 = *ens.members[0];
+  tune_launch(
+      c0, holders,
+      [&]() {
+        c0.fill(dst, 0.0, ens.n_dofs * ens.n_members, s);
+        ens.apply(src, dst, 0, s);
+      },
+      reps, s, use_coloring_out, waves_out, ms_out);
   ADM_API_END
 }
 

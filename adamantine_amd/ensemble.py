@@ -60,6 +60,10 @@ def _bind(L):
     L.adm_ensemble_compute_inverse_mass.argtypes = [_vp, _vp, _vp]
     L.adm_ensemble_set_time.argtypes = [_vp, C.c_double]
     L.adm_ensemble_vmult.argtypes = [_vp, _vp, _vp, C.c_int, _vp]
+    L.adm_ensemble_set_launch_config.argtypes = [_vp, C.c_int32, C.c_int32]
+    L.adm_ensemble_autotune.argtypes = [_vp, _vp, _vp, C.c_int32, _vp,
+                                        C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_int32), _dp]
     L.adm_ensemble_evolve.argtypes = [_vp, C.c_int, C.c_double, C.c_double,
                                       C.c_int, _vp, _dp, _vp]
     L.adm_ensemble_get_qp_state.argtypes = [_vp, C.c_int32, _dp, _dp]
@@ -240,6 +244,25 @@ class EnsembleDevice:
         self._check_vec(src)
         check(_lib().adm_ensemble_vmult(self._h, _dev_ptr(src), _dev_ptr(dst),
                                         1, _stream_ptr(stream)))
+
+    def set_launch_config(self, use_coloring: bool, waves_per_block: int):
+        check(_lib().adm_ensemble_set_launch_config(
+            self._h, int(bool(use_coloring)), int(waves_per_block)))
+
+    def autotune(self, src, reps: int = 5, stream=None) -> dict:
+        """ThermalOperatorDevice.autotune for the batched launches."""
+        self._check_vec(src)
+        dst = self.initialize_dof_vector(0.0)
+        col, w = C.c_int32(), C.c_int32()
+        ms = np.zeros(6)
+        check(_lib().adm_ensemble_autotune(
+            self._h, _dev_ptr(src), _dev_ptr(dst), int(reps),
+            _stream_ptr(stream), C.byref(col), C.byref(w),
+            ms.ctypes.data_as(_dp)))
+        configs = [(c, k) for c in (True, False) for k in (2, 3, 4)]
+        return {"use_coloring": bool(col.value),
+                "waves_per_block": int(w.value),
+                "ms": dict(zip(configs, ms.tolist()))}
 
     def evolve(self, t: float, delta_t: float, n_steps: int, solution,
                stream=None) -> float:

@@ -106,6 +106,7 @@ def ensemble_bench(n_members: int, n_steps: int):
     ens.compute_inverse_mass_matrix()
     sol = ens.initialize_dof_vector(300.0)
     t = ens.evolve(0.0, dt, 3, sol)
+    ens_launch = ens.autotune(sol, reps=3)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
@@ -127,13 +128,23 @@ def ensemble_bench(n_members: int, n_steps: int):
         ph.evolve(0.0, dt, 3, s)
     torch.cuda.synchronize()
     k_loop = max(2, n_steps // 2)
-    e0.record(stream)
-    for k in range(k_loop):
-        for ph, s in zip(phs, sols):
-            ph.evolve(3 * dt + k * dt, dt, 1, s)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    loop_ms = e0.elapsed_time(e1) / (k_loop * n_loop)
+
+    def time_loop(t_start):
+        e0.record(stream)
+        for k in range(k_loop):
+            for ph, s in zip(phs, sols):
+                ph.evolve(t_start + k * dt, dt, 1, s)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / (k_loop * n_loop)
+    # the single-member solvers with the library's default launches, then
+    # with their own best launch configuration
+    default_ms = time_loop(3 * dt)
+    loop_launch = phs[0].operator.autotune(sols[0], reps=5)
+    for ph in phs[1:]:
+        ph.operator.set_launch_config(loop_launch["use_coloring"],
+                                      loop_launch["waves_per_block"])
+    loop_ms = time_loop((3 + k_loop) * dt)
     batched = n_members / (batched_ms * 1e-3)
     loop = 1.0 / (loop_ms * 1e-3)
     return {"members": n_members, "n_dofs_per_member": base.n_dofs,
@@ -141,7 +152,14 @@ def ensemble_bench(n_members: int, n_steps: int):
             "member_steps_per_s": batched,
             "loop_member_steps_per_s": loop,
             "speedup_vs_loop": batched / loop,
-            "loop_sample": "%d members x %d steps" % (n_loop, k_loop)}
+            "loop_default_launch_member_steps_per_s": 1.0 / (default_ms *
+                                                             1e-3),
+            "speedup_vs_default_launch_loop": batched * default_ms * 1e-3,
+            "loop_sample": "%d members x %d steps" % (n_loop, k_loop),
+            "launch": {k: ens_launch[k] for k in ("use_coloring",
+                                                  "waves_per_block")},
+            "loop_launch": {k: loop_launch[k] for k in ("use_coloring",
+                                                        "waves_per_block")}}
 
 
 def adjoint_bench():

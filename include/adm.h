@@ -396,6 +396,15 @@ int adm_ensemble_set_time(void *ens, double t);
 int adm_ensemble_vmult(void *ens, const double *src, double *dst, int add,
                        void *stream);
 
+/* adm_set_launch_config and adm_autotune for the batched launches of an
+ * ensemble (no reference counterpart); src, dst: n_members * n_dofs. */
+int adm_ensemble_set_launch_config(void *ens, int32_t use_coloring,
+                                   int32_t waves_per_block);
+int adm_ensemble_autotune(void *ens, const double *src, double *dst,
+                          int32_t reps, void *stream,
+                          int32_t *use_coloring_out, int32_t *waves_out,
+                          double *ms_out);
+
 /* evolve_one_time_step of every member, n_steps times, batched. This
  * replaces the serial member loop of run_ensemble
  * (application/adamantine.hh:2137-2142). T: device, n_members*n_dofs. */

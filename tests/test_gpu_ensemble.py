@@ -103,3 +103,47 @@ def test_ensemble_rejects_bad_vectors():
     bad = torch.zeros(ens.n_dofs, dtype=torch.float64, device="cuda")
     with pytest.raises(ValueError):
         ens.vmult(bad, bad)
+
+
+def test_ensemble_launch_configs_and_autotune():
+    """The batched launch under each launch configuration meets the vmult
+    and phase-state bars of every member against its oracle; the ensemble
+    autotuner picks one and leaves every member's phase state as it was."""
+    import torch
+    from adamantine_amd.ensemble import EnsembleDevice, member_problem
+    base, members = small(3)
+    print("ensemble launch configurations: 3 members, 16x8x8 Q2", flush=True)
+    T = np.stack([np.random.RandomState(40 + j).uniform(300, 2000,
+                                                        base.n_dofs)
+                  for j in range(len(members))])
+    t = 2e-4
+    want, wq = [], []
+    for m in members:
+        o = Oracle(member_problem(base, m))
+        o.set_time(t)
+        want.append(o.vmult(T[len(want)]))
+        wq.append(o.qp_state())
+    for col in (True, False):
+        for w in (2, 3, 4):
+            ens = EnsembleDevice(base, members)
+            ens.set_launch_config(col, w)
+            ens.set_time(t)
+            dst = ens.initialize_dof_vector()
+            ens.vmult(dst, torch.tensor(T, device="cuda"))
+            torch.cuda.synchronize()
+            got = dst.cpu().numpy()
+            for j in range(len(members)):
+                assert rel(got[j], want[j]) < VMULT_TOL, (col, w, j)
+                lq, pw = ens.qp_state(j)
+                np.testing.assert_array_equal(pw, wq[j][1])
+                np.testing.assert_array_equal(lq, wq[j][0])
+    ens = EnsembleDevice(base, members)
+    ens.set_time(t)
+    before = [ens.qp_state(j) for j in range(len(members))]
+    res = ens.autotune(torch.tensor(T, device="cuda"), reps=2)
+    print("  autotune:", res, flush=True)
+    assert res["waves_per_block"] in (2, 3, 4)
+    for j in range(len(members)):
+        lq, pw = ens.qp_state(j)
+        np.testing.assert_array_equal(lq, before[j][0])
+        np.testing.assert_array_equal(pw, before[j][1])
