@@ -23,10 +23,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6
-# HBM bytes of one operator application of this workload (8 color launches
-# of the cell kernel + the face kernel), from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE runs scaled by a 256 MiB copy (profiles/hbm_traffic_plain_large.md)
-MEASURED_TRAFFIC_BYTES = 290.4e6
+# HBM bytes of one operator application of this workload (the cell kernel
+# as 8 colour launches or one atomic launch, 4 wavefronts per block, + the
+# face kernel), from rocprofv3 FETCH_SIZE and WRITE_SIZE runs scaled by a
+# 256 MiB copy (profiles/hbm_traffic_plain_large.md); keyed by use_coloring
+MEASURED_TRAFFIC_BYTES = {True: 290.4e6, False: 198.0e6}
 
 
 def algorithmic_bytes(problem, n_powder_qp: int, n_active_cells: int) -> int:
@@ -256,6 +257,9 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--launch", default=None,
+                    help="fixed launch configuration COLORING,WAVES of the "
+                         "operator (e.g. 0,4) instead of the autotuner's")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--only", choices=["all", "vmult", "ensemble", "copy",
                                        "adjoint", "deposition"],
@@ -303,7 +307,12 @@ def main():
     op.set_time(t_eval)
     # launch configuration picked by the library's autotuner (colour
     # launches or atomics, wavefronts per block); phase state left as is
-    tuned = op.autotune(T, reps=10)
+    if args.launch:
+        col, waves = (int(v) for v in args.launch.split(","))
+        op.set_launch_config(bool(col), waves)
+        tuned = {"use_coloring": bool(col), "waves_per_block": waves}
+    else:
+        tuned = op.autotune(T, reps=10)
 
     for _ in range(args.warmup):
         op.vmult_add(dst, T)
@@ -373,7 +382,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved_gbs,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS,
-                     "traffic": MEASURED_TRAFFIC_BYTES,
+                     "traffic": MEASURED_TRAFFIC_BYTES[
+                         launch["use_coloring"]],
                      "bytes_per_launch": bytes_per_vmult},
         "rk4_step_ms": rk4_ms,
         "wall_s": wall,
