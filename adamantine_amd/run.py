@@ -250,10 +250,34 @@ class EnsembleRun:
     ensemble: object = None             # EnsembleDevice
 
 
+def _cell_state(liquid: np.ndarray, powder: np.ndarray,
+                n_states: int) -> np.ndarray:
+    """MaterialProperty::set_state (source/MaterialProperty.templates.hh:
+    537-620): cell averages of the quadrature-point ratios."""
+    l = liquid.mean(axis=1)
+    p = powder.mean(axis=1)
+    out = np.zeros((len(l), n_states))
+    if n_states == 1:
+        out[:, 0] = 1.0
+    elif n_states == 2:
+        out[:, 1] = l
+        out[:, 0] = 1.0 - l
+    else:
+        out[:, 1] = l
+        out[:, 2] = p
+        out[:, 0] = 1.0 - l - p
+    return out
+
+
 def run_ensemble(db: Dict, base_dir: str = ".",
                  fe_degree: Optional[int] = None,
-                 verbose: bool = False) -> EnsembleRun:
-    """run_ensemble for the thermal physics on the device."""
+                 verbose: bool = False, output: bool = False) -> EnsembleRun:
+    """run_ensemble for the thermal physics on the device. With output=True
+    every member writes the post_processor section's VTU/PVTU/PVD files
+    (prefix _m<member>) at the reference's cadence: the initial state, every
+    time_steps_between_output steps, and around each assimilation frame
+    unless output_on_data_assimilation is false
+    (application/adamantine.hh:1872-1885, 2416-2441)."""
     import torch
     from .ensemble import (DataAssimilator, EnsembleDevice, Member,
                            draw_member_values)
@@ -323,10 +347,13 @@ def run_ensemble(db: Dict, base_dir: str = ".",
     # experimental data
     ex = db.get("experiment", {})
     stamps, cloud = None, None
-    # member 0's active cells, which the ray tracing sees
-    # (application/adamantine.hh:1834-1851)
-    active0 = (np.ones(p.n_cells_total, np.uint8) if p.cell_active is None
-               else np.asarray(p.cell_active, np.uint8).copy())
+    # every member's active cells (member 0's are what the ray tracing sees,
+    # application/adamantine.hh:1834-1851)
+    base_active = (np.ones(p.n_cells_total, np.uint8) if p.cell_active is None
+                   else np.asarray(p.cell_active, np.uint8))
+    active = [(np.asarray(m.cell_active, np.uint8) if m.cell_active is not None
+               else base_active).copy() for m in members]
+    active0 = active[0]
     if str(ex.get("read_in_experimental_data", "false")) == "true":
         stamps = read_frame_timestamps(
             os.path.join(base_dir, str(ex["log_filename"])),
@@ -364,6 +391,22 @@ def run_ensemble(db: Dict, base_dir: str = ".",
     activation_time_end = -1.0
     tol = dt * 1e-6
     run = EnsembleRun(sol, params, names, vals, ensemble=ens)
+    pps = []
+    if output:
+        from .output import PostProcessor
+        post = db.get("post_processor", {})
+        every_out = int(post.get("time_steps_between_output", 1))
+        out_on_da = str(post.get("output_on_data_assimilation",
+                                 "true")) == "true"
+        pps = [PostProcessor(post, p, j) for j in range(n)]
+
+        def write(step, t):
+            for j, pp in enumerate(pps):
+                lq, pw = ens.qp_state(j)
+                pp.write_thermal_output(
+                    step, t, sol[j], _cell_state(lq, pw, p.material.n_states),
+                    active[j])
+        write(0, time)
     while time < duration:
         if time + dt > duration:
             dt = duration - time     # application/adamantine.hh:2016-2017
@@ -378,8 +421,7 @@ def run_ensemble(db: Dict, base_dir: str = ".",
                     b = ens.elements_to_activate(j, boxes[start:end])
                     ens.add_material(j, b, dep_cos[start:end],
                                      dep_sin[start:end], new_temperature, sol)
-                    if j == 0:
-                        active0[b >= 0] = 1
+                    active[j][b >= 0] = 1
                 run.n_activations += 1
         old_time = time
         time = ens.evolve(old_time, dt, 1, sol)
@@ -416,7 +458,13 @@ def run_ensemble(db: Dict, base_dir: str = ".",
                                 power[0] = float(params[j, i])
                         if names:
                             ens.set_beam_parameters(j, 0, eta[0], power[0])
+        if pps and (n_step % every_out == 0 or
+                    (out_on_da and da_time - half_window < time <
+                     da_time + half_window)):
+            write(n_step, time)
         n_step += 1
+    for pp in pps:
+        pp.write_pvd()
     run.n_steps = n_step - 1
     run.time = time
     return run

@@ -100,3 +100,47 @@ def test_data_assimilation_ray_while_adding_material():
     print("  activations", run.n_activations, "average minimum", avg_min,
           flush=True)
     assert 200.0 <= avg_min < 300.0
+
+
+def test_run_ensemble_writes_vtu(tmp_path):
+    """per-member output of the ensemble loop (application/adamantine.hh:
+    1872-1885, 2416-2441): with a long output period, the initial state and
+    the steps around the assimilation frame; with a period of one, every
+    step, the last one holding each member's own nodal temperatures at the
+    written corners; a time series per member"""
+    from adamantine_amd.output import read_vtu
+    from adamantine_amd.run import run_ensemble
+
+    def go(sub, period):
+        out = tmp_path / sub
+        out.mkdir()
+        db = read_info(os.path.join(GOLD, "integration_da_add_material.info"))
+        db["post_processor"] = {"filename_prefix": "ens",
+                                "output_dir": str(out) + "/",
+                                "time_steps_between_output": str(period)}
+        run = run_ensemble(db, GOLD, output=True)
+        names = sorted(os.listdir(out))
+        steps = sorted({int(x.split(".")[1]) for x in names
+                        if x.startswith("ens_m0.") and x.endswith(".vtu")})
+        return run, out, names, steps
+
+    run, out, names, steps = go("da", 100000)
+    print("  member 0 wrote steps", steps, "of", run.n_steps, "DA at",
+          run.times, flush=True)
+    assert steps[0] == 0 and len(steps) >= 2 and len(run.times) == 1
+    run, out, names, steps = go("all", 1)
+    n = run.solution.shape[0]
+    for j in range(n):
+        assert "ens_m%d.pvd" % j in names
+    assert steps == list(range(run.n_steps + 1))
+    p = run.ensemble.base
+    T = run.solution.cpu().numpy()
+    h = np.array(p.extent) / np.array(p.n_cells) / p.fe_degree
+    nn = np.array(p.n_nodes)
+    for j in range(n):
+        d = read_vtu(str(out / ("ens_m%d.%d.0.vtu" % (j, run.n_steps))))
+        idx = np.rint((d["points"] - np.array(p.origin)) / h).astype(int)
+        node = idx[:, 0] + nn[0] * (idx[:, 1] + nn[1] * idx[:, 2])
+        nz = d["temperature"] != 0.0
+        assert nz.any()
+        np.testing.assert_array_equal(d["temperature"][nz], T[j][node[nz]])
