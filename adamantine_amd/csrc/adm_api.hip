@@ -585,6 +585,8 @@ Context::~Context() {
   ADM_FREE(d_powder);
   ADM_FREE(d_face_powder);
   ADM_FREE(d_faces);
+  ADM_FREE(d_beam_table);
+  ADM_FREE(d_slot);
 }
 
 void Context::upload_constants() {
@@ -1146,6 +1148,96 @@ void Context::step(int method, double t, double dt, double *T, hipStream_t s) {
       s);
 }
 
+// Stage time fractions of explicit_step's methods.
+static int n_stages(int method) {
+  return method == ADM_FORWARD_EULER ? 1
+                                     : (method == ADM_RK_FOURTH_ORDER ? 4 : 3);
+}
+static double stage_c(int method, int i) {
+  static const double c4[4] = {0.0, 0.5, 0.5, 1.0};
+  static const double c3[3] = {0.0, 0.5, 1.0};
+  if (method == ADM_FORWARD_EULER) return 0.0;
+  return method == ADM_RK_FOURTH_ORDER ? c4[i] : c3[i];
+}
+
+// The step loop as a HIP graph: one step is captured on a private stream
+// and replayed n_steps times. Only the beam states change between steps;
+// they are evaluated on the host for every stage beforehand (the same scan
+// path search, in the same order, as the direct loop), uploaded as a table,
+// and the cell kernel reads row *d_slot of it, a counter that a one-lane
+// kernel advances after every operator application. No memset or copy is
+// captured.
+void Context::evolve_graph(int method, double t, double dt, int n_steps,
+                           double *T, hipStream_t s) {
+  const int ns = n_stages(method);
+  const size_t rows = (size_t)n_steps * ns;
+  if (rows > (size_t)INT32_MAX)
+    throw AdmError(ADM_ERR_INVALID, "too many steps for one captured loop");
+  ADM_HIP(hipStreamSynchronize(s));
+  if (rows > beam_table_rows) {
+    ADM_FREE(d_beam_table);
+    d_beam_table = nullptr;
+    beam_table_rows = 0;
+    ADM_HIP(hipMalloc(&d_beam_table,
+                      sizeof(adm_beam_state) * ADM_MAX_BEAMS * rows));
+    beam_table_rows = rows;
+  }
+  if (!d_slot) ADM_HIP(hipMalloc(&d_slot, sizeof(int32_t)));
+  // beam states of every stage, from the current scan-path position on
+  std::vector<adm_beam_state> table(rows * ADM_MAX_BEAMS);
+  std::memset(table.data(), 0, sizeof(adm_beam_state) * table.size());
+  double tk = t;
+  for (int k = 0; k < n_steps; ++k) {
+    for (int i = 0; i < ns; ++i) {
+      set_time(tk + stage_c(method, i) * dt);
+      for (size_t b = 0; b < beams.size(); ++b)
+        table[((size_t)k * ns + i) * ADM_MAX_BEAMS + b] = beam_states[b];
+    }
+    tk = tk + dt;
+  }
+  ADM_HIP(hipMemcpy(d_beam_table, table.data(),
+                    sizeof(adm_beam_state) * table.size(),
+                    hipMemcpyHostToDevice));
+  // capture one step; the kernels take their beams from the table
+  hipStream_t cs;
+  ADM_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const int32_t len = (int32_t)rows;
+  ADM_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+  explicit_step(
+      method, t, dt, T, sb,
+      [&](double, const double *x, double *r) {
+        CellArgs a = cell_args();
+        a.beam_table = d_beam_table;
+        a.beam_slot = d_slot;
+        a.beam_table_len = len;
+        launch(a, face_args(), x, r, 1, cs, nz_active);
+        hipLaunchKernelGGL(advance_slot_kernel, dim3(1), dim3(64), 0, cs,
+                           d_slot);
+      },
+      cs);
+  const hipError_t cap = hipStreamEndCapture(cs, &graph);
+  if (cap != hipSuccess) {
+    (void)hipStreamDestroy(cs);
+    throw AdmError(ADM_ERR_HIP, std::string("graph capture: ") +
+                                    hipGetErrorString(cap));
+  }
+  hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(set_slot_kernel, dim3(1), dim3(64), 0, cs, d_slot, 0);
+    for (int k = 0; k < n_steps && e == hipSuccess; ++k)
+      e = hipGraphLaunch(exec, cs);
+    if (e == hipSuccess) e = hipStreamSynchronize(cs);
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  (void)hipStreamDestroy(cs);
+  if (e != hipSuccess)
+    throw AdmError(ADM_ERR_HIP, std::string("graph replay: ") +
+                                    hipGetErrorString(e));
+}
+
 // ---------------------------------------------------------------------------
 // Ensemble
 // ---------------------------------------------------------------------------
@@ -1470,12 +1562,16 @@ int adm_evolve(void *ptr, int method, double t, double dt, int n_steps,
   if (method < ADM_FORWARD_EULER || method > ADM_RK_FOURTH_ORDER)
     return adm::set_error(ADM_ERR_INVALID, "unknown time stepping method");
   if (n_steps < 0) return adm::set_error(ADM_ERR_INVALID, "n_steps < 0");
-  (void)use_graph;
   hipStream_t s = static_cast<hipStream_t>(stream);
   ctx.ensure_step_buffers(method);
-  for (int k = 0; k < n_steps; ++k) {
-    ctx.step(method, t, dt, T, s);
-    t = t + dt;
+  if (use_graph && n_steps > 0) {
+    ctx.evolve_graph(method, t, dt, n_steps, T, s);
+    for (int k = 0; k < n_steps; ++k) t = t + dt;
+  } else {
+    for (int k = 0; k < n_steps; ++k) {
+      ctx.step(method, t, dt, T, s);
+      t = t + dt;
+    }
   }
   if (t_end) *t_end = t;
   ADM_API_END

@@ -82,6 +82,9 @@ struct Context {
   void compute_inverse_mass(double *out, hipStream_t s);
   void ensure_step_buffers(int method);
   void step(int method, double t, double dt, double *T, hipStream_t s);
+  // n_steps steps as replays of one captured step (beams from a table)
+  void evolve_graph(int method, double t, double dt, int n_steps, double *T,
+                    hipStream_t s);
 
   int device = 0;
   int p = 1, n1 = 2, nq = 8, nfq = 4;
@@ -123,6 +126,11 @@ struct Context {
   double *d_powder = nullptr, *d_face_powder = nullptr;
   FaceRef *d_faces = nullptr;
   StepBuffers sb;
+  // captured step loops (adm_evolve with use_graph): the beam states of
+  // every stage and the stage counter the kernels read them by
+  adm_beam_state *d_beam_table = nullptr;
+  size_t beam_table_rows = 0;
+  int32_t *d_slot = nullptr;
 };
 
 // Members of an ensemble: one Context each (own state, tables, face lists),

@@ -142,6 +142,11 @@ struct CellArgs {
   int32_t lext[3];
   int32_t nseg;
   int32_t n_rows;
+  // captured step loops: the beams come from row *beam_slot of beam_table
+  // (ADM_MAX_BEAMS states per row, beam_table_len rows) instead of `beams`
+  const adm_beam_state *beam_table;
+  const int32_t *beam_slot;
+  int32_t beam_table_len;
 };
 
 struct FaceArgs {

@@ -324,9 +324,16 @@ __global__ void __launch_bounds__(256)
     constexpr int words = sizeof(adm_beam_state) * ADM_MAX_BEAMS / 8;
     static_assert(sizeof(adm_beam_state) % 8 == 0, "beam state layout");
     const int tid = threadIdx.x;
+    const adm_beam_state *bsrc = a.beams;
+    if (a.beam_table) {
+      int slot = *as_global(a.beam_slot);
+      slot = slot < 0 ? 0 : (slot < a.beam_table_len ? slot
+                                                     : a.beam_table_len - 1);
+      bsrc = a.beam_table + (int64_t)slot * ADM_MAX_BEAMS;
+    }
     if (tid < words)
       reinterpret_cast<uint64_t *>(sbeams)[tid] =
-          reinterpret_cast<const uint64_t *>(a.beams)[tid];
+          reinterpret_cast<const uint64_t *>(bsrc)[tid];
     constexpr int nn = kMaxN * kMaxN;
     if (tid < nn) {
       (&sk.S[0][0])[tid] = (&kc.S[0][0])[tid];
@@ -354,6 +361,12 @@ __global__ void __launch_bounds__(256)
                                        a.beam_pmod[tid], a.beam_den[tid],
                                        kc.log_01);
     if (mb) __syncthreads();
+  }
+  // any beam on (from the table row's states when the beams come from it)
+  int sources_on = a.sources_on;
+  if (a.beam_table) {
+    sources_on = 0;
+    for (int b = 0; b < a.n_beams; ++b) sources_on |= sbeams[b].on ? 1 : 0;
   }
   // uniform constants (scalar loads; S and Dc are read from the LDS copy)
   double xq[N], wq[N];
@@ -484,7 +497,7 @@ __global__ void __launch_bounds__(256)
       const double *mB = mBall + mid * kMergedLds;
       const double *mC = mB + kMergedMax;
       bool near = false;
-      if (a.sources_on) {
+      if (sources_on) {
         double lo[3], hi[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -620,7 +633,7 @@ __global__ void __launch_bounds__(256)
         }
         t[q] = t[q] + ay;
         t[q] = t[q] + az;
-        if (a.sources_on) t[q] += val[q];
+        if (sources_on) t[q] += val[q];
       }
       wave_lds_sync();
 #pragma unroll

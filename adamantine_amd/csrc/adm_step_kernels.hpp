@@ -7,6 +7,15 @@
 
 namespace adm {
 
+// Stage counter of a captured step loop: set before the replays, advanced
+// after every operator application inside the graph.
+__global__ void set_slot_kernel(int32_t *slot, int32_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) slot[0] = v;
+}
+__global__ void advance_slot_kernel(int32_t *slot) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) slot[0] += 1;
+}
+
 __global__ void fill_kernel(double *__restrict__ x, double v, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] = v;

@@ -268,9 +268,11 @@ int adm_autotune(void *ctx, const double *src, double *dst, int32_t reps,
 
 /* Repeated evolve_one_time_step (source/ThermalPhysics.templates.hh:854-888):
  * n_steps steps of `method` from time t with step dt on the device solution
- * vector T (n_dofs doubles, updated in place). use_graph is reserved (the
- * steps are enqueued as direct launches whatever its value). t_end (host,
- * may be NULL) receives the final time. */
+ * vector T (n_dofs doubles, updated in place). use_graph = 1 captures one
+ * step as a HIP graph and replays it (the beam states of every stage are
+ * evaluated on the host beforehand and read from a device table); the
+ * result equals the direct launches'. t_end (host, may be NULL) receives
+ * the final time. */
 int adm_evolve(void *ctx, int method, double t, double dt, int n_steps,
                double *T, int use_graph, double *t_end, void *stream);
 
