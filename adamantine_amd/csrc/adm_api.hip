@@ -1204,20 +1204,33 @@ void Context::evolve_graph(int method, double t, double dt, int n_steps,
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   const int32_t len = (int32_t)rows;
-  ADM_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-  explicit_step(
-      method, t, dt, T, sb,
-      [&](double, const double *x, double *r) {
-        CellArgs a = cell_args();
-        a.beam_table = d_beam_table;
-        a.beam_slot = d_slot;
-        a.beam_table_len = len;
-        launch(a, face_args(), x, r, 1, cs, nz_active);
-        hipLaunchKernelGGL(advance_slot_kernel, dim3(1), dim3(64), 0, cs,
-                           d_slot);
-      },
-      cs);
-  const hipError_t cap = hipStreamEndCapture(cs, &graph);
+  hipError_t cap = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (cap != hipSuccess) {
+    (void)hipStreamDestroy(cs);
+    throw AdmError(ADM_ERR_HIP, std::string("graph capture: ") +
+                                    hipGetErrorString(cap));
+  }
+  try {
+    explicit_step(
+        method, t, dt, T, sb,
+        [&](double, const double *x, double *r) {
+          CellArgs a = cell_args();
+          a.beam_table = d_beam_table;
+          a.beam_slot = d_slot;
+          a.beam_table_len = len;
+          launch(a, face_args(), x, r, 1, cs, nz_active);
+          hipLaunchKernelGGL(advance_slot_kernel, dim3(1), dim3(64), 0, cs,
+                             d_slot);
+        },
+        cs);
+  } catch (...) {
+    // end the capture before the stream goes away
+    if (hipStreamEndCapture(cs, &graph) == hipSuccess && graph)
+      (void)hipGraphDestroy(graph);
+    (void)hipStreamDestroy(cs);
+    throw;
+  }
+  cap = hipStreamEndCapture(cs, &graph);
   if (cap != hipSuccess) {
     (void)hipStreamDestroy(cs);
     throw AdmError(ADM_ERR_HIP, std::string("graph capture: ") +
