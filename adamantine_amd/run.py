@@ -315,6 +315,39 @@ def run_ensemble(db: Dict, base_dir: str = ".",
                 raise NotImplementedError(
                     "perturbation of %s is not on the device path" % key)
         members.append(Member(absorption_efficiency=eta, max_power=power))
+    # checkpoint / restart: one file set per member, <prefix>_<member>, and
+    # <prefix>_time.txt (application/adamantine.hh:1630-1661, 1725-1795,
+    # 2380-2400)
+    ck = db.get("checkpoint")
+    every_ck = int(ck["time_steps_between_checkpoint"]) if ck else 0
+    restart = None
+    nc = p.n_cells_total
+    dcos = [np.asarray(p.deposition_cos if p.deposition_cos is not None
+                       else np.ones(nc), float).copy() for _ in range(n)]
+    dsin = [np.asarray(p.deposition_sin if p.deposition_sin is not None
+                       else np.zeros(nc), float).copy() for _ in range(n)]
+    if db.get("restart"):
+        import copy
+        prefix = str(db["restart"]["filename_prefix"])
+        saved = []
+        for j in range(n):
+            with np.load("%s_%d.npz" % (prefix, j), allow_pickle=False) as f:
+                saved.append({k: f[k] for k in f.files})
+        p = copy.deepcopy(p)
+        ucos, usin = np.ones(nc), np.zeros(nc)
+        for j, d in enumerate(saved):
+            on = d["active"].astype(bool)
+            members[j].cell_active = d["active"].astype(np.uint8)
+            members[j].cell_state = d["cell_state"]
+            dcos[j] = np.where(on, d["deposition_cos"], 1.0)
+            dsin[j] = np.where(on, d["deposition_sin"], 0.0)
+            ucos[on], usin[on] = dcos[j][on], dsin[j][on]
+        # a cell's orientation comes from its deposition box, the same in
+        # every member that has activated it
+        p.deposition_cos, p.deposition_sin = ucos, usin
+        with open(prefix + "_time.txt") as f:
+            tok = f.read().split()
+        restart = (saved, float(tok[0]), int(tok[1]))
     ens = EnsembleDevice(p, members)
     ens.compute_inverse_mass_matrix()
     # deposition boxes of member 0's sources, shared by all members
@@ -329,7 +362,13 @@ def run_ensemble(db: Dict, base_dir: str = ".",
     new_temperature = float(_get(db, "materials.new_material_temperature",
                                  300.0))
     sol = ens.initialize_dof_vector(0.0)
-    sol += torch.tensor(T0, dtype=torch.float64, device=sol.device)[:, None]
+    if restart:
+        sol.copy_(torch.tensor(np.stack([d["temperature"]
+                                         for d in restart[0]]),
+                               dtype=torch.float64, device=sol.device))
+    else:
+        sol += torch.tensor(T0, dtype=torch.float64,
+                            device=sol.device)[:, None]
 
     # augmented state (application/adamantine.hh:1600-1700)
     da_db = db.get("data_assimilation", {})
@@ -388,6 +427,14 @@ def run_ensemble(db: Dict, base_dir: str = ".",
     time, da_time = 0.0, -1.0
     frame_index = -1
     n_step = 1
+    if restart:
+        time, n_step = restart[1], restart[2] + 1
+        # the frame counter skips the frames before the restart time, as
+        # the reference's does (application/adamantine.hh:1854-1868)
+        if stamps:
+            while frame_index + 1 < len(stamps[0]) and \
+                    stamps[0][frame_index + 1] < time:
+                frame_index += 1
     activation_time_end = -1.0
     tol = dt * 1e-6
     run = EnsembleRun(sol, params, names, vals, ensemble=ens)
@@ -421,7 +468,10 @@ def run_ensemble(db: Dict, base_dir: str = ".",
                     b = ens.elements_to_activate(j, boxes[start:end])
                     ens.add_material(j, b, dep_cos[start:end],
                                      dep_sin[start:end], new_temperature, sol)
-                    active[j][b >= 0] = 1
+                    new = b >= 0
+                    active[j][new] = 1
+                    dcos[j][new] = np.asarray(dep_cos[start:end])[b[new]]
+                    dsin[j][new] = np.asarray(dep_sin[start:end])[b[new]]
                 run.n_activations += 1
         old_time = time
         time = ens.evolve(old_time, dt, 1, sol)
@@ -458,6 +508,22 @@ def run_ensemble(db: Dict, base_dir: str = ".",
                                 power[0] = float(params[j, i])
                         if names:
                             ens.set_beam_parameters(j, 0, eta[0], power[0])
+        if every_ck and n_step % every_ck == 0:
+            prefix = str(ck["filename_prefix"])
+            if str(ck.get("overwrite_files", "true")) != "true":
+                prefix += "_%d" % n_step
+            for j in range(n):
+                lq, pw = ens.qp_state(j)
+                on = active[j].astype(bool)
+                np.savez("%s_%d.npz" % (prefix, j),
+                         temperature=sol[j].cpu().numpy(),
+                         cell_state=_cell_state(lq, pw, p.material.n_states),
+                         active=active[j],
+                         deposition_cos=np.where(on, dcos[j], 10.0),
+                         deposition_sin=np.where(on, dsin[j], 10.0),
+                         has_melted=np.zeros(nc, np.uint8))
+            with open(prefix + "_time.txt", "w") as f:
+                f.write("%r %d\n" % (time, n_step))
         if pps and (n_step % every_out == 0 or
                     (out_on_da and da_time - half_window < time <
                      da_time + half_window)):

@@ -144,3 +144,35 @@ def test_run_ensemble_writes_vtu(tmp_path):
         nz = d["temperature"] != 0.0
         assert nz.any()
         np.testing.assert_array_equal(d["temperature"][nz], T[j][node[nz]])
+
+
+def test_run_ensemble_checkpoint_restart(tmp_path):
+    """the ensemble loop's checkpoint and restart (application/adamantine.hh:
+    1630-1795, 2380-2400) on the DA-while-adding-material input: a run that
+    checkpoints every 5 steps (files kept per step), and the same input
+    restarted from its step-5 checkpoint (an activation window and the
+    assimilation frame still ahead), end on the same member temperatures
+    (the reference's restart bar, 1e-10)"""
+    from adamantine_amd.run import run_ensemble
+    db = read_info(os.path.join(GOLD, "integration_da_add_material.info"))
+    prefix = str(tmp_path / "ens_ck")
+    db["checkpoint"] = {"filename_prefix": prefix, "overwrite_files": "false",
+                        "time_steps_between_checkpoint": "5"}
+    r1 = run_ensemble(db, GOLD)
+    T1 = r1.solution.cpu().numpy()
+    for j in range(T1.shape[0]):
+        assert os.path.exists("%s_5_%d.npz" % (prefix, j))
+    with open(prefix + "_5_time.txt") as f:
+        t_ck, n_ck = f.read().split()
+    assert int(n_ck) == 5
+    db2 = read_info(os.path.join(GOLD, "integration_da_add_material.info"))
+    db2["restart"] = {"filename_prefix": prefix + "_5"}
+    r2 = run_ensemble(db2, GOLD)
+    T2 = r2.solution.cpu().numpy()
+    print("  first run %d steps, DA at %s; restart at step 5 (t = %s): "
+          "to step %d, DA at %s" % (r1.n_steps, r1.times, t_ck,
+                                       r2.n_steps, r2.times), flush=True)
+    assert r2.time == r1.time and r2.times == r1.times
+    d = np.abs(T1 - T2)
+    print("  largest difference %.3e K" % d.max(), flush=True)
+    assert (d <= 1e-10 * np.abs(T1)).all()
