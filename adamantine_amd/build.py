@@ -81,6 +81,21 @@ def build_library(verbose: bool = True, jobs: int = 0) -> str:
     return LIB
 
 
+def build_cxx_demo(verbose: bool = True) -> str:
+    """The host C++ program on include/adm_cxx.hpp (tests/cxx/), linked
+    against the in-tree libadm.so."""
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    src = os.path.join(ROOT, "tests", "cxx", "adapter_demo.cpp")
+    out = os.path.join(ROOT, "tests", "cxx", "adapter_demo")
+    cmd = [hipcc, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+           src, "-L", os.path.dirname(LIB), "-ladm",
+           "-Wl,-rpath,$ORIGIN/../../adamantine_amd", "-o", out]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return out
+
+
 def build_oracle(verbose: bool = True):
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")],
                    check=True)
@@ -88,6 +103,7 @@ def build_oracle(verbose: bool = True):
 
 def main():
     build_library()
+    build_cxx_demo()
     build_oracle()
     return 0
 
