@@ -272,9 +272,8 @@ def main():
                          "far away, liquid in the pool) or U(300, 1600) noise")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the operator is not sharded across "
-              "GPUs in this build (see DESIGN.md, 'More than one GPU').",
-              file=sys.stderr)
+        print("bench.py measures one GPU: the operator is not sharded "
+              "across GPUs; --gpus must be 1.", file=sys.stderr)
         return 2
 
     import numpy as np

@@ -24,7 +24,8 @@
  * Nodes of the FE_Q(p) space (Gauss-Lobatto support points) are numbered
  * lexicographically, x fastest:  node = ix + Nx*(iy + Ny*iz),
  * N_d = p*n_cells_d + 1.  Cells are numbered c = cx + nx*(cy + ny*cz).
- * INTEGRATION.md gives the permutation to deal.II's DoF order.
+ * A deal.II DoF maps to node (round((x - x0)/h_x * p), ...) of its support
+ * point, which gives the permutation to deal.II's DoF order.
  */
 #ifndef ADM_H
 #define ADM_H
