@@ -740,6 +740,18 @@ inline void launch_vjp(Context &c, const double *Y, const double *pw,
   ADM_ADJ(3, 1, 2)
   ADM_ADJ(3, 2, 2)
   ADM_ADJ(3, 3, 2)
+  ADM_ADJ(4, 1, 0)
+  ADM_ADJ(4, 2, 0)
+  ADM_ADJ(4, 3, 0)
+  ADM_ADJ(4, 1, 2)
+  ADM_ADJ(4, 2, 2)
+  ADM_ADJ(4, 3, 2)
+  ADM_ADJ(5, 1, 0)
+  ADM_ADJ(5, 2, 0)
+  ADM_ADJ(5, 3, 0)
+  ADM_ADJ(5, 1, 2)
+  ADM_ADJ(5, 2, 2)
+  ADM_ADJ(5, 3, 2)
 #undef ADM_ADJ
   if (!launched)
     throw AdmError(ADM_ERR_UNSUPPORTED,

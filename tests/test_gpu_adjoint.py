@@ -351,3 +351,55 @@ def test_calibration_autograd():
     print(f"  s = {s}: fd {fd:.10e} autograd {float(th.grad[1]):.10e}",
           flush=True)
     assert abs(fd) > 0 and rel(float(th.grad[1]), fd) < 1e-5
+
+
+@pytest.mark.parametrize("deg", [4, 5])
+def test_adjoint_high_degree(deg):
+    """The adjoint at FE degrees 4 and 5 (RK4, SLP tables, faces): the
+    misfit equals the oracle's, and the absorption, radius and initial-
+    temperature gradients match central differences of the oracle's
+    forward run."""
+    import torch
+    from adamantine_amd.adjoint import adjoint_gradient
+    from adamantine_amd.thermal import ThermalPhysicsDevice
+    print(f"adjoint: degree {deg}, SLP, RK4", flush=True)
+    p = small_problem("q2_slp", powder=False, method=RK_FOURTH_ORDER,
+                      rotate=False)
+    p.fe_degree = deg
+    p.n_cells = (5, 3, 3)
+    t0, dt, n = 0.0, 1e-6, 4
+    T0 = hot_field(p, 1500.0, 1690.0)
+    X = p.node_coordinates()
+    dof = np.nonzero(np.isclose(X[:, 2], p.extent[2]))[0][::5]
+    _, Tref = _oracle_misfit(p, RK_FOURTH_ORDER, t0, dt, n, T0, dof,
+                             np.zeros(dof.size))
+    obs = Tref[dof] - 2.0
+    ph = ThermalPhysicsDevice(p, RK_FOURTH_ORDER)
+    ph.setup()
+    res = adjoint_gradient(ph, t0, dt, n, torch.tensor(T0, device="cuda"),
+                           dof, obs, checkpoint_every=2)
+    torch.cuda.synchronize()
+    Jo, _ = _oracle_misfit(p, RK_FOURTH_ORDER, t0, dt, n, T0, dof, obs)
+    assert rel(res.J, Jo) < 1e-7
+    b = p.beams[0]
+    base = [b.absorption_efficiency, b.max_power, b.radius, b.depth]
+
+    def J_at(beam=None, T=T0):
+        return _oracle_misfit(p, RK_FOURTH_ORDER, t0, dt, n, T, dof, obs,
+                              beam)[0]
+    d = 1e-4 * base[0]
+    fd = (J_at([base[0] + d] + base[1:]) - J_at([base[0] - d] + base[1:])) / \
+        (2 * d)
+    print(f"  eta: fd {fd:.10e} adjoint {res.grad[0]:.10e}", flush=True)
+    assert abs(fd) > 0 and rel(res.grad[0], fd) < 1e-5
+    d = 1e-5 * base[2]
+    fd = (J_at(base[:2] + [base[2] + d, base[3]]) -
+          J_at(base[:2] + [base[2] - d, base[3]])) / (2 * d)
+    print(f"  radius: fd {fd:.10e} adjoint {res.grad[2]:.10e}", flush=True)
+    assert abs(fd) > 0 and rel(res.grad[2], fd) < 1e-5
+    v = np.random.RandomState(9).uniform(-1, 1, p.n_dofs)
+    h = 1e-4
+    fd = (J_at(T=T0 + h * v) - J_at(T=T0 - h * v)) / (2 * h)
+    adj = float(np.dot(res.lambda0.cpu().numpy(), v))
+    print(f"  T0: fd {fd:.10e} adjoint {adj:.10e}", flush=True)
+    assert abs(fd) > 0 and rel(adj, fd) < 1e-5
