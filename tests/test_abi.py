@@ -77,3 +77,23 @@ def test_oracle_is_not_linked_by_the_library():
     out = subprocess.run(["ldd", thermal.LIB_PATH], capture_output=True,
                          text=True).stdout
     assert "oracle" not in out
+
+
+def test_cxx_adapter_compiles_against_the_header_alone(tmp_path):
+    """include/adm_cxx.hpp needs nothing but adm.h and the C++ standard
+    library: a host compiler checks it without HIP headers."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "adm_cxx.hpp"\n'
+                   'int f(const adm_problem_desc &d, double *x, double *y) {\n'
+                   '  adm_cxx::ThermalPhysics ph(d, ADM_RK_FOURTH_ORDER);\n'
+                   '  ph.get_thermal_operator().vmult(y, x);\n'
+                   '  return (int)ph.get_thermal_operator().m();\n}\n')
+    r = subprocess.run([gxx, "-std=c++17", "-Wall", "-Wextra", "-Werror",
+                        "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                        str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
