@@ -148,13 +148,16 @@ class Run:
 
 
 def run(db: Dict, base_dir: str = ".", fe_degree: Optional[int] = None,
-        verbose: bool = False, output: bool = False) -> Run:
+        verbose: bool = False, output: bool = False,
+        autotune: bool = True) -> Run:
     """run for the thermal physics on the device, with material deposition
     from the scan paths or a deposition file, and the checkpoint and restart
     sections of the input (files <prefix>.npz and <prefix>_time.txt).
     With output=True the post_processor section's VTU/PVTU/PVD files are
     written at the reference's cadence (the initial state, then every
-    time_steps_between_output steps)."""
+    time_steps_between_output steps). With autotune the operator's launch
+    configuration is picked by timing on the initial temperature (the
+    phase state is left as it was)."""
     from bisect import bisect_left
     from .deposition import create_material_deposition_boxes
     from .thermal import ThermalPhysicsDevice
@@ -173,6 +176,8 @@ def run(db: Dict, base_dir: str = ".", fe_degree: Optional[int] = None,
         time, n_time_step = float(tok[0]), int(tok[1])
     else:
         T = ph.initialize_dof_vector(p.initial_temperature)
+    if autotune:
+        ph.operator.autotune(T, reps=3)
     pp = None
     if output:
         # output_pvtu (application/adamantine.hh:69-129, 978-981)
@@ -271,13 +276,15 @@ def _cell_state(liquid: np.ndarray, powder: np.ndarray,
 
 def run_ensemble(db: Dict, base_dir: str = ".",
                  fe_degree: Optional[int] = None,
-                 verbose: bool = False, output: bool = False) -> EnsembleRun:
+                 verbose: bool = False, output: bool = False,
+                 autotune: bool = True) -> EnsembleRun:
     """run_ensemble for the thermal physics on the device. With output=True
     every member writes the post_processor section's VTU/PVTU/PVD files
     (prefix _m<member>) at the reference's cadence: the initial state, every
     time_steps_between_output steps, and around each assimilation frame
     unless output_on_data_assimilation is false
-    (application/adamantine.hh:1872-1885, 2416-2441)."""
+    (application/adamantine.hh:1872-1885, 2416-2441). With autotune the
+    batched launch configuration is picked by timing."""
     import torch
     from .ensemble import (DataAssimilator, EnsembleDevice, Member,
                            draw_member_values)
@@ -369,6 +376,8 @@ def run_ensemble(db: Dict, base_dir: str = ".",
     else:
         sol += torch.tensor(T0, dtype=torch.float64,
                             device=sol.device)[:, None]
+    if autotune:
+        ens.autotune(sol, reps=2)
 
     # augmented state (application/adamantine.hh:1600-1700)
     da_db = db.get("data_assimilation", {})

@@ -17,7 +17,7 @@ GOLD = os.path.join(os.path.dirname(__file__), "golden")
 
 
 # the reference's tolerance is 10 %; the bars below are the measured gaps
-# rounded up (2.0e-3 for Q3 with faces, 1.7e-4 for Q2 anisotropic, 1.76e-2
+# rounded up (1.7e-6 for Q3 with faces, 1.7e-4 for Q2 anisotropic, 1.76e-2
 # for the five-axis wall, whose melt pool reaches 5400 K; the CPU oracle
 # replaying the same loop gives the same 1.76e-2)
 @pytest.mark.parametrize("name,gold,lo,hi,bar", [
