@@ -186,7 +186,9 @@ def test_adjoint_gradient_matches_finite_differences(method, powder):
     ph2.setup()
     sol = torch.tensor(T0, device="cuda")
     ph2.evolve(t0, dt, n, sol)
-    np.testing.assert_array_equal(res.T_final.cpu().numpy(), sol.cpu().numpy())
+    # equal up to the order of the face kernel's atomic adds
+    np.testing.assert_allclose(res.T_final.cpu().numpy(), sol.cpu().numpy(),
+                               rtol=1e-13, atol=0)
     Jo, _ = _oracle_misfit(p, method, t0, dt, n, T0, dof, obs)
     assert rel(res.J, Jo) < 1e-7
     b = p.beams[0]
@@ -226,7 +228,7 @@ def test_adjoint_gradient_matches_finite_differences(method, powder):
                             dof, obs, checkpoint_every=0)
     for k in range(3):
         assert rel(res1.grad[k], res.grad[k]) < 1e-10
-    assert res1.J == res.J
+    assert rel(res1.J, res.J) < 1e-13
 
 
 def test_adjoint_rejects_bad_arguments():
@@ -326,7 +328,7 @@ def test_calibration_autograd():
     res = adjoint_gradient(ph, t0, dt, n, torch.tensor(T0h, device="cuda"),
                            dof, obs, checkpoint_every=3)
     torch.cuda.synchronize()
-    assert float(J.detach()) == res.J
+    assert rel(float(J.detach()), res.J) < 1e-13
     for k in range(3):
         assert rel(float(theta.grad[k]), res.grad[k]) < 1e-10
     # the adjoint accumulates with atomics: equal up to summation order
