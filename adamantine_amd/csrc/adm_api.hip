@@ -350,6 +350,7 @@ int Context::property_kind() const {
 Context::Context(const adm_problem_desc &d) {
   if (const char *e = std::getenv("ADM_CELL_ATOMICS"))
     use_coloring = !(e[0] == '1');
+  if (const char *e = std::getenv("ADM_XCD_ORDER")) xcd_order = e[0] == '1';
   {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
@@ -911,6 +912,7 @@ void Context::set_time(double t) {
 CellArgs Context::cell_args() const {
   CellArgs a;
   std::memset(&a, 0, sizeof(a));
+  a.xcd_remap = xcd_order;
   a.meta = d_meta;
   a.powder_slot = d_powder_slot;
   a.powder = d_powder;

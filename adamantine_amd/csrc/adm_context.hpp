@@ -112,6 +112,9 @@ struct Context {
   // cell kernel in 8 parity colors without atomics (default), or one
   // atomic launch (ADM_CELL_ATOMICS=1 in the environment at creation)
   bool use_coloring = true;
+  // row segments in XCD-contiguous block order (default; ADM_XCD_ORDER=0 in
+  // the environment at creation restores plain block order)
+  bool xcd_order = true;
   // wavefronts (row segments) per block of the cell kernel, 2..4
   int cell_waves = 4;
   int n_cu = 256;  // compute units of the device (grid sizing)

@@ -147,6 +147,8 @@ struct CellArgs {
   const adm_beam_state *beam_table;
   const int32_t *beam_slot;
   int32_t beam_table_len;
+  // nonzero: give each XCD a contiguous run of row segments (xcd_block)
+  int32_t xcd_remap;
 };
 
 struct FaceArgs {

@@ -270,6 +270,15 @@ struct RowCell {
   int cc[3];
   bool ok;
 };
+// Workgroups are dispatched round-robin over the 8 XCDs (block b on XCD
+// b % 8). Map block b to a logical block so that each XCD runs one
+// contiguous run of the grid: row segments that share node lines (the
+// neighbours in y and z) then share that XCD's L2. A bijection on [0, g).
+__device__ __forceinline__ int xcd_block(int b, int g) {
+  const int q = g >> 3, rem = g & 7, x = b & 7, k = b >> 3;
+  return x < rem ? x * (q + 1) + k : rem * (q + 1) + (x - rem) * q + k;
+}
+
 __device__ __forceinline__ RowCell row_cell(const CellArgs &a, int r, int j,
                                             int cpw) {
   RowCell rc;
@@ -396,7 +405,8 @@ __global__ void __launch_bounds__(256)
   const double *mBall = lds;
 
   // this wavefront's row segment
-  const int r = blockIdx.x * (blockDim.x >> 6) + wave;
+  const int bx = a.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const int r = bx * (blockDim.x >> 6) + wave;
   if (r >= a.n_rows) return;
   const RowCell rc = row_cell(a, r, j, CPW);
   if (!rc.ok) return;  // whole cells (all lanes) leave together

@@ -24,10 +24,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes of one operator application of this workload (the cell kernel
-# as 8 colour launches or one atomic launch, 4 wavefronts per block, + the
-# face kernel), from rocprofv3 FETCH_SIZE and WRITE_SIZE runs scaled by a
-# 256 MiB copy (profiles/hbm_traffic_plain_large.md); keyed by use_coloring
-MEASURED_TRAFFIC_BYTES = {True: 290.4e6, False: 198.0e6}
+# as 8 colour launches or one atomic launch, 4 wavefronts per block, blocks
+# in XCD-contiguous order, + the face kernel), from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs scaled by a 256 MiB copy
+# (profiles/hbm_traffic_plain_large.md); keyed by use_coloring
+MEASURED_TRAFFIC_BYTES = {True: 355.6e6, False: 176.0e6}
 
 
 def algorithmic_bytes(problem, n_powder_qp: int, n_active_cells: int) -> int:
