@@ -56,10 +56,11 @@ def adjoint_gradient(physics: ThermalPhysicsDevice, t0: float, dt: float,
     m = physics.method if method is None else method
     check(load_library().adm_adjoint_gradient(
         op.handle, int(m), float(t0), float(dt), int(n_steps),
-        int(checkpoint_every), _dev_ptr(T0), int(dof.size),
+        int(checkpoint_every), _dev_ptr(T0, op.m()), int(dof.size),
         dof.ctypes.data_as(C.POINTER(C.c_int64)), y.ctypes.data_as(_dp),
-        None if w is None else w.ctypes.data_as(_dp), _dev_ptr(T_final),
-        C.byref(J), g.ctypes.data_as(_dp), _dev_ptr(lam), _stream_ptr(stream)))
+        None if w is None else w.ctypes.data_as(_dp), _dev_ptr(T_final, op.m()),
+        C.byref(J), g.ctypes.data_as(_dp), _dev_ptr(lam, op.m()),
+        _stream_ptr(stream)))
     return AdjointResult(J.value, g, T_final, lam)
 
 
@@ -70,7 +71,8 @@ def operator_vjp(op, t: float, T, mu, stream=None):
     out = torch.empty_like(T)
     g = np.zeros(3)
     check(load_library().adm_operator_vjp(
-        op.handle, float(t), _dev_ptr(T), _dev_ptr(mu), _dev_ptr(out),
+        op.handle, float(t), _dev_ptr(T, op.m()), _dev_ptr(mu, op.m()),
+        _dev_ptr(out, op.m()),
         g.ctypes.data_as(_dp), _stream_ptr(stream)))
     return out, g
 
@@ -85,7 +87,12 @@ class Calibration:
     value or polynomial coefficient of ``problem``; a new s rebuilds the
     device operator (the adjoint gives the derivative around the operator
     as built, so dJ/ds = (dJ/ds')|_{s'=1} / s). Absorption efficiency and
-    radius go through update_physics_parameters without a rebuild."""
+    radius go through update_physics_parameters without a rebuild.
+
+    Every run starts from the phase state the operator was built with (the
+    point and face powder and liquid ratios are saved at the build and put
+    back before each run), so J and its gradient do not depend on the runs
+    made before."""
 
     def __init__(self, problem, t0: float, dt: float, n_steps: int, obs_dof,
                  obs, obs_weight=None, checkpoint_every: int = 0,
@@ -97,6 +104,7 @@ class Calibration:
         self.method = problem.method if method is None else method
         self._physics = None
         self._scale = None
+        self._state0 = None
 
     def physics_for(self, theta) -> ThermalPhysicsDevice:
         import copy
@@ -108,6 +116,13 @@ class Calibration:
             self._physics = ThermalPhysicsDevice(p, self.method)
             self._physics.setup()
             self._scale = s
+            op = self._physics.operator
+            self._state0 = (op.qp_state(), op.face_state())
+        else:
+            (lq, pw), fp = self._state0
+            op = self._physics.operator
+            op.set_qp_state(lq, pw)
+            op.set_face_state(fp)
         b = self.problem.beams[0]
         self._physics.update_physics_parameters(0, eta, b.max_power, radius,
                                                 b.depth)

@@ -20,6 +20,7 @@
 #include "adm_context.hpp"
 #include "adm_kernels.hpp"
 #include "adm_launch.hpp"
+#include "adm_plane_kernel.hpp"
 #include "adm_step_kernels.hpp"
 
 namespace adm {
@@ -339,6 +340,32 @@ void Context::build_merged_tables() {
           mt.coef[pr][j][s][0] = A;
           mt.coef[pr][j][s][1] = S;
         }
+    // bins of the one-lookup interval search (MergedBins)
+    MergedBins &mb = mc.bins[mid];
+    std::memset(&mb, 0, sizeof(mb));
+    const double lo = B[0], hi = B[nB - 1];
+    const double w = (hi - lo) / kMergedBins;
+    mb.lo = lo;
+    mb.inv_w = w > 0.0 ? 1.0 / w : 0.0;
+    const double margin =
+        1e-9 * (hi - lo) + 1e-12 * std::max(std::fabs(lo), std::fabs(hi));
+    int maxc = 1;
+    for (int b = 0; b < kMergedBins; ++b) {
+      const double rlo = b == 0 ? -HUGE_VAL : lo + b * w - margin;
+      const double rhi = b == kMergedBins - 1 || w == 0.0
+                             ? HUGE_VAL
+                             : lo + (b + 1) * w + margin;
+      int k0 = 0, cnt = 0;
+      for (int k = 0; k < nB; ++k) {
+        if (B[k] < rlo)
+          ++k0;
+        else if (B[k] <= rhi)
+          ++cnt;
+      }
+      mb.k0[b] = (uint8_t)k0;
+      maxc = std::max(maxc, cnt);
+    }
+    mb.maxc = maxc;
   }
 }
 
@@ -350,6 +377,10 @@ int Context::property_kind() const {
 Context::Context(const adm_problem_desc &d) {
   if (const char *e = std::getenv("ADM_CELL_ATOMICS"))
     use_coloring = !(e[0] == '1');
+  {
+    const char *e = std::getenv("ADM_CELL_KERNEL");
+    use_plane = d.mesh.fe_degree <= 2 && !(e && std::string(e) == "line");
+  }
   if (const char *e = std::getenv("ADM_XCD_ORDER")) xcd_order = e[0] == '1';
   {
     int dev = 0, cus = 0;
@@ -398,6 +429,8 @@ Context::Context(const adm_problem_desc &d) {
   // cell and row-segment counts are 32-bit in the kernels
   if (n_cells > (int64_t)INT32_MAX)
     throw AdmError(ADM_ERR_UNSUPPORTED, "more than 2^31 - 1 cells");
+  // the plane kernel's gather pass indexes nodes with 32-bit integers
+  if (n_dofs >= (int64_t)INT32_MAX) use_plane = false;
   n_states = mat.n_states;
   use_table = mat.use_table != 0;
 
@@ -588,6 +621,65 @@ Context::~Context() {
   ADM_FREE(d_faces);
   ADM_FREE(d_beam_table);
   ADM_FREE(d_slot);
+  ADM_FREE(d_patch);
+  clear_dof_map();
+}
+
+void Context::clear_dof_map() {
+  ADM_FREE(d_dof_map);
+  ADM_FREE(d_map_src);
+  ADM_FREE(d_map_dst);
+  d_dof_map = nullptr;
+  d_map_src = d_map_dst = nullptr;
+  n_ext = -1;
+}
+
+// The map must be injective into the box nodes and cover every node of
+// every active cell (the nodes the operator reads and writes); the other
+// nodes hold zeros in the node vectors.
+void Context::set_dof_map(int64_t n, const int64_t *map) {
+  if (n < 0 || (n > 0 && !map))
+    throw AdmError(ADM_ERR_INVALID, "DoF map: negative size or null map");
+  std::vector<uint8_t> seen((size_t)n_dofs, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    if (map[i] < 0 || map[i] >= n_dofs)
+      throw AdmError(ADM_ERR_INVALID, "DoF map: entry out of the box nodes");
+    if (seen[(size_t)map[i]]++)
+      throw AdmError(ADM_ERR_INVALID, "DoF map: a node is mapped twice");
+  }
+  const std::vector<uint8_t> na = node_activity();
+  for (int64_t k = 0; k < n_dofs; ++k)
+    if (na[(size_t)k] && !seen[(size_t)k])
+      throw AdmError(ADM_ERR_INVALID,
+                     "DoF map: a node of an active cell has no DoF");
+  ADM_HIP(hipDeviceSynchronize());
+  clear_dof_map();
+  ADM_HIP(hipMalloc(&d_dof_map, sizeof(int64_t) * std::max<int64_t>(n, 1)));
+  ADM_HIP(hipMalloc(&d_map_src, sizeof(double) * n_dofs));
+  ADM_HIP(hipMalloc(&d_map_dst, sizeof(double) * n_dofs));
+  if (n)
+    ADM_HIP(hipMemcpy(d_dof_map, map, sizeof(int64_t) * n,
+                      hipMemcpyHostToDevice));
+  ADM_HIP(hipMemset(d_map_src, 0, sizeof(double) * n_dofs));
+  ADM_HIP(hipMemset(d_map_dst, 0, sizeof(double) * n_dofs));
+  n_ext = n;
+}
+
+void Context::to_nodes(const double *ext, double *node, hipStream_t s) const {
+  if (n_ext <= 0) return;
+  hipLaunchKernelGGL(dofs_to_nodes_kernel,
+                     dim3((unsigned)((n_ext + 255) / 256)), dim3(256), 0, s,
+                     ext, d_dof_map, n_ext, node);
+  ADM_HIP(hipGetLastError());
+}
+
+void Context::to_dofs(const double *node, double *ext, int add,
+                      hipStream_t s) const {
+  if (n_ext <= 0) return;
+  hipLaunchKernelGGL(nodes_to_dofs_kernel,
+                     dim3((unsigned)((n_ext + 255) / 256)), dim3(256), 0, s,
+                     node, d_dof_map, n_ext, ext, add);
+  ADM_HIP(hipGetLastError());
 }
 
 void Context::upload_constants() {
@@ -965,9 +1057,65 @@ void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
   const size_t lds = pk == 2 ? sizeof(double) * kMergedLds * kc.mat.n_materials
                              : 0;
   const unsigned M = n_members;
-  // the cell kernel runs once per parity color (cells of one color share no
+  if (use_plane && p <= 2) {
+    // one launch, one PX x PY cell patch of a cell layer per wavefront
+    const size_t plds =
+        pk == 2 ? sizeof(double) * kPlaneMergedLds * kc.mat.n_materials : 0;
+    CellArgs ac = a;
+    ac.color = -1;
+    for (int d = 0; d < 3; ++d) ac.lext[d] = lim[d];
+    const int px = p == 1 ? PlaneLanes<1>::PX : PlaneLanes<2>::PX;
+    const int py = p == 1 ? PlaneLanes<1>::PY : PlaneLanes<2>::PY;
+    ac.plane_npx = (lim[0] + px - 1) / px;
+    ac.plane_npy = (lim[1] + py - 1) / py;
+    ac.n_rows = ac.plane_npx * ac.plane_npy * lim[2];
+    const int64_t slots = p == 1 ? PlaneLanes<1>::patch_nodes
+                                 : PlaneLanes<2>::patch_nodes;
+    ac.plane_E_stride = (int64_t)ac.n_rows * slots;
+    ac.plane_E = ensure_patch_buffer((size_t)ac.plane_E_stride * M);
+    if (ac.n_rows > 0) {
+      // a persistent grid: as many blocks as the chip holds at the
+      // kernel's occupancy (shared among the members), each looping over
+      // patches; at least 8 blocks, one per XCD
+      const int64_t full = (ac.n_rows + cell_waves - 1) / cell_waves;
+      const int64_t resident =
+          (int64_t)n_cu * 4 * kPlaneWavesPerEU / cell_waves / (int64_t)M;
+      const int64_t nb = std::min(full, std::max<int64_t>(8, resident));
+#define ADM_DISPATCH(P_, NS_)                                              \
+  if (p == P_ && n_states == NS_) {                                         \
+    if (pk == 0)                                                            \
+      launch_plane<P_, NS_, 0>(ac, src, dst, nb, plds, M, cell_waves, s);   \
+    else if (pk == 1)                                                       \
+      launch_plane<P_, NS_, 1>(ac, src, dst, nb, plds, M, cell_waves, s);   \
+    else                                                                    \
+      launch_plane<P_, NS_, 2>(ac, src, dst, nb, plds, M, cell_waves, s);   \
+  }
+      ADM_DISPATCH(1, 1)
+      ADM_DISPATCH(1, 2)
+      ADM_DISPATCH(1, 3)
+      ADM_DISPATCH(2, 1)
+      ADM_DISPATCH(2, 2)
+      ADM_DISPATCH(2, 3)
+#undef ADM_DISPATCH
+    }
+    // second pass: every node's sum from the patch buffer into dst
+    const dim3 ggrid((unsigned)((n_dofs + 255) / 256), M);
+    if (p == 1)
+      hipLaunchKernelGGL(plane_gather_kernel<1>, ggrid, dim3(256), 0, s,
+                         ac.plane_E, ac.plane_E_stride, dst,
+                         a.members ? a.member_stride : (int64_t)0, nn[0],
+                         nn[1], nn[2], ac.plane_npx, ac.plane_npy, lim[2],
+                         a.add);
+    else
+      hipLaunchKernelGGL(plane_gather_kernel<2>, ggrid, dim3(256), 0, s,
+                         ac.plane_E, ac.plane_E_stride, dst,
+                         a.members ? a.member_stride : (int64_t)0, nn[0],
+                         nn[1], nn[2], ac.plane_npx, ac.plane_npy, lim[2],
+                         a.add);
+  }
+  // the line kernel runs once per parity color (cells of one color share no
   // node: plain read-add-write of dst), or once with atomics
-  const int n_colors = use_coloring ? 8 : 1;
+  const int n_colors = (use_plane && p <= 2) ? 0 : (use_coloring ? 8 : 1);
   const int cpw = cells_per_wave(p);
   for (int col = 0; col < n_colors; ++col) {
     CellArgs ac = a;
@@ -1041,6 +1189,27 @@ void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
 #undef ADM_DISPATCH
   }
   ADM_HIP(hipGetLastError());
+}
+
+size_t Context::patch_buffer_size(int nz_rows, unsigned n_members) const {
+  const int nz = nz_rows >= 0 ? std::min(nz_rows, nc[2]) : nc[2];
+  const int px = p == 1 ? PlaneLanes<1>::PX : PlaneLanes<2>::PX;
+  const int py = p == 1 ? PlaneLanes<1>::PY : PlaneLanes<2>::PY;
+  const size_t slots =
+      p == 1 ? PlaneLanes<1>::patch_nodes : PlaneLanes<2>::patch_nodes;
+  return (size_t)((nc[0] + px - 1) / px) * ((nc[1] + py - 1) / py) * nz *
+         slots * n_members;
+}
+
+double *Context::ensure_patch_buffer(size_t n) const {
+  if (n > patch_cap) {
+    if (d_patch) ADM_HIP(hipFree(d_patch));
+    d_patch = nullptr;
+    patch_cap = 0;
+    ADM_HIP(hipMalloc(&d_patch, sizeof(double) * n));
+    patch_cap = n;
+  }
+  return d_patch;
 }
 
 void Context::fill(double *x, double v, int64_t n, hipStream_t s) {
@@ -1200,7 +1369,9 @@ void Context::evolve_graph(int method, double t, double dt, int n_steps,
   ADM_HIP(hipMemcpy(d_beam_table, table.data(),
                     sizeof(adm_beam_state) * table.size(),
                     hipMemcpyHostToDevice));
-  // capture one step; the kernels take their beams from the table
+  // capture one step; the kernels take their beams from the table (and
+  // the plane kernel's patch buffer is allocated now, not in the capture)
+  if (plane_path()) ensure_patch_buffer(patch_buffer_size(nz_active, 1));
   hipStream_t cs;
   ADM_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
   hipGraph_t graph = nullptr;
@@ -1428,7 +1599,27 @@ const char *adm_last_error(void) { return adm::g_last_error.c_str(); }
 int adm_n_dofs(const void *ptr, int64_t *n) {
   ADM_API_BEGIN
   if (!ptr || !n) return adm::set_error(ADM_ERR_INVALID, "null argument");
+  const Context &c = *static_cast<const Context *>(ptr);
+  *n = c.n_ext >= 0 ? c.n_ext : c.n_dofs;
+  ADM_API_END
+}
+
+int adm_n_nodes(const void *ptr, int64_t *n) {
+  ADM_API_BEGIN
+  if (!ptr || !n) return adm::set_error(ADM_ERR_INVALID, "null argument");
   *n = static_cast<const Context *>(ptr)->n_dofs;
+  ADM_API_END
+}
+
+int adm_set_dof_map(void *ptr, int64_t n_dofs, const int64_t *dof_to_node) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  if (n_dofs < 0) {
+    ADM_HIP(hipDeviceSynchronize());
+    ctx.clear_dof_map();
+  } else {
+    ctx.set_dof_map(n_dofs, dof_to_node);
+  }
   ADM_API_END
 }
 
@@ -1446,7 +1637,13 @@ int adm_compute_inverse_mass(void *ptr, double *inv_mass, void *stream) {
   ADM_API_BEGIN
   ADM_CTX(ptr)
   if (!inv_mass) return adm::set_error(ADM_ERR_INVALID, "null output");
-  ctx.compute_inverse_mass(inv_mass, static_cast<hipStream_t>(stream));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (ctx.n_ext >= 0) {
+    ctx.compute_inverse_mass(ctx.d_map_dst, s);
+    ctx.to_dofs(ctx.d_map_dst, inv_mass, 0, s);
+  } else {
+    ctx.compute_inverse_mass(inv_mass, s);
+  }
   ADM_API_END
 }
 
@@ -1463,20 +1660,44 @@ int adm_vmult(void *ptr, const double *src, double *dst, int add,
   ADM_CTX(ptr)
   if (!src || !dst) return adm::set_error(ADM_ERR_INVALID, "null vector");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  // nodes on brick faces are accumulated atomically: start from zero
-  if (!add) ctx.fill(dst, 0.0, ctx.n_dofs, s);
-  ctx.apply(src, dst, add, s);
+  if (ctx.n_ext >= 0) {
+    // vectors in the caller's DoF order: to box nodes, apply, back
+    ctx.to_nodes(src, ctx.d_map_src, s);
+    if (!ctx.plane_path()) ctx.fill(ctx.d_map_dst, 0.0, ctx.n_dofs, s);
+    ctx.apply(ctx.d_map_src, ctx.d_map_dst, 0, s);
+    ctx.to_dofs(ctx.d_map_dst, dst, add, s);
+  } else {
+    // the line kernel accumulates into dst: start from zero (the plane
+    // kernel's gather pass writes every node)
+    if (!add && !ctx.plane_path()) ctx.fill(dst, 0.0, ctx.n_dofs, s);
+    ctx.apply(src, dst, add, s);
+  }
   ADM_API_END
 }
 
-int adm_set_launch_config(void *ptr, int32_t use_coloring,
-                          int32_t waves_per_block) {
+// checks a launch configuration for a context of FE degree p
+static int check_launch_config(int p, int64_t n_dofs, int32_t mode,
+                               int32_t waves) {
+  if (waves < 2 || waves > 4)
+    return adm::set_error(ADM_ERR_INVALID, "waves_per_block must be 2..4");
+  if (mode < ADM_LAUNCH_LINE_ATOMIC || mode > ADM_LAUNCH_PLANE)
+    return adm::set_error(ADM_ERR_INVALID, "unknown launch mode");
+  if (mode == ADM_LAUNCH_PLANE && p > 2)
+    return adm::set_error(ADM_ERR_UNSUPPORTED,
+                          "the plane kernel exists for FE degrees 1 and 2");
+  if (mode == ADM_LAUNCH_PLANE && n_dofs >= (int64_t)INT32_MAX)
+    return adm::set_error(ADM_ERR_UNSUPPORTED,
+                          "the plane kernel takes fewer than 2^31 nodes");
+  return ADM_OK;
+}
+
+int adm_set_launch_config(void *ptr, int32_t mode, int32_t waves_per_block) {
   ADM_API_BEGIN
   ADM_CTX(ptr)
-  if (waves_per_block < 2 || waves_per_block > 4)
-    return adm::set_error(ADM_ERR_INVALID, "waves_per_block must be 2..4");
+  if (int e = check_launch_config(ctx.p, ctx.n_dofs, mode, waves_per_block))
+    return e;
   ADM_HIP(hipDeviceSynchronize());
-  ctx.use_coloring = use_coloring != 0;
+  ctx.set_launch_mode(mode);
   ctx.cell_waves = waves_per_block;
   ADM_API_END
 }
@@ -1489,7 +1710,7 @@ template <class Apply>
 static void tune_launch(adm::Context &cfg,
                         const std::vector<adm::Context *> &holders,
                         Apply apply, int reps, hipStream_t s,
-                        int32_t *use_coloring_out, int32_t *waves_out,
+                        int32_t *mode_out, int32_t *waves_out,
                         double *ms_out) {
   using namespace adm;
   ADM_HIP(hipStreamSynchronize(s));
@@ -1520,10 +1741,17 @@ static void tune_launch(adm::Context &cfg,
   ADM_HIP(hipEventCreate(&e0));
   ADM_HIP(hipEventCreate(&e1));
   double best = 1e300;
-  int best_col = cfg.use_coloring ? 1 : 0, best_w = cfg.cell_waves, k = 0;
-  for (int col = 1; col >= 0; --col)
+  int best_mode = cfg.launch_mode(), best_w = cfg.cell_waves, k = 0;
+  const int modes[3] = {ADM_LAUNCH_LINE_COLORED, ADM_LAUNCH_LINE_ATOMIC,
+                        ADM_LAUNCH_PLANE};
+  for (int mi = 0; mi < 3; ++mi)
     for (int w = 2; w <= 4; ++w, ++k) {
-      cfg.use_coloring = col != 0;
+      const int mode = modes[mi];
+      if (mode == ADM_LAUNCH_PLANE && (cfg.p > 2 || cfg.n_dofs >= (int64_t)INT32_MAX)) {
+        if (ms_out) ms_out[k] = -1.0;
+        continue;
+      }
+      cfg.set_launch_mode(mode);
       cfg.cell_waves = w;
       apply();  // warm-up
       ADM_HIP(hipEventRecord(e0, s));
@@ -1536,7 +1764,7 @@ static void tune_launch(adm::Context &cfg,
       if (ms_out) ms_out[k] = per;
       if (per < best) {
         best = per;
-        best_col = col;
+        best_mode = mode;
         best_w = w;
       }
       restore();
@@ -1544,16 +1772,16 @@ static void tune_launch(adm::Context &cfg,
   ADM_HIP(hipEventDestroy(e0));
   ADM_HIP(hipEventDestroy(e1));
   for (const Saved &v : saved) ADM_HIP(hipFree(v.copy));
-  cfg.use_coloring = best_col != 0;
+  cfg.set_launch_mode(best_mode);
   cfg.cell_waves = best_w;
-  if (use_coloring_out) *use_coloring_out = best_col;
+  if (mode_out) *mode_out = best_mode;
   if (waves_out) *waves_out = best_w;
 }
 }  // extern "C++"
 
 int adm_autotune(void *ptr, const double *src, double *dst, int32_t reps,
-                 void *stream, int32_t *use_coloring_out,
-                 int32_t *waves_out, double *ms_out) {
+                 void *stream, int32_t *mode_out, int32_t *waves_out,
+                 double *ms_out) {
   ADM_API_BEGIN
   ADM_CTX(ptr)
   if (!src || !dst || reps < 1)
@@ -1562,10 +1790,10 @@ int adm_autotune(void *ptr, const double *src, double *dst, int32_t reps,
   tune_launch(
       ctx, {&ctx},
       [&]() {
-        ctx.fill(dst, 0.0, ctx.n_dofs, s);
+        if (!ctx.plane_path()) ctx.fill(dst, 0.0, ctx.n_dofs, s);
         ctx.apply(src, dst, 0, s);
       },
-      reps, s, use_coloring_out, waves_out, ms_out);
+      reps, s, mode_out, waves_out, ms_out);
   ADM_API_END
 }
 
@@ -1579,6 +1807,14 @@ int adm_evolve(void *ptr, int method, double t, double dt, int n_steps,
   if (n_steps < 0) return adm::set_error(ADM_ERR_INVALID, "n_steps < 0");
   hipStream_t s = static_cast<hipStream_t>(stream);
   ctx.ensure_step_buffers(method);
+  // with a DoF map the solution is in the caller's DoF order: one gather
+  // into the box-node vector before the steps and one scatter after them
+  double *T_ext = nullptr;
+  if (ctx.n_ext >= 0) {
+    T_ext = T;
+    T = ctx.d_map_src;
+    ctx.to_nodes(T_ext, T, s);
+  }
   if (use_graph && n_steps > 0) {
     ctx.evolve_graph(method, t, dt, n_steps, T, s);
     for (int k = 0; k < n_steps; ++k) t = t + dt;
@@ -1588,6 +1824,7 @@ int adm_evolve(void *ptr, int method, double t, double dt, int n_steps,
       t = t + dt;
     }
   }
+  if (T_ext) ctx.to_dofs(T, T_ext, 0, s);
   if (t_end) *t_end = t;
   ADM_API_END
 }
@@ -1653,6 +1890,26 @@ int adm_get_face_state(void *ptr, double *face_powder) {
       for (int q = 0; q < nf; ++q)
         face_powder[(c * 6 + fid) * nf + q] = fpw[f * nf + q];
     }
+  }
+  ADM_API_END
+}
+
+int adm_set_face_state(void *ptr, const double *face_powder) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  if (!face_powder) return adm::set_error(ADM_ERR_INVALID, "null input");
+  const int nf = ctx.nfq;
+  if (ctx.d_face_powder && !ctx.faces.empty()) {
+    std::vector<double> fpw(ctx.faces.size() * nf);
+    for (size_t f = 0; f < ctx.faces.size(); ++f) {
+      const int64_t c = ctx.faces[f].cell;
+      const int fid = ctx.faces[f].info & 7;
+      for (int q = 0; q < nf; ++q)
+        fpw[f * nf + q] = face_powder[(c * 6 + fid) * nf + q];
+    }
+    ADM_HIP(hipDeviceSynchronize());
+    ADM_HIP(hipMemcpy(ctx.d_face_powder, fpw.data(),
+                      sizeof(double) * fpw.size(), hipMemcpyHostToDevice));
   }
   ADM_API_END
 }
@@ -1772,27 +2029,28 @@ int adm_ensemble_vmult(void *ptr, const double *src, double *dst, int add,
   ADM_ENS(ptr)
   if (!src || !dst) return adm::set_error(ADM_ERR_INVALID, "null vector");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (!add) ens.members[0]->fill(dst, 0.0, ens.n_dofs * ens.n_members, s);
+  if (!add && !ens.members[0]->plane_path())
+    ens.members[0]->fill(dst, 0.0, ens.n_dofs * ens.n_members, s);
   ens.apply(src, dst, add, s);
   ADM_API_END
 }
 
-int adm_ensemble_set_launch_config(void *ptr, int32_t use_coloring,
+int adm_ensemble_set_launch_config(void *ptr, int32_t mode,
                                    int32_t waves_per_block) {
   ADM_API_BEGIN
   ADM_ENS(ptr)
-  if (waves_per_block < 2 || waves_per_block > 4)
-    return adm::set_error(ADM_ERR_INVALID, "waves_per_block must be 2..4");
+  if (int e = check_launch_config(ens.members[0]->p, ens.n_dofs, mode,
+                                  waves_per_block))
+    return e;
   ADM_HIP(hipDeviceSynchronize());
-  ens.members[0]->use_coloring = use_coloring != 0;
+  ens.members[0]->set_launch_mode(mode);
   ens.members[0]->cell_waves = waves_per_block;
   ADM_API_END
 }
 
 int adm_ensemble_autotune(void *ptr, const double *src, double *dst,
-                          int32_t reps, void *stream,
-                          int32_t *use_coloring_out, int32_t *waves_out,
-                          double *ms_out) {
+                          int32_t reps, void *stream, int32_t *mode_out,
+                          int32_t *waves_out, double *ms_out) {
   ADM_API_BEGIN
   ADM_ENS(ptr)
   if (!src || !dst || reps < 1)
@@ -1804,10 +2062,10 @@ int adm_ensemble_autotune(void *ptr, const double *src, double *dst,
   tune_launch(
       c0, holders,
       [&]() {
-        c0.fill(dst, 0.0, ens.n_dofs * ens.n_members, s);
+        if (!c0.plane_path()) c0.fill(dst, 0.0, ens.n_dofs * ens.n_members, s);
         ens.apply(src, dst, 0, s);
       },
-      reps, s, use_coloring_out, waves_out, ms_out);
+      reps, s, mode_out, waves_out, ms_out);
   ADM_API_END
 }
 
@@ -1874,8 +2132,32 @@ int adm_add_material(void *ptr, const int32_t *box_of_cell,
   ADM_CTX(ptr)
   if (!box_of_cell || !box_cos || !box_sin)
     return adm::set_error(ADM_ERR_INVALID, "null argument");
+  if (ctx.n_ext >= 0)
+    return adm::set_error(ADM_ERR_UNSUPPORTED,
+                          "activation changes the DoFs: clear the DoF map, "
+                          "activate on the box-node vector, set the new map");
   ctx.add_material(box_of_cell, box_cos, box_sin, new_temperature, T,
                    static_cast<hipStream_t>(stream));
+  ADM_API_END
+}
+
+int adm_dofs_to_nodes(void *ptr, const double *ext, double *node,
+                      void *stream) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  if (!ext || !node) return adm::set_error(ADM_ERR_INVALID, "null vector");
+  if (ctx.n_ext < 0) return adm::set_error(ADM_ERR_INVALID, "no DoF map set");
+  ctx.to_nodes(ext, node, static_cast<hipStream_t>(stream));
+  ADM_API_END
+}
+
+int adm_nodes_to_dofs(void *ptr, const double *node, double *ext, int add,
+                      void *stream) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  if (!ext || !node) return adm::set_error(ADM_ERR_INVALID, "null vector");
+  if (ctx.n_ext < 0) return adm::set_error(ADM_ERR_INVALID, "no DoF map set");
+  ctx.to_dofs(node, ext, add, static_cast<hipStream_t>(stream));
   ADM_API_END
 }
 

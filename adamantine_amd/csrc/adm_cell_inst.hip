@@ -2,6 +2,9 @@
 // -DADM_NS and -DADM_PK at compile time.
 #include "adm_kernels.hpp"
 #include "adm_launch.hpp"
+#if ADM_P <= 2
+#include "adm_plane_kernel.hpp"
+#endif
 
 namespace adm {
 
@@ -15,6 +18,22 @@ void launch_cell(const CellArgs &a, const double *src, double *dst,
   hipLaunchKernelGGL((cell_kernel<P, NS, PK>), dim3(grid, n_members),
                      dim3(64 * waves), lds_bytes + xbytes, s, a, src, dst);
 }
+
+#if ADM_P <= 2
+template <int P, int NS, int PK>
+void launch_plane(const CellArgs &a, const double *src, double *dst,
+                  int64_t n_blocks, size_t lds_bytes, unsigned n_members,
+                  int waves, hipStream_t s) {
+  const size_t xbytes = sizeof(double) * waves * PlaneLanes<P>::wave_lds;
+  hipLaunchKernelGGL((plane_kernel<P, NS, PK>), dim3((unsigned)n_blocks,
+                     n_members), dim3(64 * waves), lds_bytes + xbytes, s, a,
+                     src, dst);
+}
+template void launch_plane<ADM_P, ADM_NS, ADM_PK>(const CellArgs &,
+                                                  const double *, double *,
+                                                  int64_t, size_t, unsigned,
+                                                  int, hipStream_t);
+#endif
 
 template <int P, int NS, int PK>
 void launch_face(const FaceArgs &a, const double *src, double *dst,

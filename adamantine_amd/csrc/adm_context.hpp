@@ -79,6 +79,12 @@ struct Context {
               double *dst, unsigned n_members, hipStream_t s,
               int nz_rows = -1) const;
   void fill(double *x, double v, int64_t n, hipStream_t s);
+  // the plane kernel's patch buffer, grown to n doubles if smaller
+  double *ensure_patch_buffer(size_t n) const;
+  // doubles of the patch buffer for launches over nz_rows cell layers
+  size_t patch_buffer_size(int nz_rows, unsigned n_members) const;
+  // the cell term goes through the plane kernel (which writes every node)
+  bool plane_path() const { return use_plane && p <= 2; }
   void compute_inverse_mass(double *out, hipStream_t s);
   void ensure_step_buffers(int method);
   void step(int method, double t, double dt, double *T, hipStream_t s);
@@ -109,9 +115,22 @@ struct Context {
   std::vector<int> old_segment;
   std::vector<adm_beam_state> beam_states;
   bool sources_on = false;
-  // cell kernel in 8 parity colors without atomics (default), or one
-  // atomic launch (ADM_CELL_ATOMICS=1 in the environment at creation)
+  // line kernel: 8 parity colors without atomics, or one atomic launch
+  // (ADM_CELL_ATOMICS=1 in the environment at creation)
   bool use_coloring = true;
+  // plane kernel (FE degrees 1 and 2; the default there unless
+  // ADM_CELL_KERNEL=line is in the environment at creation)
+  bool use_plane = false;
+  // ADM_LAUNCH_* mode of the fields above
+  int launch_mode() const {
+    return use_plane ? ADM_LAUNCH_PLANE
+                     : (use_coloring ? ADM_LAUNCH_LINE_COLORED
+                                     : ADM_LAUNCH_LINE_ATOMIC);
+  }
+  void set_launch_mode(int mode) {
+    use_plane = mode == ADM_LAUNCH_PLANE;
+    use_coloring = mode == ADM_LAUNCH_LINE_COLORED;
+  }
   // row segments in XCD-contiguous block order (default; ADM_XCD_ORDER=0 in
   // the environment at creation restores plain block order)
   bool xcd_order = true;
@@ -134,6 +153,19 @@ struct Context {
   adm_beam_state *d_beam_table = nullptr;
   size_t beam_table_rows = 0;
   int32_t *d_slot = nullptr;
+  // plane kernel patch buffer (members x patches x nodes per patch)
+  mutable double *d_patch = nullptr;
+  mutable size_t patch_cap = 0;
+  // DoF map of the caller's vectors (adm_set_dof_map): external DoF i is
+  // box node dof_map[i]; n_ext < 0 without a map. map_src / map_dst are
+  // box-node vectors the mapped entry points work on.
+  int64_t n_ext = -1;
+  int64_t *d_dof_map = nullptr;
+  double *d_map_src = nullptr, *d_map_dst = nullptr;
+  void set_dof_map(int64_t n, const int64_t *map);
+  void clear_dof_map();
+  void to_nodes(const double *ext, double *node, hipStream_t s) const;
+  void to_dofs(const double *node, double *ext, int add, hipStream_t s) const;
 };
 
 // Members of an ensemble: one Context each (own state, tables, face lists),

@@ -45,6 +45,16 @@ struct MergedTable {
   double B[kMergedMax];
   double coef[ADM_N_STATE_PROPS][kMergedMax][ADM_MAX_STATES][2];
 };
+// Uniform bins over [B_0, B_last] that turn the interval search into one
+// lookup: bin b = clamp(floor((T - lo) * inv_w)) holds k0[b], the number of
+// breakpoints below the bin (widened by a margin far above the rounding of
+// the bin computation), and at most maxc breakpoints fall in a widened bin,
+// so j = #{B_k <= T} = k0[b] + #{c < maxc : B[k0[b] + c] <= T} exactly.
+constexpr int kMergedBins = 128;
+struct MergedBins {
+  double lo, inv_w, maxc, pad_;
+  uint8_t k0[kMergedBins];
+};
 
 struct MatConst {
   int32_t n_materials, n_states, use_table, p_order, single_layout, pad_;
@@ -63,6 +73,7 @@ struct MatConst {
   // breakpoints fit kMergedMax - 1
   uint8_t merged_ok;
   MergedTable merged[ADM_MAX_MATERIALS];
+  MergedBins bins[ADM_MAX_MATERIALS];
 };
 
 struct KConst {
@@ -149,6 +160,13 @@ struct CellArgs {
   int32_t beam_table_len;
   // nonzero: give each XCD a contiguous run of row segments (xcd_block)
   int32_t xcd_remap;
+  // plane kernel: patches per x row and per y column of a cell layer (a
+  // launch has n_rows = plane_npx * plane_npy * lext[2] patches), and the
+  // patch buffer the kernel writes its nodes' sums to (plane_slots doubles
+  // per patch, plane_E_stride per ensemble member) for the gather kernel
+  int32_t plane_npx, plane_npy;
+  double *plane_E;
+  int64_t plane_E_stride;
 };
 
 struct FaceArgs {

@@ -21,6 +21,25 @@ __global__ void fill_kernel(double *__restrict__ x, double v, int64_t n) {
   if (i < n) x[i] = v;
 }
 
+// DoF map (adm_set_dof_map): an external vector in the caller's DoF order
+// into the box-node vector (node[map[i]] = ext[i]), and back
+// (ext[i] = node[map[i]], or += with add).
+__global__ void dofs_to_nodes_kernel(const double *__restrict__ ext,
+                                     const int64_t *__restrict__ map,
+                                     int64_t n_ext, double *__restrict__ node) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_ext) node[map[i]] = ext[i];
+}
+__global__ void nodes_to_dofs_kernel(const double *__restrict__ node,
+                                     const int64_t *__restrict__ map,
+                                     int64_t n_ext, double *__restrict__ ext,
+                                     int add) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_ext) return;
+  const double v = node[map[i]];
+  ext[i] = add ? ext[i] + v : v;
+}
+
 // T[idx[k]] = v (values of newly activated nodes)
 __global__ void scatter_value_kernel(double *__restrict__ T,
                                      const int64_t *__restrict__ idx, double v,

@@ -23,7 +23,8 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import numpy as np
 
 from .problem import (MAX_BEAMS, STATE_PROPERTY_NAMES, Problem, ProblemDesc)
-from .thermal import (_dev_ptr, _stream_ptr, _vp, check, load_library)
+from .thermal import (LAUNCH_CONFIGS, LAUNCH_LINE_COLORED, N_LAUNCH_CONFIGS,
+                      _dev_ptr, _stream_ptr, _vp, check, load_library)
 
 _dp = C.POINTER(C.c_double)
 
@@ -245,22 +246,25 @@ class EnsembleDevice:
         check(_lib().adm_ensemble_vmult(self._h, _dev_ptr(src), _dev_ptr(dst),
                                         1, _stream_ptr(stream)))
 
-    def set_launch_config(self, use_coloring: bool, waves_per_block: int):
+    def set_launch_config(self, mode: int, waves_per_block: int):
+        """mode: LAUNCH_LINE_ATOMIC (0), LAUNCH_LINE_COLORED (1) or
+        LAUNCH_PLANE (2, FE degrees 1 and 2); 2..4 wavefronts per block."""
         check(_lib().adm_ensemble_set_launch_config(
-            self._h, int(bool(use_coloring)), int(waves_per_block)))
+            self._h, int(mode), int(waves_per_block)))
 
     def autotune(self, src, reps: int = 5, stream=None) -> dict:
         """ThermalOperatorDevice.autotune for the batched launches."""
         self._check_vec(src)
         dst = self.initialize_dof_vector(0.0)
         col, w = C.c_int32(), C.c_int32()
-        ms = np.zeros(6)
+        ms = np.zeros(N_LAUNCH_CONFIGS)
         check(_lib().adm_ensemble_autotune(
             self._h, _dev_ptr(src), _dev_ptr(dst), int(reps),
             _stream_ptr(stream), C.byref(col), C.byref(w),
             ms.ctypes.data_as(_dp)))
-        configs = [(c, k) for c in (True, False) for k in (2, 3, 4)]
-        return {"use_coloring": bool(col.value),
+        configs = LAUNCH_CONFIGS
+        return {"mode": int(col.value),
+                "use_coloring": col.value == LAUNCH_LINE_COLORED,
                 "waves_per_block": int(w.value),
                 "ms": dict(zip(configs, ms.tolist()))}
 

@@ -370,5 +370,6 @@ def set_with_experimental_data(operator, points_values: PointsValues,
         raise ValueError("mapping and values differ in length")
     check(_lib().adm_set_values(operator.handle, dofs.size,
                                 dofs.ctypes.data_as(_ip64),
-                                vals.ctypes.data_as(_dp), _dev_ptr(temperature),
+                                vals.ctypes.data_as(_dp),
+                                _dev_ptr(temperature, operator.m()),
                                 _stream_ptr(stream)))

@@ -21,6 +21,15 @@ import numpy as np
 
 from .problem import BeamState, Problem, ProblemDesc
 
+# cell-kernel launch modes of adm_set_launch_config (include/adm.h)
+LAUNCH_LINE_ATOMIC = 0
+LAUNCH_LINE_COLORED = 1
+LAUNCH_PLANE = 2
+# the autotuner's configurations, in the order of its timings
+LAUNCH_CONFIGS = [(m, w) for m in (LAUNCH_LINE_COLORED, LAUNCH_LINE_ATOMIC,
+                                   LAUNCH_PLANE) for w in (2, 3, 4)]
+N_LAUNCH_CONFIGS = len(LAUNCH_CONFIGS)
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libadm.so")
 _lib = None
@@ -61,6 +70,12 @@ def load_library():
     L.adm_get_cell_state.argtypes = [_vp, _dp]
     L.adm_get_qp_state.argtypes = [_vp, _dp, _dp]
     L.adm_set_qp_state.argtypes = [_vp, _dp, _dp]
+    L.adm_n_nodes.argtypes = [_vp, C.POINTER(C.c_int64)]
+    L.adm_set_dof_map.argtypes = [_vp, C.c_int64, C.POINTER(C.c_int64)]
+    L.adm_dofs_to_nodes.argtypes = [_vp, _vp, _vp, _vp]
+    L.adm_nodes_to_dofs.argtypes = [_vp, _vp, _vp, C.c_int, _vp]
+    L.adm_get_face_state.argtypes = [_vp, _dp]
+    L.adm_set_face_state.argtypes = [_vp, _dp]
     L.adm_set_deposition.argtypes = [_vp, _dp, _dp]
     L.adm_set_launch_config.argtypes = [_vp, C.c_int32, C.c_int32]
     L.adm_autotune.argtypes = [_vp, _vp, _vp, C.c_int32, _vp,
@@ -93,8 +108,14 @@ def _stream_ptr(stream=None):
     return C.c_void_p(s.cuda_stream)
 
 
-def _dev_ptr(t):
+def _dev_ptr(t, n: Optional[int] = None):
+    """Device address of a vector handed to the library. With n, the vector
+    must hold exactly n values: the kernels index it up to n - 1, so a
+    shorter one is refused here, before any launch."""
     import torch
+    if n is not None and (not isinstance(t, torch.Tensor) or t.numel() != n):
+        raise AdmError("device vector of %s values where %d are needed" %
+                       (t.numel() if isinstance(t, torch.Tensor) else "?", n))
     if not (isinstance(t, torch.Tensor) and t.is_cuda and
             t.dtype == torch.float64 and t.is_contiguous()):
         raise AdmError("device vectors must be contiguous float64 HIP tensors")
@@ -141,17 +162,50 @@ class ThermalOperatorDevice:
     def n(self) -> int:
         return self._n_dofs
 
+    def n_nodes(self) -> int:
+        """Nodes of the structured box numbering."""
+        n = C.c_int64()
+        check(load_library().adm_n_nodes(self._h, C.byref(n)))
+        return n.value
+
+    def set_dof_map(self, dof_to_node=None):
+        """Vectors in the caller's DoF order (DoF i = box node
+        dof_to_node[i]) for vmult, the inverse mass and evolve; None goes
+        back to the box-node numbering. m() and n() follow."""
+        if dof_to_node is None:
+            check(load_library().adm_set_dof_map(self._h, -1, None))
+        else:
+            mp = np.ascontiguousarray(dof_to_node, dtype=np.int64)
+            check(load_library().adm_set_dof_map(
+                self._h, mp.size, mp.ctypes.data_as(C.POINTER(C.c_int64))))
+        n = C.c_int64()
+        check(load_library().adm_n_dofs(self._h, C.byref(n)))
+        self._n_dofs = n.value
+        self._inverse_mass = None
+
+    def dofs_to_nodes(self, ext, node, stream=None):
+        check(load_library().adm_dofs_to_nodes(
+            self._h, _dev_ptr(ext, self._n_dofs),
+            _dev_ptr(node, self.n_nodes()), _stream_ptr(stream)))
+
+    def nodes_to_dofs(self, node, ext, add: bool = False, stream=None):
+        check(load_library().adm_nodes_to_dofs(
+            self._h, _dev_ptr(node, self.n_nodes()),
+            _dev_ptr(ext, self._n_dofs), int(add), _stream_ptr(stream)))
+
     def initialize_dof_vector(self, value: float = 0.0, device=None):
         import torch
         return torch.full((self._n_dofs,), float(value), dtype=torch.float64,
                           device=device or "cuda")
 
     def vmult(self, dst, src, stream=None):
-        check(load_library().adm_vmult(self._h, _dev_ptr(src), _dev_ptr(dst), 0,
+        check(load_library().adm_vmult(self._h, _dev_ptr(src, self._n_dofs),
+                                       _dev_ptr(dst, self._n_dofs), 0,
                                        _stream_ptr(stream)))
 
     def vmult_add(self, dst, src, stream=None):
-        check(load_library().adm_vmult(self._h, _dev_ptr(src), _dev_ptr(dst), 1,
+        check(load_library().adm_vmult(self._h, _dev_ptr(src, self._n_dofs),
+                                       _dev_ptr(dst, self._n_dofs), 1,
                                        _stream_ptr(stream)))
 
     def compute_inverse_mass_matrix(self, stream=None):
@@ -185,22 +239,26 @@ class ThermalOperatorDevice:
                                                 s.ctypes.data_as(_dp)))
 
     # quadrature-point state --------------------------------------------------
-    def set_launch_config(self, use_coloring: bool, waves_per_block: int):
+    def set_launch_config(self, mode: int, waves_per_block: int):
+        """mode: LAUNCH_LINE_ATOMIC (0), LAUNCH_LINE_COLORED (1) or
+        LAUNCH_PLANE (2, FE degrees 1 and 2); 2..4 wavefronts per block."""
         check(load_library().adm_set_launch_config(
-            self._h, int(bool(use_coloring)), int(waves_per_block)))
+            self._h, int(mode), int(waves_per_block)))
 
     def autotune(self, src, reps: int = 20, stream=None) -> dict:
-        """Time vmult(src) under the 6 launch configurations, keep the
+        """Time vmult(src) under every launch configuration, keep the
         fastest; the phase state is left as it was."""
         dst = self.initialize_dof_vector()
         col, w = C.c_int32(), C.c_int32()
-        ms = np.zeros(6)
+        ms = np.zeros(N_LAUNCH_CONFIGS)
         check(load_library().adm_autotune(
-            self._h, _dev_ptr(src), _dev_ptr(dst), int(reps),
+            self._h, _dev_ptr(src, self._n_dofs),
+            _dev_ptr(dst, self._n_dofs), int(reps),
             _stream_ptr(stream), C.byref(col), C.byref(w),
             ms.ctypes.data_as(_dp)))
-        configs = [(c, k) for c in (True, False) for k in (2, 3, 4)]
-        return {"use_coloring": bool(col.value),
+        configs = LAUNCH_CONFIGS
+        return {"mode": int(col.value),
+                "use_coloring": col.value == LAUNCH_LINE_COLORED,
                 "waves_per_block": int(w.value),
                 "ms": dict(zip(configs, ms.tolist()))}
 
@@ -227,6 +285,22 @@ class ThermalOperatorDevice:
         check(load_library().adm_set_qp_state(
             self._h, None if lq is None else lq.ctypes.data_as(_dp),
             None if pw is None else pw.ctypes.data_as(_dp)))
+
+    def face_state(self):
+        """Powder ratio at the points of the boundary faces with a
+        boundary term, [n_cells][6][n_face_q] (zero elsewhere)."""
+        fp = np.zeros((self.n_cells, 6, self.n_face_q))
+        check(load_library().adm_get_face_state(self._h,
+                                                fp.ctypes.data_as(_dp)))
+        return fp
+
+    def set_face_state(self, face_powder):
+        fp = _host(face_powder)
+        if fp.size != self.n_cells * 6 * self.n_face_q:
+            raise AdmError("face state of %d values where %d are needed" %
+                           (fp.size, self.n_cells * 6 * self.n_face_q))
+        check(load_library().adm_set_face_state(self._h,
+                                                fp.ctypes.data_as(_dp)))
 
     def beam_state_at(self, beam: int, t: float) -> BeamState:
         s = BeamState()
@@ -267,7 +341,8 @@ class ThermalPhysicsDevice:
         t_end = C.c_double()
         check(load_library().adm_evolve(
             self.operator.handle, int(self.method), float(t), float(delta_t),
-            int(n_steps), _dev_ptr(solution), int(use_graph), C.byref(t_end),
+            int(n_steps), _dev_ptr(solution, self.operator.m()),
+            int(use_graph), C.byref(t_end),
             _stream_ptr(stream)))
         return t_end.value
 
@@ -304,7 +379,8 @@ class ThermalPhysicsDevice:
             self.operator.handle, bc.ctypes.data_as(C.POINTER(C.c_int32)),
             c.ctypes.data_as(_dp), s_.ctypes.data_as(_dp),
             float(new_material_temperature),
-            None if solution is None else _dev_ptr(solution),
+            None if solution is None else _dev_ptr(solution,
+                                                   self.operator.m()),
             _stream_ptr(stream)))
         # compute_inverse_mass_matrix (source/ThermalPhysics.templates.hh:741)
         self.operator._inverse_mass = None

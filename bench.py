@@ -23,12 +23,17 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6
-# HBM bytes of one operator application of this workload (the cell kernel
-# as 8 colour launches or one atomic launch, 4 wavefronts per block, blocks
-# in XCD-contiguous order, + the face kernel), from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE runs scaled by a 256 MiB copy
-# (profiles/hbm_traffic_plain_large.md); keyed by use_coloring
-MEASURED_TRAFFIC_BYTES = {True: 355.6e6, False: 176.0e6}
+# names of the cell-kernel launch modes (adm_set_launch_config)
+LAUNCH_MODE_NAMES = {0: "line_atomic", 1: "line_colored", 2: "plane"}
+# HBM bytes of one operator application of this workload per launch mode
+# (cell kernel + face kernel), from rocprofv3 FETCH_SIZE and WRITE_SIZE runs
+# scaled by a 256 MiB copy; the profile named beside each value holds the
+# command and the counter values. None: not profiled.
+MEASURED_TRAFFIC_BYTES = {
+    "line_colored": 355.6e6,   # profiles/hbm_traffic_plain_large.md
+    "line_atomic": 176.0e6,    # profiles/hbm_traffic_plain_large.md
+    "plane": 116.8e6,          # profiles/hbm_traffic_plane_plain_large.md
+}
 
 
 def algorithmic_bytes(problem, n_powder_qp: int, n_active_cells: int) -> int:
@@ -144,7 +149,7 @@ def ensemble_bench(n_members: int, n_steps: int):
     default_ms = time_loop(3 * dt)
     loop_launch = phs[0].operator.autotune(sols[0], reps=5)
     for ph in phs[1:]:
-        ph.operator.set_launch_config(loop_launch["use_coloring"],
+        ph.operator.set_launch_config(loop_launch["mode"],
                                       loop_launch["waves_per_block"])
     loop_ms = time_loop((3 + k_loop) * dt)
     batched = n_members / (batched_ms * 1e-3)
@@ -158,10 +163,11 @@ def ensemble_bench(n_members: int, n_steps: int):
                                                              1e-3),
             "speedup_vs_default_launch_loop": batched * default_ms * 1e-3,
             "loop_sample": "%d members x %d steps" % (n_loop, k_loop),
-            "launch": {k: ens_launch[k] for k in ("use_coloring",
-                                                  "waves_per_block")},
-            "loop_launch": {k: loop_launch[k] for k in ("use_coloring",
-                                                        "waves_per_block")}}
+            "launch": {"mode": LAUNCH_MODE_NAMES[ens_launch["mode"]],
+                       "waves_per_block": ens_launch["waves_per_block"]},
+            "loop_launch": {"mode": LAUNCH_MODE_NAMES[loop_launch["mode"]],
+                            "waves_per_block":
+                                loop_launch["waves_per_block"]}}
 
 
 def adjoint_bench():
@@ -259,14 +265,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--launch", default=None,
-                    help="fixed launch configuration COLORING,WAVES of the "
-                         "operator (e.g. 0,4) instead of the autotuner's")
+                    help="fixed launch configuration MODE,WAVES of the "
+                         "operator (mode 0 line kernel with atomics, 1 line "
+                         "kernel in colour launches, 2 plane kernel; e.g. "
+                         "2,4) instead of the autotuner's")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--only", choices=["all", "vmult", "ensemble", "copy",
                                        "adjoint", "deposition"],
                     default="all",
                     help="profiling aid: run one part and print its timing")
     ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--assign", action="store_true",
+                    help="time vmult (dst cleared, then dst = A src) instead "
+                         "of vmult_add (dst += A src)")
     ap.add_argument("--field", choices=["meltpool", "random"],
                     default="meltpool",
                     help="operator input: a melt pool under the beam (solid "
@@ -309,26 +320,27 @@ def main():
     # launches or atomics, wavefronts per block); phase state left as is
     if args.launch:
         col, waves = (int(v) for v in args.launch.split(","))
-        op.set_launch_config(bool(col), waves)
-        tuned = {"use_coloring": bool(col), "waves_per_block": waves}
+        op.set_launch_config(col, waves)
+        tuned = {"mode": col, "waves_per_block": waves}
     else:
         tuned = op.autotune(T, reps=10)
 
+    apply = op.vmult if args.assign else op.vmult_add
     for _ in range(args.warmup):
-        op.vmult_add(dst, T)
+        apply(dst, T)
     torch.cuda.synchronize()
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     start.record(stream)
     for _ in range(args.steps):
-        op.vmult_add(dst, T)
+        apply(dst, T)
     end.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = start.elapsed_time(end) / args.steps
 
-    launch = {"use_coloring": tuned["use_coloring"],
+    launch = {"mode": LAUNCH_MODE_NAMES[tuned["mode"]],
               "waves_per_block": tuned["waves_per_block"]}
     if args.only == "vmult":
         print(json.dumps({"vmult_ms": ms, "steps": args.steps,
@@ -345,8 +357,7 @@ def main():
     # explicit RK4 steps of the same workload (context of the metric)
     ph = ThermalPhysicsDevice(problem)
     ph.setup()
-    ph.operator.set_launch_config(launch["use_coloring"],
-                                  launch["waves_per_block"])
+    ph.operator.set_launch_config(tuned["mode"], tuned["waves_per_block"])
     sol = ph.initialize_dof_vector(300.0)
     ph.evolve(0.0, 5e-7, 5, sol)
     torch.cuda.synchronize()
@@ -382,8 +393,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved_gbs,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS,
-                     "traffic": MEASURED_TRAFFIC_BYTES[
-                         launch["use_coloring"]],
+                     "traffic": MEASURED_TRAFFIC_BYTES.get(launch["mode"]),
                      "bytes_per_launch": bytes_per_vmult},
         "rk4_step_ms": rk4_ms,
         "wall_s": wall,

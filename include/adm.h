@@ -217,9 +217,40 @@ int adm_destroy(void *ctx);
 /* Message of the calling thread's last error (ASSERT_THROW text). */
 const char *adm_last_error(void);
 
-/* m() / n() (source/ThermalOperatorBase.hh:26-32): number of nodes of the
- * structured numbering (nodes of inactive cells included; they are inert). */
+/* m() / n() (source/ThermalOperatorBase.hh:26-32): the number of DoFs of
+ * the caller's numbering when a DoF map is set (adm_set_dof_map), else the
+ * number of nodes of the structured numbering (nodes of inactive cells
+ * included; they are inert). */
 int adm_n_dofs(const void *ctx, int64_t *n_dofs);
+
+/* Number of nodes of the structured box numbering, (p n_x + 1)(p n_y + 1)
+ * (p n_z + 1), whatever the DoF map. */
+int adm_n_nodes(const void *ctx, int64_t *n_nodes);
+
+/* DoF map of the caller's vectors, for a drop-in behind deal.II's
+ * numbering (ThermalOperatorBase::vmult takes LA::distributed::Vector in
+ * the DoFHandler's order, source/ThermalOperatorBase.hh:26-55; DoFs are
+ * distributed by ThermalPhysics::setup_dofs,
+ * source/ThermalPhysics.templates.hh:508-524, with FE_Nothing on inactive
+ * cells, so n_dofs counts only the nodes of active cells). dof_to_node
+ * (host, n_dofs entries): DoF i is box node dof_to_node[i] (x fastest).
+ * The map must be injective and cover every node of every active cell;
+ * n_dofs < 0 clears it. While a map is set, adm_vmult,
+ * adm_compute_inverse_mass and adm_evolve take and return vectors of
+ * n_dofs entries in the caller's order (one gather into box nodes before,
+ * one scatter after, per call); adm_add_material refuses (activation
+ * renumbers the DoFs: clear the map, activate on the box-node vector, set
+ * the new map); every other entry point keeps box-node indexing. */
+int adm_set_dof_map(void *ctx, int64_t n_dofs, const int64_t *dof_to_node);
+
+/* With a DoF map set: box-node vector (adm_n_nodes entries) from a vector
+ * in the caller's order (entries of unmapped nodes left as they are), and
+ * back (ext = node values of the mapped nodes, or += with add). Device
+ * vectors, stream-ordered. */
+int adm_dofs_to_nodes(void *ctx, const double *ext, double *node,
+                      void *stream);
+int adm_nodes_to_dofs(void *ctx, const double *node, double *ext, int add,
+                      void *stream);
 
 /* Number of cells and of quadrature points per cell. */
 int adm_sizes(const void *ctx, int64_t *n_cells, int32_t *n_q_per_cell,
@@ -249,22 +280,31 @@ int adm_set_time(void *ctx, double t);
 int adm_vmult(void *ctx, const double *src, double *dst, int add,
               void *stream);
 
-/* Launch configuration of the cell kernel (no reference counterpart; the
- * reference's matrix-free loop has no such choice): 8 parity-colour
- * launches with plain read-add-write (use_coloring = 1, the default) or one
- * launch with FP64 atomics (0), and 2..4 wavefronts (row segments) per
- * block (default 4). */
-int adm_set_launch_config(void *ctx, int32_t use_coloring,
-                          int32_t waves_per_block);
+/* Cell-kernel launch modes (no reference counterpart; the reference's
+ * matrix-free loop has no such choice). */
+#define ADM_LAUNCH_LINE_ATOMIC 0  /* line kernel, one launch, FP64 atomics */
+#define ADM_LAUNCH_LINE_COLORED 1 /* line kernel, 8 parity-colour launches
+                                     with plain read-add-write */
+#define ADM_LAUNCH_PLANE 2        /* plane kernel (FE degree 1 or 2 only):
+                                     one launch, shared nodes summed in the
+                                     wavefront, then atomics */
+#define ADM_N_LAUNCH_CONFIGS 9    /* 3 modes x 2..4 wavefronts per block */
+
+/* Launch configuration of the cell kernel: one of the ADM_LAUNCH_* modes and
+ * 2..4 wavefronts per block (default: the plane kernel for FE degrees 1 and
+ * 2, the colour launches of the line kernel otherwise, 4 wavefronts). The
+ * plane mode is refused for FE degrees above 2. */
+int adm_set_launch_config(void *ctx, int32_t mode, int32_t waves_per_block);
 
 /* Autotuner (no reference counterpart): times `reps` vmults of src into dst
- * (device vectors) under each of the 6 launch configurations (colouring
- * 1 then 0, each with 2, 3, 4 wavefronts per block; ms_out, host, may be
- * NULL, receives the 6 per-vmult times in that order), keeps the fastest
- * and returns it. The phase state is restored afterwards, so later results
- * do not depend on the tuning. */
+ * (device vectors) under each launch configuration (modes 1, 0, 2, each
+ * with 2, 3, 4 wavefronts per block; ms_out, host, may be NULL, receives
+ * the ADM_N_LAUNCH_CONFIGS per-vmult times in that order, -1 for the plane
+ * mode where the FE degree has none), keeps the fastest and returns it.
+ * The phase state is restored afterwards, so later results do not depend
+ * on the tuning. */
 int adm_autotune(void *ctx, const double *src, double *dst, int32_t reps,
-                 void *stream, int32_t *use_coloring_out, int32_t *waves_out,
+                 void *stream, int32_t *mode_out, int32_t *waves_out,
                  double *ms_out);
 
 /* Repeated evolve_one_time_step (source/ThermalPhysics.templates.hh:854-888):
@@ -301,6 +341,12 @@ int adm_set_qp_state(void *ctx, const double *liquid, const double *powder);
  * (_face_powder_ratio of source/ThermalOperator.hh:225), host out
  * [n_cells][6][n_face_q]; faces without a boundary term read as zero. */
 int adm_get_face_state(void *ctx, double *face_powder);
+/* Sets the face powder ratios from the same layout (the entries of faces
+ * without a boundary term are ignored): the restart counterpart of
+ * adm_get_face_state for checkpoints that keep the face history
+ * (the reference re-derives it from the cell state,
+ * source/ThermalPhysics.templates.hh:1084-1185). */
+int adm_set_face_state(void *ctx, const double *face_powder);
 
 /* set_material_deposition_orientation (source/ThermalOperator.templates.hh:
  * 897-940): per-cell cos/sin (host, n_cells each). */
@@ -401,12 +447,11 @@ int adm_ensemble_vmult(void *ens, const double *src, double *dst, int add,
 
 /* adm_set_launch_config and adm_autotune for the batched launches of an
  * ensemble (no reference counterpart); src, dst: n_members * n_dofs. */
-int adm_ensemble_set_launch_config(void *ens, int32_t use_coloring,
+int adm_ensemble_set_launch_config(void *ens, int32_t mode,
                                    int32_t waves_per_block);
 int adm_ensemble_autotune(void *ens, const double *src, double *dst,
-                          int32_t reps, void *stream,
-                          int32_t *use_coloring_out, int32_t *waves_out,
-                          double *ms_out);
+                          int32_t reps, void *stream, int32_t *mode_out,
+                          int32_t *waves_out, double *ms_out);
 
 /* evolve_one_time_step of every member, n_steps times, batched. This
  * replaces the serial member loop of run_ensemble

@@ -97,3 +97,18 @@ def test_cxx_adapter_compiles_against_the_header_alone(tmp_path):
                         "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
                         str(src)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_wrong_length_vector_is_refused_before_any_launch():
+    """A device vector whose length is not the one the kernels index is
+    refused by the wrapper, before the library is called."""
+    import torch
+    from adamantine_amd.thermal import AdmError, _dev_ptr
+    with pytest.raises(AdmError, match="5 values where 6 are needed"):
+        _dev_ptr(torch.zeros(5, dtype=torch.float64), 6)
+    with pytest.raises(AdmError, match="where 6 are needed"):
+        _dev_ptr(None, 6)
+    # without a length only the kind of tensor is checked (a CPU tensor is
+    # not a device vector)
+    with pytest.raises(AdmError, match="HIP tensors"):
+        _dev_ptr(torch.zeros(6, dtype=torch.float64), 6)

@@ -355,6 +355,39 @@ def test_calibration_autograd():
     assert abs(fd) > 0 and rel(float(th.grad[1]), fd) < 1e-5
 
 
+def test_calibration_history_independent():
+    """With powder that melts during the run, two misfit calls at the same
+    theta give the same J and gradients, and a call after a run at another
+    theta still does: each run starts from the phase state the operator was
+    built with."""
+    import torch
+    from adamantine_amd.adjoint import Calibration
+    print("calibration history: 10x6x5 Q2 SLP with powder, 7 RK4 steps",
+          flush=True)
+    p = small_problem("q2_slp", powder=True, method=RK_FOURTH_ORDER)
+    t0, dt, n = 0.0, 1e-6, 7
+    T0 = torch.tensor(hot_field(p, 1600.0, 1760.0), device="cuda")
+    X = p.node_coordinates()
+    dof = np.nonzero(np.isclose(X[:, 2], p.extent[2]))[0][::7]
+    obs = np.random.RandomState(5).uniform(1550.0, 1650.0, dof.size)
+    b = p.beams[0]
+    cal = Calibration(p, t0, dt, n, dof, obs, checkpoint_every=3)
+    theta = np.array([b.absorption_efficiency, 1.0, b.radius])
+    ph = cal.physics_for(theta)
+    _, pw0 = ph.operator.qp_state()
+    r1 = cal.run(theta, T0)
+    _, pw1 = ph.operator.qp_state()
+    assert (pw1 < pw0).sum() > 0, "the run must melt some powder"
+    r2 = cal.run(theta, T0)
+    cal.run(theta * np.array([1.1, 1.0, 0.9]), T0)
+    r3 = cal.run(theta, T0)
+    torch.cuda.synchronize()
+    for r in (r2, r3):
+        assert rel(r.J, r1.J) < 1e-13
+        for k in range(3):
+            assert rel(r.grad[k], r1.grad[k]) < 1e-10, (k, r.grad, r1.grad)
+
+
 @pytest.mark.parametrize("deg", [4, 5])
 def test_adjoint_high_degree(deg):
     """The adjoint at FE degrees 4 and 5 (RK4, SLP tables, faces): the

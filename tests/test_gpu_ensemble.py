@@ -123,7 +123,9 @@ def test_ensemble_launch_configs_and_autotune():
         o.set_time(t)
         want.append(o.vmult(T[len(want)]))
         wq.append(o.qp_state())
-    for col in (True, False):
+    from adamantine_amd.thermal import (LAUNCH_LINE_ATOMIC,
+                                        LAUNCH_LINE_COLORED, LAUNCH_PLANE)
+    for col in (LAUNCH_LINE_COLORED, LAUNCH_LINE_ATOMIC, LAUNCH_PLANE):
         for w in (2, 3, 4):
             ens = EnsembleDevice(base, members)
             ens.set_launch_config(col, w)

@@ -15,6 +15,8 @@ from adamantine_amd.problem import (BC_ADIABATIC, BC_CONVECTIVE,
 from adamantine_amd.workloads import (plain_large, plain_small,
                                       random_temperature, smooth_temperature)
 from oracle.oracle import Oracle
+from adamantine_amd.thermal import (LAUNCH_LINE_ATOMIC, LAUNCH_LINE_COLORED,
+                                    LAUNCH_PLANE)
 
 pytestmark = pytest.mark.gpu
 
@@ -405,10 +407,11 @@ def test_degree_four_and_five(p, nc):
 
 
 def test_launch_configs_and_autotune():
-    """Every launch configuration of the cell kernel (8 colour launches or
-    one atomic launch; 2, 3 or 4 wavefronts per block) meets the vmult and
-    phase-state bars against the oracle; the autotuner picks one of them and
-    leaves the phase state as it found it."""
+    """Every launch configuration of the cell kernel (the line kernel as 8
+    colour launches or one atomic launch, the plane kernel; 2, 3 or 4
+    wavefronts per block) meets the vmult and phase-state bars against the
+    oracle; the autotuner picks one of them and leaves the phase state as it
+    found it."""
     torch = dev()
     from adamantine_amd.thermal import ThermalOperatorDevice
     p = plain_large(n_cells=(24, 12, 8))
@@ -418,7 +421,7 @@ def test_launch_configs_and_autotune():
     o.set_time(t)
     want = o.vmult(T)
     olq, opw = o.qp_state()
-    for col in (True, False):
+    for col in (LAUNCH_LINE_COLORED, LAUNCH_LINE_ATOMIC, LAUNCH_PLANE):
         for w in (2, 3, 4):
             op = ThermalOperatorDevice(p)
             op.set_launch_config(col, w)
@@ -454,6 +457,9 @@ def test_evolve_graph_replays():
     b = ThermalPhysicsDevice(p, FORWARD_EULER)
     a.setup()
     b.setup()
+    # fixed accumulation order for the bit-for-bit comparison
+    a.operator.set_launch_config(LAUNCH_LINE_COLORED, 4)
+    b.operator.set_launch_config(LAUNCH_LINE_COLORED, 4)
     sa = torch.full((p.n_dofs,), 300.0, dtype=torch.float64, device="cuda")
     sb = sa.clone()
     ta = a.evolve(0.0, 1e-6, 60, sa)
