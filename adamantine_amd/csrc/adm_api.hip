@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -374,6 +375,31 @@ int Context::property_kind() const {
   return kc.mat.merged_ok ? 2 : 1;
 }
 
+// Scratch of adm_vmult_noalloc: the plane kernel's patch buffer in a
+// static device array, so that the call allocates nothing. Each device the
+// library runs on holds its own copy for the life of the process; host code
+// uses the address hipGetSymbolAddress gives on that device, never the
+// array's name.
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+// The array's address on device dev (dev current), resolved once per device.
+static double *scratch_on_device(int dev) {
+  static std::mutex mu;
+  static double *ptr[64] = {};
+  if (dev < 0 || dev >= 64)
+    throw AdmError(ADM_ERR_UNSUPPORTED, "device index above 63");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[dev]) {
+    double *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw AdmError(ADM_ERR_HIP, std::string("scratch address: ") +
+                                      hipGetErrorString(e));
+    ptr[dev] = p;
+  }
+  return ptr[dev];
+}
+
 Context::Context(const adm_problem_desc &d) {
   if (const char *e = std::getenv("ADM_CELL_ATOMICS"))
     use_coloring = !(e[0] == '1');
@@ -590,6 +616,7 @@ Context::Context(const adm_problem_desc &d) {
   std::memset(beam_states.data(), 0, sizeof(adm_beam_state) * ADM_MAX_BEAMS);
 
   ADM_HIP(hipGetDevice(&device));
+  d_scratch = scratch_on_device(device);
   ADM_HIP(hipMalloc(&d_kc, sizeof(KConst)));
   ADM_HIP(hipMalloc(&d_meta, sizeof(uint32_t) * n_cells));
   ADM_HIP(hipMalloc(&d_powder_slot, sizeof(int32_t) * n_cells));
@@ -1039,17 +1066,18 @@ FaceArgs Context::face_args() const {
   return a;
 }
 
-void Context::apply(const double *src, double *dst, int add, hipStream_t s) {
+void Context::apply(const double *src, double *dst, int add, hipStream_t s,
+                    double *patch) {
   CellArgs a = cell_args();
   a.add = add;
-  launch(a, face_args(), src, dst, 1, s, nz_active);
+  launch(a, face_args(), src, dst, 1, s, nz_active, patch);
 }
 
 // Cell and face kernels of the context's configuration; with n_members > 1
 // the arguments carry the member table and the grid's y index is the member.
 void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
                      double *dst, unsigned n_members, hipStream_t s,
-                     int nz_rows) const {
+                     int nz_rows, double *patch) const {
   // cell layers above the highest active one hold no active cell
   const int lim[3] = {nc[0], nc[1],
                       nz_rows >= 0 ? std::min(nz_rows, nc[2]) : nc[2]};
@@ -1072,7 +1100,17 @@ void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
     const int64_t slots = p == 1 ? PlaneLanes<1>::patch_nodes
                                  : PlaneLanes<2>::patch_nodes;
     ac.plane_E_stride = (int64_t)ac.n_rows * slots;
-    ac.plane_E = ensure_patch_buffer((size_t)ac.plane_E_stride * M);
+    if (patch) {
+      if ((size_t)ac.plane_E_stride * M > kNoAllocScratchDoubles)
+        throw AdmError(ADM_ERR_UNSUPPORTED,
+                       "the mesh needs a patch buffer of " +
+                           std::to_string((size_t)ac.plane_E_stride * M) +
+                           " doubles, more than the scratch array's " +
+                           std::to_string(kNoAllocScratchDoubles));
+      ac.plane_E = patch;
+    } else {
+      ac.plane_E = ensure_patch_buffer((size_t)ac.plane_E_stride * M);
+    }
     if (ac.n_rows > 0) {
       // a persistent grid: as many blocks as the chip holds at the
       // kernel's occupancy (shared among the members), each looping over
@@ -1689,6 +1727,25 @@ static int check_launch_config(int p, int64_t n_dofs, int32_t mode,
     return adm::set_error(ADM_ERR_UNSUPPORTED,
                           "the plane kernel takes fewer than 2^31 nodes");
   return ADM_OK;
+}
+
+int adm_vmult_noalloc(void *ptr, const double *src, double *dst, int add,
+                      void *stream) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  if (!src || !dst) return adm::set_error(ADM_ERR_INVALID, "null vector");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // as adm_vmult, with the scratch array as the patch buffer
+  if (ctx.n_ext >= 0) {
+    ctx.to_nodes(src, ctx.d_map_src, s);
+    if (!ctx.plane_path()) ctx.fill(ctx.d_map_dst, 0.0, ctx.n_dofs, s);
+    ctx.apply(ctx.d_map_src, ctx.d_map_dst, 0, s, ctx.d_scratch);
+    ctx.to_dofs(ctx.d_map_dst, dst, add, s);
+  } else {
+    if (!add && !ctx.plane_path()) ctx.fill(dst, 0.0, ctx.n_dofs, s);
+    ctx.apply(src, dst, add, s, ctx.d_scratch);
+  }
+  ADM_API_END
 }
 
 int adm_set_launch_config(void *ptr, int32_t mode, int32_t waves_per_block) {

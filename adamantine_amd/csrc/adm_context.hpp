@@ -72,12 +72,15 @@ struct Context {
   FaceArgs face_args() const;
   void build_merged_tables();
   int property_kind() const;
-  void apply(const double *src, double *dst, int add, hipStream_t s);
+  // patch: the plane kernel's patch buffer for this call (NULL: the
+  // context's own, grown on demand)
+  void apply(const double *src, double *dst, int add, hipStream_t s,
+             double *patch = nullptr);
   // nz_rows: cell layers (from z = 0) that can hold an active cell; the
   // launches cover only those (-1: all)
   void launch(const CellArgs &a, const FaceArgs &fa, const double *src,
               double *dst, unsigned n_members, hipStream_t s,
-              int nz_rows = -1) const;
+              int nz_rows = -1, double *patch = nullptr) const;
   void fill(double *x, double v, int64_t n, hipStream_t s);
   // the plane kernel's patch buffer, grown to n doubles if smaller
   double *ensure_patch_buffer(size_t n) const;
@@ -156,6 +159,9 @@ struct Context {
   // plane kernel patch buffer (members x patches x nodes per patch)
   mutable double *d_patch = nullptr;
   mutable size_t patch_cap = 0;
+  // the library's static scratch array on this context's device
+  // (adm_vmult_noalloc), resolved when the context is created
+  double *d_scratch = nullptr;
   // DoF map of the caller's vectors (adm_set_dof_map): external DoF i is
   // box node dof_map[i]; n_ext < 0 without a map. map_src / map_dst are
   // box-node vectors the mapped entry points work on.

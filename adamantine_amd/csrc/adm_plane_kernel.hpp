@@ -61,6 +61,11 @@ struct PlaneLanes {
   static constexpr int patch_nodes = patch_nx * patch_ny * N;
 };
 
+// Doubles of the library's static scratch array (adm_vmult_noalloc), which
+// holds the plane kernel's patch buffer there: 40 MiB per device, enough
+// for the 128x64x32 Q2 workload's 4.2 million patch-buffer doubles.
+constexpr size_t kNoAllocScratchDoubles = (size_t)5 << 20;
+
 // doubles of one material's merged tables staged in LDS by the plane
 // kernel: B, the coefficients of density .. thermal_conductivity_z, then
 // the MergedBins record

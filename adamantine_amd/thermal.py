@@ -70,6 +70,7 @@ def load_library():
     L.adm_get_cell_state.argtypes = [_vp, _dp]
     L.adm_get_qp_state.argtypes = [_vp, _dp, _dp]
     L.adm_set_qp_state.argtypes = [_vp, _dp, _dp]
+    L.adm_vmult_noalloc.argtypes = [_vp, _vp, _vp, C.c_int, _vp]
     L.adm_n_nodes.argtypes = [_vp, C.POINTER(C.c_int64)]
     L.adm_set_dof_map.argtypes = [_vp, C.c_int64, C.POINTER(C.c_int64)]
     L.adm_dofs_to_nodes.argtypes = [_vp, _vp, _vp, _vp]

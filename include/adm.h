@@ -280,6 +280,22 @@ int adm_set_time(void *ctx, double t);
 int adm_vmult(void *ctx, const double *src, double *dst, int add,
               void *stream);
 
+/* adm_vmult that allocates nothing. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory during the call (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). adm_vmult may grow the plane kernel's patch buffer with
+ * hipMalloc on its first call or after an activation; this entry point
+ * uses instead a static device array of the library, 40 MiB per device
+ * (5 * 2^20 doubles), held until the process ends, and refuses
+ * (ADM_ERR_UNSUPPORTED) a mesh whose patch buffer needs more (the
+ * 128x64x32 Q2 workload needs 4.2 million doubles). Every call on one
+ * device shares that array: calls that overlap on one device (several
+ * streams, several host threads) race; calls in one stream, one after the
+ * other, are safe (the array needs no clearing: every slot a call reads
+ * it writes first). Same arguments and results as adm_vmult. */
+int adm_vmult_noalloc(void *ctx, const double *src, double *dst, int add,
+                      void *stream);
+
 /* Cell-kernel launch modes (no reference counterpart; the reference's
  * matrix-free loop has no such choice). */
 #define ADM_LAUNCH_LINE_ATOMIC 0  /* line kernel, one launch, FP64 atomics */
