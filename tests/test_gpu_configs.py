@@ -1,0 +1,284 @@
+"""The workloads at their full size, and the device paths no other GPU test
+exercises, against the oracle.
+
+- Electron-beam source (source/ElectronBeamHeatSource.cc:95-158): vmult and
+  forward-Euler steps.
+- Tables: the general-table evaluator (tables outside the regular padded
+  form) and a regular table with all 12 entries, whose last interval takes
+  the last value (source/MaterialProperty.templates.hh:1192-1196).
+- The batched EnKF ensemble at size: 64 members of the 64x32x16 Q2 box,
+  RK4 steps of every member checked on sampled members against each one's
+  oracle, then one analysis with 500 observations and 2 augmented
+  parameters. Without localisation the analysis of a state entry depends
+  only on that entry's and the observed entries' anomalies, so the oracle's
+  dense restatement runs on the observed entries, a sample of the others
+  and the parameters, and the device analysis of the full state must agree
+  on them.
+- The deposition workload at size: 32 members of the 96x96x40 Q2 box with
+  the electron beam; from a common state at step 190 sampled members step
+  through their second-layer deposition against their oracles.
+"""
+import numpy as np
+import pytest
+
+from adamantine_amd.problem import (FORWARD_EULER, RK_FOURTH_ORDER,
+                                    SOURCE_ELECTRON_BEAM, Material)
+from adamantine_amd.workloads import plain_large, random_temperature
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+VMULT_TOL = 1e-12
+STEP_TOL = 1e-10
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _vmult(op, T, t):
+    import torch
+    op.set_time(t)
+    dst = op.initialize_dof_vector()
+    op.vmult(dst, torch.tensor(T, device="cuda"))
+    torch.cuda.synchronize()
+    return dst.cpu().numpy()
+
+
+def eb_problem(n_cells=(16, 12, 8), method=FORWARD_EULER):
+    p = plain_large(n_cells=n_cells, method=method)
+    b = p.beams[0]
+    b.type = SOURCE_ELECTRON_BEAM
+    b.max_power = 1500.0
+    b.absorption_efficiency = 0.85
+    b.radius = 150e-6
+    b.depth = 150e-6
+    return p
+
+
+@pytest.mark.parametrize("mode", ["plane", "line"])
+def test_electron_beam_vmult_and_steps(mode):
+    import torch
+    from adamantine_amd.thermal import (LAUNCH_LINE_COLORED, LAUNCH_PLANE,
+                                        ThermalOperatorDevice,
+                                        ThermalPhysicsDevice)
+    launch = LAUNCH_PLANE if mode == "plane" else LAUNCH_LINE_COLORED
+    print("electron beam:", mode, flush=True)
+    p = eb_problem()
+    op = ThermalOperatorDevice(p)
+    op.set_launch_config(launch, 4)
+    o = Oracle(p)
+    T = random_temperature(p, seed=21, low=300.0, high=2200.0)
+    for t in (1e-4, 3e-4, 6e-4):
+        o.set_time(t)
+        want = o.vmult(T)
+        got = _vmult(op, T, t)
+        assert rel(got, want) < VMULT_TOL, (t, rel(got, want))
+    # the source is on and matters: the vmult differs from the one without it
+    q = eb_problem()
+    q.beams[0].max_power = 0.0
+    oq, o2 = Oracle(q), Oracle(p)
+    oq.set_time(3e-4)
+    o2.set_time(3e-4)
+    assert rel(oq.vmult(T), o2.vmult(T)) > 1e-3
+    lq, pw = op.qp_state()
+    olq, opw = o.qp_state()
+    np.testing.assert_array_equal(pw, opw)
+    np.testing.assert_array_equal(lq, olq)
+    ph = ThermalPhysicsDevice(p)
+    ph.operator.set_launch_config(launch, 4)
+    ph.setup()
+    sol = torch.full((p.n_dofs,), 300.0, dtype=torch.float64, device="cuda")
+    ph.evolve(0.0, 2e-7, 40, sol)
+    want, _ = Oracle(p).evolve(FORWARD_EULER, 0.0, 2e-7, 40,
+                               np.full(p.n_dofs, 300.0))
+    torch.cuda.synchronize()
+    got = sol.cpu().numpy()
+    assert got.max() > 310.0  # the beam heated the top
+    assert rel(got, want) < STEP_TOL
+
+
+def table_material(kind):
+    """Solid/liquid/powder tables; kind 'general': a step in the specific
+    heat (a repeated temperature), outside the regular form; kind 'twelve':
+    regular tables with all 12 entries."""
+    m = Material(n_materials=1, n_states=3, use_table=True)
+    if kind == "twelve":
+        Ts = np.linspace(300.0, 2400.0, 12)
+        cp = [(t, 500.0 + 0.2 * (t - 300.0) + 7.0 * np.sin(t / 97.0))
+              for t in Ts]
+        k = [(t, 12.0 + 0.011 * t + np.cos(t / 131.0)) for t in Ts]
+    else:
+        cp = [(300.0, 520.0), (900.0, 640.0), (900.0, 700.0),
+              (1500.0, 760.0), (2400.0, 820.0)]
+        k = [(300.0, 14.0), (1200.0, 24.0), (2400.0, 31.0)]
+    for st, rho_f, k_f in (("solid", 1.0, 1.0), ("liquid", 0.95, 1.3),
+                           ("powder", 0.6, 0.01)):
+        m.set_table(0, "density", st, [(300.0, 7900.0 * rho_f)])
+        m.set_table(0, "specific_heat", st, cp)
+        for name in ("thermal_conductivity_x", "thermal_conductivity_y",
+                     "thermal_conductivity_z"):
+            m.set_table(0, name, st, [(t, v * k_f) for t, v in k])
+        m.set_table(0, "emissivity", st, [(300.0, 0.3)])
+        m.set_table(0, "convection_heat_transfer_coef", st, [(300.0, 10.0)])
+    m.set_property(0, "solidus", 1675.0)
+    m.set_property(0, "liquidus", 1708.0)
+    m.set_property(0, "latent_heat", 2.6e5)
+    m.set_property(0, "radiation_temperature_infty", 300.0)
+    m.set_property(0, "convection_temperature_infty", 300.0)
+    return m
+
+
+@pytest.mark.parametrize("kind", ["general", "twelve"])
+@pytest.mark.parametrize("mode", ["plane", "line"])
+def test_table_forms(kind, mode):
+    import torch
+    from adamantine_amd.thermal import (LAUNCH_LINE_COLORED, LAUNCH_PLANE,
+                                        ThermalOperatorDevice,
+                                        ThermalPhysicsDevice)
+    launch = LAUNCH_PLANE if mode == "plane" else LAUNCH_LINE_COLORED
+    print("tables:", kind, mode, flush=True)
+    p = plain_large(n_cells=(14, 9, 8), method=RK_FOURTH_ORDER)
+    p.material = table_material(kind)
+    hz = p.extent[2] / p.n_cells[2]
+    p.set_powder_layer(p.extent[2], 2 * hz)
+    op = ThermalOperatorDevice(p)
+    op.set_launch_config(launch, 4)
+    o = Oracle(p)
+    # temperatures across every interval, the last one included
+    T = random_temperature(p, seed=31, low=250.0, high=2600.0)
+    for t in (0.0, 2e-4):
+        o.set_time(t)
+        assert rel(_vmult(op, T, t), o.vmult(T)) < VMULT_TOL
+    lq, pw = op.qp_state()
+    olq, opw = o.qp_state()
+    np.testing.assert_array_equal(pw, opw)
+    np.testing.assert_array_equal(lq, olq)
+    ph = ThermalPhysicsDevice(p)
+    ph.operator.set_launch_config(launch, 4)
+    ph.setup()
+    T0 = random_temperature(p, seed=32, low=1500.0, high=2300.0)
+    sol = torch.tensor(T0, device="cuda")
+    ph.evolve(0.0, 5e-7, 6, sol)
+    want, _ = Oracle(p).evolve(RK_FOURTH_ORDER, 0.0, 5e-7, 6, T0.copy())
+    torch.cuda.synchronize()
+    assert rel(sol.cpu().numpy(), want) < STEP_TOL
+
+
+def test_enkf_ensemble_at_size():
+    """64 members x 64x32x16 Q2 (276,705 nodes each): 6 RK4 steps of the
+    batched ensemble, members 0, 21, 42, 63 against their oracles; then one
+    analysis, 500 observations at the top of the scanned region, R = 4 I,
+    augmented with absorption efficiency and conductivity scale."""
+    import torch
+    from adamantine_amd.ensemble import (LOCALIZATION, DataAssimilator,
+                                         EnsembleDevice, member_problem)
+    from adamantine_amd.workloads import ensemble_workload
+    from tests.test_oracle_enkf import L, p as ptr
+    print("config 3 at size: 64 members, 6 RK4 steps + analysis", flush=True)
+    base, members = ensemble_workload(n_members=64)
+    ens = EnsembleDevice(base, members)
+    ens.compute_inverse_mass_matrix()
+    n = ens.n_dofs
+    T0 = np.full((64, n), 300.0)
+    sol = torch.tensor(T0, device="cuda")
+    dt = 5e-7
+    t0 = 2e-4  # the beam is on its first track
+    t_end = ens.evolve(t0, dt, 6, sol)
+    torch.cuda.synchronize()
+    X = sol.cpu().numpy()
+    for j in (0, 21, 42, 63):
+        want, t_ref = Oracle(member_problem(base, members[j])).evolve(
+            RK_FOURTH_ORDER, t0, dt, 6, np.full(n, 300.0))
+        assert t_ref == t_end
+        assert rel(X[j], want) < STEP_TOL, (j, rel(X[j], want))
+    assert X.max() > 350.0  # the beam heated the members
+    # analysis
+    rng = np.random.RandomState(5)
+    Xc = base.node_coordinates()
+    zt = Xc[:, 2].max() * 0.75  # the lowest member's top surface
+    pts = np.stack([rng.uniform(0.2e-3, 3.8e-3, 500),
+                    rng.uniform(0.7e-3, 1.3e-3, 500), np.full(500, zt)],
+                   axis=1)
+    da = DataAssimilator()
+    obs = da.update_dof_mapping(ens, pts)
+    m = obs.size
+    assert m == 500
+    y = X[:, obs].mean(axis=0) + rng.normal(scale=2.0, size=m)
+    R = 4.0 * np.eye(m)
+    noise = da.fill_noise_vectors(64, R)
+    params = np.ascontiguousarray(np.stack(
+        [[mm.absorption_efficiency[0] for mm in members],
+         [mm.conductivity_scale for mm in members]], axis=1))
+    got_p = params.copy()
+    dev = torch.tensor(X, device="cuda")
+    da.update_ensemble(ens, dev, y, R, params=got_p, noise=noise)
+    torch.cuda.synchronize()
+    got = dev.cpu().numpy()
+    # the oracle on the observed entries, 300 others and the parameters
+    others = np.setdiff1d(rng.choice(n, 400, replace=False), obs)[:300]
+    sub = np.concatenate([np.unique(obs), others])
+    pos = {int(d): i for i, d in enumerate(sub)}
+    obs_sub = np.array([pos[int(d)] for d in obs], dtype=np.int64)
+    aug = np.ascontiguousarray(np.concatenate([X[:, sub], params], axis=1))
+    coords = np.ascontiguousarray(Xc[sub])
+    L().oracle_update_ensemble(64, aug.shape[1], sub.size, ptr(aug),
+                               ptr(coords), 3, LOCALIZATION["none"],
+                               np.finfo(float).max, m, ptr(obs_sub), ptr(y),
+                               ptr(R), ptr(noise))
+    want_X, want_p = aug[:, :sub.size], aug[:, sub.size:]
+    scale = np.abs(want_X - X[:, sub]).max()
+    assert scale > 1e-3  # the analysis moved the states
+    assert np.abs(got[:, sub] - want_X).max() <= 1e-10 * scale + 1e-12 * 1e3
+    pscale = np.abs(want_p - params).max()
+    assert pscale > 0
+    np.testing.assert_allclose(got_p, want_p, rtol=0, atol=1e-10 * pscale)
+
+
+def test_deposition_ensemble_at_size():
+    """32 members x 96x96x40 Q2 (3,017,169 nodes each), electron beam,
+    forward Euler 2e-7 s: all members step to 190; from there members 0 and
+    7 (deposition delays below 40 steps) step to 240 through their second
+    layer's activation against their oracles, started from the device
+    temperature at step 190."""
+    import torch
+    from adamantine_amd.ensemble import EnsembleDevice, member_problem
+    from adamantine_amd.workloads import deposition_workload
+    print("config 5 at size: 32 members of 96x96x40, deposition", flush=True)
+    w = deposition_workload(n_members=32)
+    ens = EnsembleDevice(w.base, w.members)
+    ens.compute_inverse_mass_matrix()
+    sol = ens.initialize_dof_vector(300.0)
+    t = ens.evolve(0.0, w.dt, 190, sol)
+    torch.cuda.synchronize()
+    picks = [j for j in range(32) if w.activation_step[j, 1] < 240][:2]
+    assert len(picks) == 2
+    T190 = {j: sol[j].cpu().numpy().copy() for j in picks}
+    oracles = {}
+    for j in picks:
+        o = Oracle(member_problem(w.base, w.members[j]))
+        oracles[j] = o
+    n_act = 0
+    for step in range(190, 240):
+        for j in np.nonzero(w.activation_step[:, 1] == step)[0]:
+            b = ens.elements_to_activate(int(j), w.layer_boxes[1:2])
+            ens.add_material(int(j), b, [1.0], [0.0], 300.0, sol)
+            n_act += 1
+        t = ens.evolve(t, w.dt, 1, sol)
+    torch.cuda.synchronize()
+    for j in picks:
+        o = oracles[j]
+        T = T190[j]
+        tj = 0.0
+        for k in range(190):  # the time the device loop reached
+            tj = tj + w.dt
+        for step in range(190, 240):
+            if w.activation_step[j, 1] == step:
+                b = o.cells_to_activate(w.layer_boxes[1:2])
+                assert (b >= 0).sum() > 0
+                o.add_material(b, np.array([1.0]), np.array([0.0]), 300.0, T)
+            T, tj = o.evolve(FORWARD_EULER, tj, w.dt, 1, T)
+        got = sol[j].cpu().numpy()
+        assert rel(got, T) < STEP_TOL, (j, rel(got, T))
+        assert got.max() > 400.0  # the beam heated the member
+    assert n_act > 0
