@@ -646,6 +646,10 @@ Context::~Context() {
   ADM_FREE(d_powder);
   ADM_FREE(d_face_powder);
   ADM_FREE(d_faces);
+  ADM_FREE(d_face_out);
+  ADM_FREE(d_fn_node);
+  ADM_FREE(d_fn_offset);
+  ADM_FREE(d_fn_slot);
   ADM_FREE(d_beam_table);
   ADM_FREE(d_slot);
   ADM_FREE(d_patch);
@@ -804,6 +808,67 @@ void Context::rebuild_cells() {
     if (n_states == 3)
       ADM_HIP(hipMalloc(&d_face_powder, sizeof(double) * faces.size() * nfq));
   }
+  build_face_nodes();
+}
+
+// The faces' node lists of the face gather pass: every (face, local node)
+// slot, grouped by node (nodes in increasing order, each node's slots in
+// increasing face order).
+void Context::build_face_nodes() {
+  ADM_FREE(d_face_out);
+  ADM_FREE(d_fn_node);
+  ADM_FREE(d_fn_offset);
+  ADM_FREE(d_fn_slot);
+  d_face_out = nullptr;
+  d_fn_node = nullptr;
+  d_fn_offset = nullptr;
+  d_fn_slot = nullptr;
+  n_fnodes = 0;
+  if (faces.empty()) return;
+  const int N = p + 1, NF = N * N;
+  const int64_t ns[3] = {1, (int64_t)nn[0], (int64_t)nn[0] * nn[1]};
+  std::vector<std::pair<int64_t, int32_t>> ent;
+  ent.reserve(faces.size() * NF);
+  for (size_t f = 0; f < faces.size(); ++f) {
+    const int64_t c = faces[f].cell;
+    const int fid = faces[f].info & 7, d = fid / 2, side = fid % 2;
+    const int64_t cc[3] = {c % nc[0], (c / nc[0]) % nc[1],
+                           c / ((int64_t)nc[0] * nc[1])};
+    const int64_t base = cc[0] * p + ns[1] * cc[1] * p + ns[2] * cc[2] * p +
+                         (side ? p : 0) * ns[d];
+    const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+    for (int q = 0; q < NF; ++q)
+      ent.emplace_back(base + (q % N) * ns[t1] + (q / N) * ns[t2],
+                       (int32_t)(f * NF + q));
+  }
+  if (ent.size() > (size_t)INT32_MAX)
+    throw AdmError(ADM_ERR_UNSUPPORTED, "too many boundary-face nodes");
+  std::stable_sort(ent.begin(), ent.end(),
+                   [](const std::pair<int64_t, int32_t> &x,
+                      const std::pair<int64_t, int32_t> &y) {
+                     return x.first < y.first;
+                   });
+  std::vector<int64_t> node;
+  std::vector<int32_t> off, slot;
+  for (size_t e = 0; e < ent.size(); ++e) {
+    if (e == 0 || ent[e].first != ent[e - 1].first) {
+      node.push_back(ent[e].first);
+      off.push_back((int32_t)e);
+    }
+    slot.push_back(ent[e].second);
+  }
+  off.push_back((int32_t)ent.size());
+  n_fnodes = (int64_t)node.size();
+  ADM_HIP(hipMalloc(&d_face_out, sizeof(double) * faces.size() * NF));
+  ADM_HIP(hipMalloc(&d_fn_node, sizeof(int64_t) * node.size()));
+  ADM_HIP(hipMalloc(&d_fn_offset, sizeof(int32_t) * off.size()));
+  ADM_HIP(hipMalloc(&d_fn_slot, sizeof(int32_t) * slot.size()));
+  ADM_HIP(hipMemcpy(d_fn_node, node.data(), sizeof(int64_t) * node.size(),
+                    hipMemcpyHostToDevice));
+  ADM_HIP(hipMemcpy(d_fn_offset, off.data(), sizeof(int32_t) * off.size(),
+                    hipMemcpyHostToDevice));
+  ADM_HIP(hipMemcpy(d_fn_slot, slot.data(), sizeof(int32_t) * slot.size(),
+                    hipMemcpyHostToDevice));
 }
 
 // get_state_from_material_properties: every quadrature point of a cell (and
@@ -1063,6 +1128,11 @@ FaceArgs Context::face_args() const {
     a.nn[k] = nn[k];
   }
   a.n_faces = (int64_t)faces.size();
+  a.face_out = d_face_out;
+  a.fn_node = d_fn_node;
+  a.fn_offset = d_fn_offset;
+  a.fn_slot = d_fn_slot;
+  a.n_fnodes = n_fnodes;
   return a;
 }
 
@@ -1225,6 +1295,10 @@ void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
     ADM_DISPATCH(5, 2)
     ADM_DISPATCH(5, 3)
 #undef ADM_DISPATCH
+    if (fa.n_fnodes)
+      hipLaunchKernelGGL(face_gather_kernel,
+                         dim3((unsigned)((fa.n_fnodes + 255) / 256), M),
+                         dim3(256), 0, s, fa, dst);
   }
   ADM_HIP(hipGetLastError());
 }
@@ -1507,6 +1581,7 @@ void Ensemble::refresh_members() {
   const Context &c0 = *members[0];
   std::vector<MemberArgs> h(n);
   max_faces = 0;
+  max_fnodes = 0;
   for (int j = 0; j < n; ++j) {
     const Context &c = *members[j];
     if (c.property_kind() != c0.property_kind())
@@ -1527,6 +1602,12 @@ void Ensemble::refresh_members() {
     m.faces = c.d_faces;
     m.n_faces = (int64_t)c.faces.size();
     max_faces = std::max(max_faces, m.n_faces);
+    m.face_out = c.d_face_out;
+    m.fn_node = c.d_fn_node;
+    m.fn_offset = c.d_fn_offset;
+    m.fn_slot = c.d_fn_slot;
+    m.n_fnodes = c.n_fnodes;
+    max_fnodes = std::max(max_fnodes, m.n_fnodes);
     for (size_t b = 0; b < c.beams.size(); ++b)
       m.beam_gain[b] =
           c.beams[b].desc.absorption_efficiency * c.beams[b].desc.max_power;
@@ -1566,6 +1647,7 @@ void Ensemble::apply(const double *src, double *dst, int add, hipStream_t s) {
   fa.members = d_members;
   fa.member_stride = n_dofs;
   fa.n_faces = max_faces;
+  fa.n_fnodes = max_fnodes;
   int nz = 0;
   for (const auto &m : members) nz = std::max(nz, m->nz_active);
   c0.launch(a, fa, src, dst, (unsigned)n_members, s, nz);
@@ -2152,6 +2234,37 @@ int adm_ensemble_get_qp_state(void *ptr, int32_t member, double *liquid,
     return adm::set_error(ADM_ERR_INVALID, "member index out of range");
   ADM_HIP(hipDeviceSynchronize());
   ens.members[member]->get_qp_state(liquid, powder);
+  ADM_API_END
+}
+
+int adm_ensemble_set_qp_state(void *ptr, int32_t member, const double *liquid,
+                              const double *powder) {
+  ADM_API_BEGIN
+  ADM_ENS(ptr)
+  if (member < 0 || member >= ens.n_members)
+    return adm::set_error(ADM_ERR_INVALID, "member index out of range");
+  ADM_HIP(hipDeviceSynchronize());
+  ens.members[member]->set_qp_state(liquid, powder);
+  ADM_API_END
+}
+
+int adm_ensemble_get_face_state(void *ptr, int32_t member,
+                                double *face_powder) {
+  ADM_API_BEGIN
+  ADM_ENS(ptr)
+  if (member < 0 || member >= ens.n_members)
+    return adm::set_error(ADM_ERR_INVALID, "member index out of range");
+  return adm_get_face_state(ens.members[member].get(), face_powder);
+  ADM_API_END
+}
+
+int adm_ensemble_set_face_state(void *ptr, int32_t member,
+                                const double *face_powder) {
+  ADM_API_BEGIN
+  ADM_ENS(ptr)
+  if (member < 0 || member >= ens.n_members)
+    return adm::set_error(ADM_ERR_INVALID, "member index out of range");
+  return adm_set_face_state(ens.members[member].get(), face_powder);
   ADM_API_END
 }
 

@@ -57,6 +57,7 @@ struct Context {
   void upload_constants();
   uint32_t face_mask(int64_t c) const;
   void rebuild_cells();
+  void build_face_nodes();
   void set_cell_state(const double *cs);
   void get_qp_state(double *liquid, double *powder);
   void set_qp_state(const double *liquid, const double *powder);
@@ -113,6 +114,13 @@ struct Context {
   // 1 + the highest cell layer (z index) holding an active cell, 0 if none
   int nz_active = 0;
   std::vector<FaceRef> faces;
+  // the faces' nodes: node list, per-node offsets into the slot list, and
+  // the slots f * (p+1)^2 + j of face_out (FaceArgs)
+  int64_t n_fnodes = 0;
+  double *d_face_out = nullptr;
+  int64_t *d_fn_node = nullptr;
+  int32_t *d_fn_offset = nullptr;
+  int32_t *d_fn_slot = nullptr;
 
   std::vector<BeamData> beams;
   std::vector<int> old_segment;
@@ -192,6 +200,7 @@ struct Ensemble {
   int n_members = 0;
   int64_t n_dofs = 0;
   int64_t max_faces = 0;
+  int64_t max_fnodes = 0;
   std::vector<std::unique_ptr<Context>> members;
   MemberArgs *d_members = nullptr;
   double pmod[ADM_MAX_BEAMS] = {0, 0, 0, 0};

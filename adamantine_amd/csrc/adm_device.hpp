@@ -115,6 +115,12 @@ struct MemberArgs {
   const FaceRef *faces;
   int64_t n_faces;
   double beam_gain[ADM_MAX_BEAMS];
+  // the member's face-node lists (FaceArgs)
+  double *face_out;
+  const int64_t *fn_node;
+  const int32_t *fn_offset;
+  const int32_t *fn_slot;
+  int64_t n_fnodes;
 };
 
 // Arguments of the cell kernel (by value).
@@ -178,6 +184,15 @@ struct FaceArgs {
   int64_t n_faces;      // ensemble: the largest member face count
   const MemberArgs *members;  // NULL: one operator
   int64_t member_stride;
+  // The face kernel writes node j of face f to face_out[f * (p+1)^2 + j];
+  // the face gather pass then adds, for boundary node k (fn_node[k]), the
+  // slots fn_slot[fn_offset[k] .. fn_offset[k+1]) (increasing face order)
+  // into dst. Every run adds the same terms in the same order.
+  double *face_out;
+  const int64_t *fn_node;
+  const int32_t *fn_offset;
+  const int32_t *fn_slot;
+  int64_t n_fnodes;     // ensemble: the largest member count
 };
 
 // ---------------------------------------------------------------------------

@@ -136,6 +136,7 @@ __global__ void __launch_bounds__(256)
   const int qa = q % N, qb = q / N;
   const ADM_CONSTANT MemberArgs *mb =
       a.members ? as_constant(a.members) + blockIdx.y : nullptr;
+  ADM_GLOBAL double *face_out = as_global(mb ? mb->face_out : a.face_out);
   const int64_t fi = ((int64_t)blockIdx.x * 4 + wave) * FPW + jf;
   // whole faces (all lanes of a face) agree
   if (jf >= FPW || fi >= (mb ? mb->n_faces : a.n_faces)) return;
@@ -209,7 +210,25 @@ __global__ void __launch_bounds__(256)
   double out = kc.S[0][qa] * ft[0];
 #pragma unroll
   for (int i = 1; i < N; ++i) out = __builtin_fma(kc.S[i][qa], ft[i], out);
-  unsafeAtomicAdd(dst + node, out);
+  face_out[fi * NF + q] = out;
+}
+
+// Second pass of the face term: one lane per boundary node adds its faces'
+// slots, in increasing face order, to dst.
+static __global__ void __launch_bounds__(256)
+    face_gather_kernel(const FaceArgs a, double *__restrict__ dst) {
+  const ADM_CONSTANT MemberArgs *mb =
+      a.members ? as_constant(a.members) + blockIdx.y : nullptr;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (mb ? mb->n_fnodes : a.n_fnodes)) return;
+  const ADM_GLOBAL double *fo = as_global(mb ? mb->face_out : a.face_out);
+  const ADM_GLOBAL int32_t *off = as_global(mb ? mb->fn_offset : a.fn_offset);
+  const ADM_GLOBAL int32_t *slot = as_global(mb ? mb->fn_slot : a.fn_slot);
+  const int64_t node = as_global(mb ? mb->fn_node : a.fn_node)[k];
+  if (mb) dst += (int64_t)blockIdx.y * a.member_stride;
+  double sum = 0.0;
+  for (int32_t e = off[k]; e < off[k + 1]; ++e) sum += fo[slot[e]];
+  dst[node] += sum;
 }
 
 // dot product of row r of M with an N-line of V (same chain as sweep())

@@ -68,6 +68,9 @@ def _bind(L):
     L.adm_ensemble_evolve.argtypes = [_vp, C.c_int, C.c_double, C.c_double,
                                       C.c_int, _vp, _dp, _vp]
     L.adm_ensemble_get_qp_state.argtypes = [_vp, C.c_int32, _dp, _dp]
+    L.adm_ensemble_set_qp_state.argtypes = [_vp, C.c_int32, _dp, _dp]
+    L.adm_ensemble_get_face_state.argtypes = [_vp, C.c_int32, _dp]
+    L.adm_ensemble_set_face_state.argtypes = [_vp, C.c_int32, _dp]
     _ip = C.POINTER(C.c_int64)
     L.adm_da_create.argtypes = [C.POINTER(_vp)]
     L.adm_da_destroy.argtypes = [_vp]
@@ -202,6 +205,7 @@ class EnsembleDevice:
         self.n_dofs = n.value
         self.n_cells = nc
         self.n_q = (base.fe_degree + 1) ** base.dim
+        self.n_face_q = (base.fe_degree + 1) ** (base.dim - 1)
         self._inverse_mass = None
 
     def __del__(self):
@@ -324,6 +328,28 @@ class EnsembleDevice:
             self._h, int(member), liq.ctypes.data_as(_dp),
             pw.ctypes.data_as(_dp)))
         return liq, pw
+
+    def set_qp_state(self, member: int, liquid, powder):
+        lq = np.ascontiguousarray(liquid, dtype=np.float64)
+        pw = np.ascontiguousarray(powder, dtype=np.float64)
+        if lq.size != self.n_cells * self.n_q or pw.size != lq.size:
+            raise ValueError("quadrature-point state of the wrong size")
+        check(_lib().adm_ensemble_set_qp_state(
+            self._h, int(member), lq.ctypes.data_as(_dp),
+            pw.ctypes.data_as(_dp)))
+
+    def face_state(self, member: int):
+        fp = np.zeros((self.n_cells, 6, self.n_face_q))
+        check(_lib().adm_ensemble_get_face_state(
+            self._h, int(member), fp.ctypes.data_as(_dp)))
+        return fp
+
+    def set_face_state(self, member: int, face_powder):
+        fp = np.ascontiguousarray(face_powder, dtype=np.float64)
+        if fp.size != self.n_cells * 6 * self.n_face_q:
+            raise ValueError("face state of the wrong size")
+        check(_lib().adm_ensemble_set_face_state(
+            self._h, int(member), fp.ctypes.data_as(_dp)))
 
 
 # ---------------------------------------------------------------------------

@@ -357,6 +357,13 @@ def run_ensemble(db: Dict, base_dir: str = ".",
         restart = (saved, float(tok[0]), int(tok[1]))
     ens = EnsembleDevice(p, members)
     ens.compute_inverse_mass_matrix()
+    if restart:
+        # the saved point histories (files without them: every point took
+        # its cell's state above)
+        for j, d in enumerate(restart[0]):
+            if "powder_qp" in d:
+                ens.set_qp_state(j, d["liquid_qp"], d["powder_qp"])
+                ens.set_face_state(j, d["face_powder"])
     # deposition boxes of member 0's sources, shared by all members
     # (application/adamantine.hh:1917-1925)
     from bisect import bisect_left
@@ -530,7 +537,9 @@ def run_ensemble(db: Dict, base_dir: str = ".",
                          active=active[j],
                          deposition_cos=np.where(on, dcos[j], 10.0),
                          deposition_sin=np.where(on, dsin[j], 10.0),
-                         has_melted=np.zeros(nc, np.uint8))
+                         has_melted=np.zeros(nc, np.uint8),
+                         liquid_qp=lq, powder_qp=pw,
+                         face_powder=ens.face_state(j))
             with open(prefix + "_time.txt", "w") as f:
                 f.write("%r %d\n" % (time, n_step))
         if pps and (n_step % every_out == 0 or

@@ -394,10 +394,14 @@ class ThermalPhysicsDevice:
         per cell the state ratios, the deposition cos/sin (10 on inactive
         cells), has_melted and the active flag, plus the temperature, into
         <filename>.npz. The mesh is the input file's box, so it is not
-        saved. The state is the cell average of the quadrature-point state
-        at this moment."""
+        saved. The cell state is the average of the quadrature-point state
+        at this moment; the quadrature-point liquid and powder ratios and
+        the boundary faces' powder are saved too, so that a restart goes on
+        from the same point history (the reference keeps only the cell
+        averages)."""
         active, cos_, sin_ = self.operator.cell_data()
         state = self.set_state_to_material_properties()
+        lq, pw = self.operator.qp_state()
         on = active.astype(bool)
         T = solution.detach().cpu().numpy() if hasattr(solution, "cpu") \
             else np.asarray(solution, np.float64)
@@ -405,13 +409,17 @@ class ThermalPhysicsDevice:
                  cell_state=state, active=active,
                  deposition_cos=np.where(on, cos_, 10.0),
                  deposition_sin=np.where(on, sin_, 10.0),
-                 has_melted=np.zeros(self.operator.n_cells, np.uint8))
+                 has_melted=np.zeros(self.operator.n_cells, np.uint8),
+                 liquid_qp=lq, powder_qp=pw,
+                 face_powder=self.operator.face_state())
 
     def load_checkpoint(self, filename: str):
         """load_checkpoint (source/ThermalPhysics.templates.hh:974-1085):
         rebuilds the operator with the saved active cells, orientations and
-        cell states (every quadrature point takes its cell's state), redoes
-        setup(), and returns the saved temperature on the device."""
+        cell states, redoes setup(), puts back the saved quadrature-point
+        and face-point state (a file without it: every point takes its
+        cell's state, as in the reference), and returns the saved
+        temperature on the device."""
         import dataclasses
         import torch
         with np.load(filename + ".npz", allow_pickle=False) as f:
@@ -430,5 +438,9 @@ class ThermalPhysicsDevice:
         self.problem = p
         self.operator = ThermalOperatorDevice(p)
         self.setup()
+        if "powder_qp" in data:
+            # the saved point history, not the cell averages
+            self.operator.set_qp_state(data["liquid_qp"], data["powder_qp"])
+            self.operator.set_face_state(data["face_powder"])
         return torch.tensor(data["temperature"], dtype=torch.float64,
                             device="cuda")

@@ -479,6 +479,15 @@ int adm_ensemble_evolve(void *ens, int method, double t, double dt,
  * [n_cells][n_q]; either pointer may be NULL). */
 int adm_ensemble_get_qp_state(void *ens, int32_t member, double *liquid,
                               double *powder);
+/* The member's counterparts of adm_set_qp_state, adm_get_face_state and
+ * adm_set_face_state (checkpoint and restart of an ensemble with its
+ * quadrature-point history). */
+int adm_ensemble_set_qp_state(void *ens, int32_t member, const double *liquid,
+                              const double *powder);
+int adm_ensemble_get_face_state(void *ens, int32_t member,
+                                double *face_powder);
+int adm_ensemble_set_face_state(void *ens, int32_t member,
+                                const double *face_powder);
 
 /* update_physics_parameters of one member (source/ThermalPhysics.templates.hh:
  * 827-852) for beam `beam`'s absorption efficiency and max power, e.g. after
