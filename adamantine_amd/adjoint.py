@@ -10,7 +10,7 @@ initial temperature and with respect to three physics parameters:
   2  radius of beam 0.
 
 The work is done by ``adm_adjoint_gradient`` of libadm.so: the transposed
-stage sequence of forward Euler or classic RK4, the linearised cell and face
+stage sequence of forward Euler, RK3 or classic RK4, the linearised cell and face
 operators as HIP kernels, the trajectory recomputed from checkpoints.
 """
 from __future__ import annotations

@@ -4,7 +4,7 @@
 //   J = 1/2 sum_k w_k (T_final[dof_k] - y_k)^2
 // this computes dJ/dT0 and dJ/dtheta for theta = (absorption efficiency of
 // beam 0, a factor on every thermal conductivity at its current value,
-// radius of beam 0) by the exact transposition of the forward Euler or
+// radius of beam 0) by the exact transposition of the forward Euler, RK3 or
 // classic RK4 stage sequence (the forward run is the one of
 // ThermalPhysics::evolve_one_time_step, source/ThermalPhysics.templates.hh:
 // 854-888), with the trajectory recomputed from checkpoints.
@@ -766,7 +766,7 @@ struct AdjointRun {
   int64_t n;
   int64_t npw, nfpw;  // powder values of cells and faces
   int n_stages;
-  double a_next[4], b[4], cst[4];
+  double a[4][4], b[4], cst[4];  // Butcher tableau of the forward method
   DevBuf partials;
   DevBuf K[4], Y, R;
   DevBuf pbar, fpbar;  // adjoints of the carried cell and face powder
@@ -776,23 +776,37 @@ struct AdjointRun {
     n = c.n_dofs;
     npw = c.n_powder_cells * c.nq;
     nfpw = (c.n_states == 3) ? (int64_t)c.faces.size() * c.nfq : 0;
+    std::memset(a, 0, sizeof(a));
     if (method == ADM_FORWARD_EULER) {
       n_stages = 1;
       b[0] = 1.0;
       cst[0] = 0.0;
     } else if (method == ADM_RK_FOURTH_ORDER) {
+      // classic RK4 (TimeStepping::RK_CLASSIC_FOURTH_ORDER)
       n_stages = 4;
       const double bb[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
       const double cc[4] = {0.0, 0.5, 0.5, 1.0};
-      const double an[4] = {0.5, 0.5, 1.0, 0.0};
       for (int i = 0; i < 4; ++i) {
         b[i] = bb[i];
         cst[i] = cc[i];
-        a_next[i] = an[i];
       }
+      a[1][0] = 0.5;
+      a[2][1] = 0.5;
+      a[3][2] = 1.0;
+    } else if (method == ADM_RK_THIRD_ORDER) {
+      // Kutta's third order (TimeStepping::RK_THIRD_ORDER)
+      n_stages = 3;
+      const double bb[3] = {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0};
+      const double cc[3] = {0.0, 0.5, 1.0};
+      for (int i = 0; i < 3; ++i) {
+        b[i] = bb[i];
+        cst[i] = cc[i];
+      }
+      a[1][0] = 0.5;
+      a[2][0] = -1.0;
+      a[2][1] = 2.0;
     } else {
-      throw AdmError(ADM_ERR_UNSUPPORTED,
-                     "the adjoint supports forward Euler and RK4");
+      throw AdmError(ADM_ERR_INVALID, "unknown time-stepping method");
     }
     for (int i = 0; i < n_stages; ++i) K[i].alloc(n);
     Y.alloc(n);
@@ -818,11 +832,19 @@ struct AdjointRun {
   };
   void forward_step(double t, double dt, double *T, const StageRecord *rec) {
     for (int i = 0; i < n_stages; ++i) {
-      // stage input: Y_i = T + dt a_{i,i-1} K_{i-1} (zero a_ij of RK4 skipped)
+      // stage input: Y_i = T + sum_j dt a_ij K_j in j order (the zero
+      // entries skipped, as the stage kernels of the step do)
       const double *Yi = T;
-      if (i > 0) {
-        hipLaunchKernelGGL(axpy_copy_kernel, dim3(grid(n)), dim3(256), 0, s,
-                           Y.p, T, dt * a_next[i - 1], K[i - 1].p, n);
+      bool first = true;
+      for (int j = 0; j < i; ++j) {
+        if (a[i][j] == 0.0) continue;
+        if (first)
+          hipLaunchKernelGGL(axpy_copy_kernel, dim3(grid(n)), dim3(256), 0, s,
+                             Y.p, T, dt * a[i][j], K[j].p, n);
+        else
+          hipLaunchKernelGGL(axpy_kernel, dim3(grid(n)), dim3(256), 0, s, Y.p,
+                             K[j].p, dt * a[i][j], n);
+        first = false;
         Yi = Y.p;
       }
       if (rec) {
@@ -993,7 +1015,7 @@ int adm_adjoint_gradient(void *ptr, int method, double t0, double dt,
       for (int k = k1 - 1; k >= k0; --k) {
         const adm::AdjointRun::StageRecord *rk = &rec[(size_t)(k - k0) * S];
         // stage adjoints kb_i start at dt b_i Tb; Tb_new = Tb + sum Ybar_i;
-        // kb_{i-1} += dt a_{i,i-1} Ybar_i (reverse stage order)
+        // kb_j += dt a_ij Ybar_i for j < i (reverse stage order)
         for (int i = 0; i < S; ++i)
           hipLaunchKernelGGL(adm::scale_kernel, dim3(gn), dim3(256), 0, s,
                              run.K[i].p, Tb.p, dt * run.b[i], n);
@@ -1004,10 +1026,10 @@ int adm_adjoint_gradient(void *ptr, int method, double t0, double dt,
           run.vjp(rk[i], times[k] + run.cst[i] * dt, mu.p, Ybar.p, inv_eta0);
           hipLaunchKernelGGL(adm::axpy_kernel, dim3(gn), dim3(256), 0, s, Tb.p,
                              Ybar.p, 1.0, n);
-          if (i > 0)
-            hipLaunchKernelGGL(adm::axpy_kernel, dim3(gn), dim3(256), 0, s,
-                               run.K[i - 1].p, Ybar.p, dt * run.a_next[i - 1],
-                               n);
+          for (int j = 0; j < i; ++j)
+            if (run.a[i][j] != 0.0)
+              hipLaunchKernelGGL(adm::axpy_kernel, dim3(gn), dim3(256), 0, s,
+                                 run.K[j].p, Ybar.p, dt * run.a[i][j], n);
         }
       }
     }

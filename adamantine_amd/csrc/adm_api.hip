@@ -1619,6 +1619,37 @@ void Ensemble::refresh_members() {
 
 Ensemble::~Ensemble() {
   if (d_members) (void)hipFree(d_members);
+  if (d_da) (void)hipFree(d_da);
+  if (h_da) (void)hipHostFree(h_da);
+  if (da_done) (void)hipEventDestroy(da_done);
+}
+
+double *Ensemble::da_device(size_t n) {
+  if (n > da_cap) {
+    if (d_da) {
+      ADM_HIP(hipDeviceSynchronize());
+      ADM_HIP(hipFree(d_da));
+      d_da = nullptr;
+    }
+    const size_t cap = std::max(n, da_cap * 2);
+    ADM_HIP(hipMalloc(&d_da, sizeof(double) * cap));
+    da_cap = cap;
+  }
+  return d_da;
+}
+
+// Pinned staging, free to write: waits for the last copy that read it.
+double *Ensemble::da_host(size_t n) {
+  if (!da_done) ADM_HIP(hipEventCreateWithFlags(&da_done, hipEventDisableTiming));
+  ADM_HIP(hipEventSynchronize(da_done));
+  if (n > h_da_cap) {
+    if (h_da) ADM_HIP(hipHostFree(h_da));
+    h_da = nullptr;
+    const size_t cap = std::max(n, h_da_cap * 2);
+    ADM_HIP(hipHostMalloc(&h_da, sizeof(double) * cap, hipHostMallocDefault));
+    h_da_cap = cap;
+  }
+  return h_da;
 }
 
 void Ensemble::set_time(double t) {

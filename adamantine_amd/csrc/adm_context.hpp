@@ -206,6 +206,16 @@ struct Ensemble {
   double pmod[ADM_MAX_BEAMS] = {0, 0, 0, 0};
   double den[ADM_MAX_BEAMS] = {1, 1, 1, 1};
   StepBuffers sb;
+  // Analysis workspace, grown on demand and kept: device doubles and pinned
+  // host staging. da_done is recorded after the last copy that reads the
+  // staging, so it is rewritten only once that copy has finished.
+  double *d_da = nullptr;
+  size_t da_cap = 0;
+  double *h_da = nullptr;
+  size_t h_da_cap = 0;
+  hipEvent_t da_done = nullptr;
+  double *da_device(size_t n_doubles);
+  double *da_host(size_t n_doubles);
 };
 
 }  // namespace adm

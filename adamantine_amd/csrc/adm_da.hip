@@ -316,21 +316,25 @@ static void launch_transform(int NT, double *X, const double *Tm, int N,
   ADM_HIP_DA(hipGetLastError());
 }
 
-// X <- X Tm on the device (Tm host, N x N row-major).
-void ensemble_transform(double *X, const double *Tm_host, int N, int64_t n,
+// X <- X Tm on the device (Tm host, N x N row-major), the padded matrix
+// staged through the ensemble's pinned workspace; no allocation once the
+// workspace has grown to the member count, no stream synchronisation.
+void ensemble_transform(Ensemble &ens, double *X, const double *Tm_host,
                         hipStream_t s) {
+  const int N = ens.n_members;
+  const int64_t n = ens.n_dofs;
   const int NT = (N + 15) / 16;
   const int NP = NT * 16;
-  std::vector<double> Tp((size_t)NP * NP, 0.0);
+  const size_t np2 = (size_t)NP * NP;
+  double *Tp = ens.da_host(np2);
+  std::fill(Tp, Tp + np2, 0.0);
   for (int a = 0; a < N; ++a)
     for (int b = 0; b < N; ++b) Tp[(size_t)a * NP + b] = Tm_host[(size_t)a * N + b];
-  double *d_T = nullptr;
-  ADM_HIP_DA(hipMalloc(&d_T, sizeof(double) * Tp.size()));
-  ADM_HIP_DA(hipMemcpyAsync(d_T, Tp.data(), sizeof(double) * Tp.size(),
+  double *d_T = ens.da_device(np2);
+  ADM_HIP_DA(hipMemcpyAsync(d_T, Tp, sizeof(double) * np2,
                             hipMemcpyHostToDevice, s));
+  ADM_HIP_DA(hipEventRecord(ens.da_done, s));
   launch_transform(NT, X, d_T, N, n, s);
-  ADM_HIP_DA(hipStreamSynchronize(s));
-  ADM_HIP_DA(hipFree(d_T));
 }
 
 void ensemble_analysis(Ensemble &ens, double *T, int n_params, double *params,
@@ -353,21 +357,25 @@ void ensemble_analysis(Ensemble &ens, double *T, int n_params, double *params,
   // H X on the host
   std::vector<double> HX((size_t)N * m);
   {
-    int64_t *d_obs = nullptr;
-    double *d_hx = nullptr;
-    ADM_HIP_DA(hipMalloc(&d_obs, sizeof(int64_t) * m));
-    ADM_HIP_DA(hipMalloc(&d_hx, sizeof(double) * HX.size()));
-    ADM_HIP_DA(hipMemcpyAsync(d_obs, obs_dof, sizeof(int64_t) * m,
+    // workspace: m observation indices (as doubles' storage), then H X
+    const size_t n_ws = (size_t)m + (size_t)N * m;
+    double *ws = ens.da_device(n_ws);
+    double *stage = ens.da_host(n_ws);
+    int64_t *d_obs = reinterpret_cast<int64_t *>(ws);
+    double *d_hx = ws + m;
+    std::memcpy(stage, obs_dof, sizeof(int64_t) * m);
+    ADM_HIP_DA(hipMemcpyAsync(d_obs, stage, sizeof(int64_t) * m,
                               hipMemcpyHostToDevice, s));
     const int64_t tot = (int64_t)m * N;
     hipLaunchKernelGGL(gather_obs_kernel, dim3((unsigned)((tot + 255) / 256)),
                        dim3(256), 0, s, T, d_obs, n, m, N, d_hx);
     ADM_HIP_DA(hipGetLastError());
-    ADM_HIP_DA(hipMemcpyAsync(HX.data(), d_hx, sizeof(double) * HX.size(),
+    ADM_HIP_DA(hipMemcpyAsync(stage + m, d_hx, sizeof(double) * HX.size(),
                               hipMemcpyDeviceToHost, s));
-    ADM_HIP_DA(hipStreamSynchronize(s));
-    ADM_HIP_DA(hipFree(d_obs));
-    ADM_HIP_DA(hipFree(d_hx));
+    ADM_HIP_DA(hipEventRecord(ens.da_done, s));
+    // the host solve needs H X: wait for this copy only
+    ADM_HIP_DA(hipEventSynchronize(ens.da_done));
+    std::memcpy(HX.data(), stage + m, sizeof(double) * HX.size());
   }
   // anomalies at the observations (the mean as calc_sample_covariance_sparse
   // forms it: sum in member order, then / N) and the perturbed innovations
@@ -478,7 +486,7 @@ void ensemble_analysis(Ensemble &ens, double *T, int n_params, double *params,
       for (int l = 0; l < N; ++l)
         Tm[(size_t)l * N + j] = (l == j ? 1.0 : 0.0) + (W0[(size_t)l * N + j] - cs);
     }
-    ensemble_transform(T, Tm.data(), N, n, s);
+    ensemble_transform(ens, T, Tm.data(), s);
     return;
   }
   if (N > 64)
@@ -488,16 +496,17 @@ void ensemble_analysis(Ensemble &ens, double *T, int n_params, double *params,
   std::vector<double> Ek((size_t)m * N);
   for (int k = 0; k < m; ++k)
     for (int j = 0; j < N; ++j) Ek[(size_t)k * N + j] = E[(size_t)j * m + k];
-  double *d_S = nullptr, *d_E = nullptr, *d_oc = nullptr;
-  ADM_HIP_DA(hipMalloc(&d_S, sizeof(double) * S.size()));
-  ADM_HIP_DA(hipMalloc(&d_E, sizeof(double) * Ek.size()));
-  ADM_HIP_DA(hipMalloc(&d_oc, sizeof(double) * ocoord.size()));
-  ADM_HIP_DA(hipMemcpyAsync(d_S, S.data(), sizeof(double) * S.size(),
+  const size_t n_ws = S.size() + Ek.size() + ocoord.size();
+  double *d_S = ens.da_device(n_ws);
+  double *d_E = d_S + S.size();
+  double *d_oc = d_E + Ek.size();
+  double *stage = ens.da_host(n_ws);
+  std::copy(S.begin(), S.end(), stage);
+  std::copy(Ek.begin(), Ek.end(), stage + S.size());
+  std::copy(ocoord.begin(), ocoord.end(), stage + S.size() + Ek.size());
+  ADM_HIP_DA(hipMemcpyAsync(d_S, stage, sizeof(double) * n_ws,
                             hipMemcpyHostToDevice, s));
-  ADM_HIP_DA(hipMemcpyAsync(d_E, Ek.data(), sizeof(double) * Ek.size(),
-                            hipMemcpyHostToDevice, s));
-  ADM_HIP_DA(hipMemcpyAsync(d_oc, ocoord.data(), sizeof(double) * ocoord.size(),
-                            hipMemcpyHostToDevice, s));
+  ADM_HIP_DA(hipEventRecord(ens.da_done, s));
   const unsigned grid = (unsigned)((n + 255) / 256);
   if (N <= 16)
     hipLaunchKernelGGL(localized_update_kernel<16>, dim3(grid), dim3(256), 0, s,
@@ -509,10 +518,6 @@ void ensemble_analysis(Ensemble &ens, double *T, int n_params, double *params,
     hipLaunchKernelGGL(localized_update_kernel<64>, dim3(grid), dim3(256), 0, s,
                        T, N, n, d_S, d_E, d_oc, m, fn, cutoff, geo);
   ADM_HIP_DA(hipGetLastError());
-  ADM_HIP_DA(hipStreamSynchronize(s));
-  ADM_HIP_DA(hipFree(d_S));
-  ADM_HIP_DA(hipFree(d_E));
-  ADM_HIP_DA(hipFree(d_oc));
 }
 
 // T[idx[k]] = v[k] (distinct indices)
@@ -761,8 +766,7 @@ int adm_ensemble_transform(void *ens, double *T, const double *Tm,
   ADM_DA_BEGIN
   if (!ens || !T || !Tm) return adm::set_error(ADM_ERR_INVALID, "null argument");
   adm::Ensemble &e = *static_cast<adm::Ensemble *>(ens);
-  adm::ensemble_transform(T, Tm, e.n_members, e.n_dofs,
-                          static_cast<hipStream_t>(stream));
+  adm::ensemble_transform(e, T, Tm, static_cast<hipStream_t>(stream));
   ADM_DA_END
 }
 

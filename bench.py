@@ -49,12 +49,22 @@ def algorithmic_bytes(problem, n_powder_qp: int, n_active_cells: int) -> int:
             problem.n_cells_total)
 
 
-def cpu_baseline(problem, seconds: float = 12.0):
-    """The CPU oracle (C restatement, OpenMP) on the host cores, timed on
-    whole operator applications of the same workload."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def time_cpu_vmults(problem, threads: int, seconds: float):
+    """Whole operator applications of the CPU oracle on `threads` host
+    threads for about `seconds`: (applications, elapsed s)."""
     from oracle import oracle as orc
     from adamantine_amd.workloads import random_temperature
-    threads = os.cpu_count() or 1
     orc.set_threads(threads)
     o = orc.Oracle(problem)
     T = random_temperature(problem, low=300.0, high=1600.0)
@@ -67,9 +77,24 @@ def cpu_baseline(problem, seconds: float = 12.0):
         n += 1
         el = time.perf_counter() - t0
         if el > seconds:
-            break
+            return n, el
+
+
+def cpu_baseline(problem, seconds: float = 12.0):
+    """The CPU oracle (C restatement, OpenMP) on the host cores, timed on
+    whole operator applications of the same workload, once on every
+    thread the process may use and once on one core."""
+    threads = len(os.sched_getaffinity(0)) if hasattr(
+        os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = max(1, min(threads, int(os.environ.get("OMP_NUM_THREADS",
+                                                     threads))))
+    n, el = time_cpu_vmults(problem, threads, seconds)
+    n1, el1 = time_cpu_vmults(problem, 1, seconds / 2)
     return {"value": problem.n_dofs * n / el, "unit": "DoF/s",
-            "cores": threads, "kind": "port",
+            "cores": threads, "kind": "port", "cpu_model": cpu_model(),
+            "one_core": {"value": problem.n_dofs * n1 / el1, "unit": "DoF/s",
+                         "cores": 1, "sample": "%d full vmults, %.1f s"
+                                               % (n1, el1)},
             "sample": "%d full vmults of the 2,154,945-node Q2 workload on "
                       "%d host threads (%.1f s)" % (n, threads, el)}
 
@@ -122,6 +147,9 @@ def ensemble_bench(n_members: int, n_steps: int):
     e1.record(stream)
     torch.cuda.synchronize()
     batched_ms = e0.elapsed_time(e1) / n_steps
+    analysis = analysis_bench(ens, sol, base, members, stream)
+    analysis["cycle_ms"] = 50 * batched_ms + analysis["analysis_ms"]
+    analysis["cycle"] = "50 batched RK4 steps + 1 analysis"
     del ens, sol
     # serial loop: every member's own solver, one step each, member by member
     n_loop = min(n_members, 16)
@@ -156,6 +184,7 @@ def ensemble_bench(n_members: int, n_steps: int):
     loop = 1.0 / (loop_ms * 1e-3)
     return {"members": n_members, "n_dofs_per_member": base.n_dofs,
             "method": "rk4", "batched_step_ms": batched_ms,
+            "enkf": analysis,
             "member_steps_per_s": batched,
             "loop_member_steps_per_s": loop,
             "speedup_vs_loop": batched / loop,
@@ -168,6 +197,69 @@ def ensemble_bench(n_members: int, n_steps: int):
             "loop_launch": {"mode": LAUNCH_MODE_NAMES[loop_launch["mode"]],
                             "waves_per_block":
                                 loop_launch["waves_per_block"]}}
+
+
+def analysis_bench(ens, sol, base, members, stream, reps: int = 5):
+    """The EnKF analysis of the ensemble workload: 500 observations on the
+    scanned top surface, R = 4 I, absorption efficiency and conductivity
+    scale as augmented parameters, no localisation (update_ensemble,
+    source/DataAssimilator.cc:95-189), timed whole (host solve included);
+    and the member transform X <- X Tm by itself (ensemble_transform_kernel,
+    2 N^2 n flops over 2 N n 8 bytes: 8 flop/B at N = 64), with HIP events
+    on the stream it runs on."""
+    import numpy as np
+    import torch
+    from adamantine_amd.ensemble import DataAssimilator, ensemble_transform
+    N, n = ens.n_members, ens.n_dofs
+    rng = np.random.RandomState(5)
+    zt = base.node_coordinates()[:, 2].max() * 0.75
+    pts = np.stack([rng.uniform(0.2e-3, 3.8e-3, 500),
+                    rng.uniform(0.7e-3, 1.3e-3, 500), np.full(500, zt)],
+                   axis=1)
+    da = DataAssimilator()
+    obs = da.update_dof_mapping(ens, pts)
+    X = sol.clone()
+    y = X[:, torch.tensor(obs, device=X.device)].mean(dim=0).cpu().numpy()
+    y = y + rng.normal(scale=2.0, size=obs.size)
+    R = 4.0 * np.eye(obs.size)
+    params0 = np.ascontiguousarray(np.stack(
+        [[m.absorption_efficiency[0] for m in members],
+         [m.conductivity_scale for m in members]], axis=1))
+    noise = da.fill_noise_vectors(N, R)
+    times = []
+    for _ in range(reps + 1):
+        X.copy_(sol)
+        p = params0.copy()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        da.update_ensemble(ens, X, y, R, params=p, noise=noise, stream=stream)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    analysis_ms = 1e3 * float(np.median(times[1:]))
+    Tm = np.eye(N) + 1e-3 * rng.standard_normal((N, N))
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    ensemble_transform(ens, X, Tm, stream=stream)
+    torch.cuda.synchronize()
+    k = 20
+    e0.record(stream)
+    for _ in range(k):
+        ensemble_transform(ens, X, Tm, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    tr_ms = e0.elapsed_time(e1) / k
+    flops = 2.0 * N * N * n
+    bytes_ = 2.0 * N * n * 8
+    return {"observations": int(obs.size), "augmented_parameters": 2,
+            "analysis_ms": analysis_ms,
+            "transform_ms": tr_ms,
+            "transform_TFLOPs": flops / (tr_ms * 1e-3) / 1e12,
+            "transform_GBs": bytes_ / (tr_ms * 1e-3) / 1e9,
+            "transform_flop_per_byte": flops / bytes_,
+            "transform_frac_fp64_mfma": flops / (tr_ms * 1e-3) / 1e12 /
+            FP64_PEAK_TFLOPS,
+            "transform_frac_hbm": bytes_ / (tr_ms * 1e-3) / 1e9 /
+            HBM_PEAK_GBS}
 
 
 def adjoint_bench():

@@ -581,8 +581,8 @@ int adm_ensemble_transform(void *ens, double *T, const double *Tm,
 /* ------------------------------------------------------------------------ */
 
 /* No reference counterpart (the reference estimates parameters only through
- * the EnKF). Runs n_steps of `method` (ADM_FORWARD_EULER or
- * ADM_RK_FOURTH_ORDER) from T0 at t0, exactly as adm_evolve does, and the
+ * the EnKF). Runs n_steps of `method` (ADM_FORWARD_EULER, ADM_RK_THIRD_ORDER
+ * or ADM_RK_FOURTH_ORDER) from T0 at t0, exactly as adm_evolve does, and the
  * transposed stage sequence backwards from checkpoints taken every
  * checkpoint_every steps (<= 0: one checkpoint), for the misfit
  *   J = 1/2 sum_k w_k (T(t_end)[obs_dof_k] - obs_k)^2.
