@@ -58,10 +58,11 @@ def build_library(verbose: bool = True, jobs: int = 0) -> str:
     jobs_list.append(([hipcc] + HIPCC_FLAGS + inc + ["-c", "-o", da_obj,
                       os.path.join(CSRC, "adm_da.hip")], verbose))
     objs.append(da_obj)
-    adj_obj = os.path.join(objdir, "adm_adjoint.o")
-    jobs_list.append(([hipcc] + HIPCC_FLAGS + inc + ["-c", "-o", adj_obj,
-                      os.path.join(CSRC, "adm_adjoint.hip")], verbose))
-    objs.append(adj_obj)
+    for unit in ("adm_adjoint", "adm_activation"):
+        obj = os.path.join(objdir, unit + ".o")
+        jobs_list.append(([hipcc] + HIPCC_FLAGS + inc + ["-c", "-o", obj,
+                          os.path.join(CSRC, unit + ".hip")], verbose))
+        objs.append(obj)
     for (p, ns, t) in CELL_CONFIGS:
         obj = os.path.join(objdir, "adm_cell_p%d_s%d_t%d.o" % (p, ns, t))
         jobs_list.append(([hipcc] + HIPCC_FLAGS + inc + [

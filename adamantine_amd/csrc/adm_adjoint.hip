@@ -775,7 +775,7 @@ struct AdjointRun {
       : c(c_), s(s_), method(method_) {
     n = c.n_dofs;
     npw = c.n_powder_cells * c.nq;
-    nfpw = (c.n_states == 3) ? (int64_t)c.faces.size() * c.nfq : 0;
+    nfpw = (c.n_states == 3) ? c.n_faces * c.nfq : 0;
     std::memset(a, 0, sizeof(a));
     if (method == ADM_FORWARD_EULER) {
       n_stages = 1;

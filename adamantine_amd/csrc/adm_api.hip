@@ -762,6 +762,7 @@ void Context::rebuild_cells() {
     w |= (uint32_t)material[c] << kMaterialShift;
     if (active[c]) {
       const uint32_t fm = face_mask(c);
+      w |= fm << kFaceShift;
       for (int f = 0; f < 6; ++f) {
         if (!(fm & (1u << f))) continue;
         const int cc[3] = {(int)(c % nc[0]), (int)((c / nc[0]) % nc[1]),
@@ -801,12 +802,18 @@ void Context::rebuild_cells() {
   d_faces = nullptr;
   if (n_powder_cells > 0)
     ADM_HIP(hipMalloc(&d_powder, sizeof(double) * n_powder_cells * nq));
+  n_faces = (int64_t)faces.size();
+  faces_stale = false;
+  face_cap = face_powder_cap = 0;
   if (!faces.empty()) {
     ADM_HIP(hipMalloc(&d_faces, sizeof(FaceRef) * faces.size()));
     ADM_HIP(hipMemcpy(d_faces, faces.data(), sizeof(FaceRef) * faces.size(),
                       hipMemcpyHostToDevice));
-    if (n_states == 3)
+    face_cap = faces.size();
+    if (n_states == 3) {
       ADM_HIP(hipMalloc(&d_face_powder, sizeof(double) * faces.size() * nfq));
+      face_powder_cap = faces.size() * nfq;
+    }
   }
   build_face_nodes();
 }
@@ -824,6 +831,7 @@ void Context::build_face_nodes() {
   d_fn_offset = nullptr;
   d_fn_slot = nullptr;
   n_fnodes = 0;
+  face_out_cap = fn_node_cap = fn_offset_cap = fn_slot_cap = 0;
   if (faces.empty()) return;
   const int N = p + 1, NF = N * N;
   const int64_t ns[3] = {1, (int64_t)nn[0], (int64_t)nn[0] * nn[1]};
@@ -863,6 +871,10 @@ void Context::build_face_nodes() {
   ADM_HIP(hipMalloc(&d_fn_node, sizeof(int64_t) * node.size()));
   ADM_HIP(hipMalloc(&d_fn_offset, sizeof(int32_t) * off.size()));
   ADM_HIP(hipMalloc(&d_fn_slot, sizeof(int32_t) * slot.size()));
+  face_out_cap = faces.size() * NF;
+  fn_node_cap = node.size();
+  fn_offset_cap = off.size();
+  fn_slot_cap = slot.size();
   ADM_HIP(hipMemcpy(d_fn_node, node.data(), sizeof(int64_t) * node.size(),
                     hipMemcpyHostToDevice));
   ADM_HIP(hipMemcpy(d_fn_offset, off.data(), sizeof(int32_t) * off.size(),
@@ -910,6 +922,7 @@ void Context::set_cell_state(const double *cs) {
     ADM_HIP(hipMemcpy(d_powder, pw.data(), sizeof(double) * pw.size(),
                       hipMemcpyHostToDevice));
   }
+  sync_faces_host();
   if (n_states == 3 && !faces.empty()) {
     std::vector<double> fpw(faces.size() * nfq);
     for (size_t f = 0; f < faces.size(); ++f)
@@ -1049,7 +1062,35 @@ void Context::cells_to_activate(int n_boxes, const double *boxes,
 void Context::add_material(const int32_t *box_of_cell, const double *box_cos,
                            const double *box_sin, double new_temperature,
                            double *T, hipStream_t s) {
+  if (device_activation_fits()) {
+    // boxes named by inactive cells (the others' entries are ignored)
+    int nb = 0;
+    std::vector<int32_t> boc(box_of_cell, box_of_cell + n_cells);
+    for (int64_t c = 0; c < n_cells; ++c) {
+      if (active[c]) boc[c] = -1;
+      nb = std::max(nb, boc[c] + 1);
+    }
+    box_of_cell = boc.data();
+    activate(nb, nullptr, box_of_cell, box_cos, box_sin, new_temperature, T,
+             nullptr, s);
+  } else {
+    add_material_host(box_of_cell, box_cos, box_sin, new_temperature, T, s);
+  }
+}
+
+// the device activation's counters and scans are 32-bit
+bool Context::device_activation_fits() const {
+  const int64_t NF = (int64_t)(p + 1) * (p + 1);
+  return n_cells * 6 * NF < (int64_t)INT32_MAX && n_dofs < (int64_t)INT32_MAX;
+}
+
+// The same on the host (meshes too large for the device activation).
+void Context::add_material_host(const int32_t *box_of_cell,
+                                const double *box_cos, const double *box_sin,
+                                double new_temperature, double *T,
+                                hipStream_t s) {
   ADM_HIP(hipStreamSynchronize(s));
+  sync_faces_host();
   std::vector<double> cs((size_t)n_cells * n_states);
   cell_state_means(cs.data());
   const std::vector<uint8_t> before = node_activity();
@@ -1127,7 +1168,7 @@ FaceArgs Context::face_args() const {
     a.nc[k] = nc[k];
     a.nn[k] = nn[k];
   }
-  a.n_faces = (int64_t)faces.size();
+  a.n_faces = n_faces;
   a.face_out = d_face_out;
   a.fn_node = d_fn_node;
   a.fn_offset = d_fn_offset;
@@ -1571,6 +1612,8 @@ Ensemble::Ensemble(const adm_problem_desc &base, int n,
     members.emplace_back(new Context(d));
   }
   n_dofs = members[0]->n_dofs;
+  auto work = std::make_shared<ActWork>();
+  for (auto &m : members) m->act = work;
   refresh_members();
 }
 
@@ -1600,7 +1643,7 @@ void Ensemble::refresh_members() {
     m.dep_sin = a.dep_sin;
     m.kc = a.kc;
     m.faces = c.d_faces;
-    m.n_faces = (int64_t)c.faces.size();
+    m.n_faces = c.n_faces;
     max_faces = std::max(max_faces, m.n_faces);
     m.face_out = c.d_face_out;
     m.fn_node = c.d_fn_node;
@@ -1901,7 +1944,7 @@ static void tune_launch(adm::Context &cfg,
     keep(h->d_liquid, sizeof(double) * h->n_cells * h->nq);
     keep(h->d_liquid_dirty, (size_t)h->n_cells);
     keep(h->d_powder, sizeof(double) * h->n_powder_cells * h->nq);
-    keep(h->d_face_powder, sizeof(double) * h->faces.size() * h->nfq);
+    keep(h->d_face_powder, sizeof(double) * h->n_faces * h->nfq);
   }
   auto restore = [&]() {
     for (const Saved &v : saved)
@@ -2047,6 +2090,7 @@ int adm_get_face_state(void *ptr, double *face_powder) {
   ADM_API_BEGIN
   ADM_CTX(ptr)
   if (!face_powder) return adm::set_error(ADM_ERR_INVALID, "null output");
+  ctx.sync_faces_host();
   const int nf = ctx.nfq;
   std::fill(face_powder, face_powder + (size_t)ctx.n_cells * 6 * nf, 0.0);
   if (ctx.d_face_powder && !ctx.faces.empty()) {
@@ -2068,6 +2112,7 @@ int adm_set_face_state(void *ptr, const double *face_powder) {
   ADM_API_BEGIN
   ADM_CTX(ptr)
   if (!face_powder) return adm::set_error(ADM_ERR_INVALID, "null input");
+  ctx.sync_faces_host();
   const int nf = ctx.nfq;
   if (ctx.d_face_powder && !ctx.faces.empty()) {
     std::vector<double> fpw(ctx.faces.size() * nf);
@@ -2342,6 +2387,26 @@ int adm_add_material(void *ptr, const int32_t *box_of_cell,
   ADM_API_END
 }
 
+int adm_activate(void *ptr, int32_t n_boxes, const double *boxes,
+                 const double *box_cos, const double *box_sin,
+                 double new_temperature, double *T, int32_t *box_of_cell,
+                 void *stream) {
+  ADM_API_BEGIN
+  ADM_CTX(ptr)
+  if (n_boxes < 0 || (n_boxes && (!boxes || !box_cos || !box_sin)))
+    return adm::set_error(ADM_ERR_INVALID, "bad box list");
+  if (ctx.n_ext >= 0)
+    return adm::set_error(ADM_ERR_UNSUPPORTED,
+                          "activation changes the DoFs: clear the DoF map, "
+                          "activate on the box-node vector, set the new map");
+  if (!ctx.device_activation_fits())
+    return adm::set_error(ADM_ERR_UNSUPPORTED,
+                          "mesh too large for the device activation");
+  ctx.activate(n_boxes, boxes, nullptr, box_cos, box_sin, new_temperature, T,
+               box_of_cell, static_cast<hipStream_t>(stream));
+  ADM_API_END
+}
+
 int adm_dofs_to_nodes(void *ptr, const double *ext, double *node,
                       void *stream) {
   ADM_API_BEGIN
@@ -2391,10 +2456,31 @@ int adm_ensemble_add_material(void *ptr, int32_t member,
   c.add_material(box_of_cell, box_cos, box_sin, new_temperature,
                  T ? T + (int64_t)member * ens.n_dofs : nullptr, s);
   ens.refresh_members();
-  if (ens.sb.inv_mass) {
+  if (ens.sb.inv_mass)
     c.compute_inverse_mass(ens.sb.inv_mass + (int64_t)member * ens.n_dofs, s);
-    ADM_HIP(hipStreamSynchronize(s));
-  }
+  ADM_API_END
+}
+
+int adm_ensemble_activate(void *ptr, int32_t member, int32_t n_boxes,
+                          const double *boxes, const double *box_cos,
+                          const double *box_sin, double new_temperature,
+                          double *T, int32_t *box_of_cell, void *stream) {
+  ADM_API_BEGIN
+  ADM_ENS(ptr)
+  if (member < 0 || member >= ens.n_members)
+    return adm::set_error(ADM_ERR_INVALID, "member index out of range");
+  if (n_boxes < 0 || (n_boxes && (!boxes || !box_cos || !box_sin)))
+    return adm::set_error(ADM_ERR_INVALID, "bad box list");
+  adm::Context &c = *ens.members[member];
+  if (!c.device_activation_fits())
+    return adm::set_error(ADM_ERR_UNSUPPORTED,
+                          "mesh too large for the device activation");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  c.activate(n_boxes, boxes, nullptr, box_cos, box_sin, new_temperature,
+             T ? T + (int64_t)member * ens.n_dofs : nullptr, box_of_cell, s);
+  ens.refresh_members();
+  if (ens.sb.inv_mass)
+    c.compute_inverse_mass(ens.sb.inv_mass + (int64_t)member * ens.n_dofs, s);
   ADM_API_END
 }
 

@@ -48,6 +48,28 @@ struct StepBuffers {
   void ensure(int method, int64_t n_);
 };
 
+// Device workspace of the material activation, allocated at the first one
+// and kept (cells x 20 bytes + nodes x 16 bytes + scan storage).
+struct ActWork {
+  int32_t *boc = nullptr;          // box of each cell
+  double *boxes = nullptr;         // 8 per box: bounds, cos, sin
+  size_t boxes_cap = 0;
+  int32_t *count = nullptr, *start = nullptr;  // faces per cell, scanned
+  double *cell_powder = nullptr;   // cell means of the powder ratio
+  uint64_t *packed = nullptr, *scan = nullptr;  // face slots per node
+  void *temp = nullptr;
+  size_t temp_bytes = 0;
+  int64_t *totals = nullptr;
+  int64_t *h_totals = nullptr;     // pinned
+  int32_t *h_boc = nullptr;        // pinned
+  double *h_boxes = nullptr;       // pinned staging of the boxes
+  size_t h_boxes_cap = 0;
+  ActWork() = default;
+  ActWork(const ActWork &) = delete;
+  ActWork &operator=(const ActWork &) = delete;
+  ~ActWork();
+};
+
 struct Context {
   explicit Context(const adm_problem_desc &d);
   ~Context();
@@ -69,6 +91,18 @@ struct Context {
   void add_material(const int32_t *box_of_cell, const double *box_cos,
                     const double *box_sin, double new_temperature, double *T,
                     hipStream_t s);
+  void add_material_host(const int32_t *box_of_cell, const double *box_cos,
+                         const double *box_sin, double new_temperature,
+                         double *T, hipStream_t s);
+  // the device activation (cells to activate from boxes when box_of_cell
+  // is NULL) and whether this mesh fits its 32-bit counters
+  void activate(int n_boxes, const double *boxes, const int32_t *box_of_cell,
+                const double *box_cos, const double *box_sin,
+                double new_temperature, double *T, int32_t *box_of_cell_out,
+                hipStream_t s);
+  bool device_activation_fits() const;
+  // faces <- d_faces after a device activation
+  void sync_faces_host();
   CellArgs cell_args() const;
   FaceArgs face_args() const;
   void build_merged_tables();
@@ -113,7 +147,17 @@ struct Context {
   int64_t n_powder_cells = 0;
   // 1 + the highest cell layer (z index) holding an active cell, 0 if none
   int nz_active = 0;
+  // boundary-term faces: d_faces holds n_faces of them; the host copy
+  // `faces` is behind it after a device activation until sync_faces_host
   std::vector<FaceRef> faces;
+  int64_t n_faces = 0;
+  bool faces_stale = false;
+  // allocated lengths of the face and face-node lists
+  size_t face_cap = 0, face_out_cap = 0, face_powder_cap = 0;
+  size_t fn_node_cap = 0, fn_offset_cap = 0, fn_slot_cap = 0;
+  // activation workspace (the members of an ensemble share one; their
+  // activations run one after the other)
+  std::shared_ptr<ActWork> act;
   // the faces' nodes: node list, per-node offsets into the slot list, and
   // the slots f * (p+1)^2 + j of face_out (FaceArgs)
   int64_t n_fnodes = 0;

@@ -30,6 +30,8 @@ constexpr uint32_t kMaterialShift = 1;   // 2 bits
 constexpr uint32_t kFaceShift = 3;       // 6 bits: faces with a boundary term
 constexpr uint32_t kHasPowder = 1u << 9;
 constexpr uint32_t kRotated = 1u << 10;
+// set by the device activation on the cells it activated, until the next one
+constexpr uint32_t kNewlyActive = 1u << 11;
 
 // per-(material, property, state) table kind
 constexpr uint8_t kTableGeneral = 0;

@@ -84,6 +84,8 @@ def _bind(L):
     _ip32 = C.POINTER(C.c_int32)
     L.adm_ensemble_cells_to_activate.argtypes = [_vp, C.c_int32, C.c_int32,
                                                  _dp, _ip32]
+    L.adm_ensemble_activate.argtypes = [_vp, C.c_int32, C.c_int32, _dp, _dp,
+                                        _dp, C.c_double, _vp, _ip32, _vp]
     L.adm_ensemble_add_material.argtypes = [_vp, C.c_int32, _ip32, _dp, _dp,
                                             C.c_double, _vp, _vp]
     L.adm_ensemble_set_beam_gain.argtypes = [_vp, C.c_int32, C.c_int32,
@@ -311,6 +313,28 @@ class EnsembleDevice:
             None if solution is None else _dev_ptr(solution),
             _stream_ptr(stream)))
         self._inverse_mass = None
+
+    def activate(self, member: int, boxes, box_cos, box_sin,
+                 new_material_temperature: float, solution,
+                 stream=None) -> np.ndarray:
+        """elements_to_activate + add_material for one member in one call,
+        on the device; returns the box that activated each cell (or -1)."""
+        b = np.ascontiguousarray(boxes, dtype=np.float64).reshape(-1, 6)
+        c = np.ascontiguousarray(box_cos, dtype=np.float64)
+        s_ = np.ascontiguousarray(box_sin, dtype=np.float64)
+        if c.size != b.shape[0] or s_.size != b.shape[0]:
+            raise ValueError("one cos and one sin per box")
+        if solution is not None:
+            self._check_vec(solution)
+        out = np.zeros(self.n_cells, dtype=np.int32)
+        check(_lib().adm_ensemble_activate(
+            self._h, int(member), b.shape[0], b.ctypes.data_as(_dp),
+            c.ctypes.data_as(_dp), s_.ctypes.data_as(_dp),
+            float(new_material_temperature),
+            None if solution is None else _dev_ptr(solution),
+            out.ctypes.data_as(C.POINTER(C.c_int32)), _stream_ptr(stream)))
+        self._inverse_mass = None
+        return out
 
     def set_beam_parameters(self, member: int, beam: int,
                             absorption_efficiency: float, max_power: float):

@@ -215,9 +215,8 @@ def run(db: Dict, base_dir: str = ".", fe_degree: Optional[int] = None,
                                       duration)
             end = bisect_left(dep_t, activation_time_end)
             if start < end:
-                b = ph.elements_to_activate(boxes[start:end])
-                ph.add_material(b, dep_cos[start:end], dep_sin[start:end],
-                                new_temperature, T)
+                b = ph.activate(boxes[start:end], dep_cos[start:end],
+                                dep_sin[start:end], new_temperature, T)
                 out.n_activations += 1
                 if verbose:
                     print("t = %g: boxes %d..%d, %d cells" %
@@ -481,8 +480,7 @@ def run_ensemble(db: Dict, base_dir: str = ".",
             end = bisect_left(dep_t, activation_time_end)
             if start < end:
                 for j in range(n):
-                    b = ens.elements_to_activate(j, boxes[start:end])
-                    ens.add_material(j, b, dep_cos[start:end],
+                    b = ens.activate(j, boxes[start:end], dep_cos[start:end],
                                      dep_sin[start:end], new_temperature, sol)
                     new = b >= 0
                     active[j][new] = 1

@@ -90,6 +90,8 @@ def load_library():
     _ip32 = C.POINTER(C.c_int32)
     L.adm_cells_to_activate.argtypes = [_vp, C.c_int32, _dp, _ip32]
     L.adm_add_material.argtypes = [_vp, _ip32, _dp, _dp, C.c_double, _vp, _vp]
+    L.adm_activate.argtypes = [_vp, C.c_int32, _dp, _dp, _dp, C.c_double, _vp,
+                               _ip32, _vp]
     L.adm_adjoint_gradient.argtypes = [
         _vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, _vp, C.c_int32,
         C.POINTER(C.c_int64), _dp, _dp, _vp, _dp, _dp, _vp, _vp]
@@ -385,6 +387,27 @@ class ThermalPhysicsDevice:
             _stream_ptr(stream)))
         # compute_inverse_mass_matrix (source/ThermalPhysics.templates.hh:741)
         self.operator._inverse_mass = None
+
+    def activate(self, boxes, box_cos, box_sin,
+                 new_material_temperature: float, solution,
+                 stream=None) -> np.ndarray:
+        """elements_to_activate + add_material in one call, on the device
+        (adm_activate); returns the box that activated each cell (or -1)."""
+        b = np.ascontiguousarray(boxes, dtype=np.float64).reshape(-1, 6)
+        c = _host(box_cos)
+        s_ = _host(box_sin)
+        if c.size != b.shape[0] or s_.size != b.shape[0]:
+            raise ValueError("one cos and one sin per box")
+        out = np.zeros(self.operator.n_cells, dtype=np.int32)
+        check(load_library().adm_activate(
+            self.operator.handle, b.shape[0], b.ctypes.data_as(_dp),
+            c.ctypes.data_as(_dp), s_.ctypes.data_as(_dp),
+            float(new_material_temperature),
+            None if solution is None else _dev_ptr(solution,
+                                                   self.operator.m()),
+            out.ctypes.data_as(C.POINTER(C.c_int32)), _stream_ptr(stream)))
+        self.operator._inverse_mass = None
+        return out
 
     def set_state_to_material_properties(self):
         return self.operator.set_state_to_material_properties()

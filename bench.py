@@ -314,7 +314,8 @@ def deposition_bench(n_members: int = 32, start: int = 180, n: int = 125):
     """The deposition workload (32 members of the 96x96x40 Q2 box, forward
     Euler 2e-7 s): steps start .. start+n-1, in which every member deposits
     its second layer (at step 200 + U{0..100}), timed with the activations
-    (cell query on the host, state transfer and inverse mass on the device)."""
+    (cell query, state transfer, face lists and inverse mass on the device:
+    adm_ensemble_activate)."""
     import numpy as np
     import torch
     from adamantine_amd.ensemble import EnsembleDevice
@@ -331,8 +332,8 @@ def deposition_bench(n_members: int = 32, start: int = 180, n: int = 125):
         for j in np.nonzero(w.activation_step[:, 1] == step)[0]:
             torch.cuda.synchronize()  # steps so far are not activation time
             a0 = time.perf_counter()
-            b = ens.elements_to_activate(int(j), w.layer_boxes[1:2])
-            ens.add_material(int(j), b, [1.0], [0.0], 300.0, sol)
+            ens.activate(int(j), w.layer_boxes[1:2], [1.0], [0.0], 300.0,
+                         sol)
             torch.cuda.synchronize()
             act_s += time.perf_counter() - a0
             n_act += 1

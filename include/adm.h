@@ -407,6 +407,20 @@ int adm_add_material(void *ctx, const int32_t *box_of_cell,
                      const double *box_cos, const double *box_sin,
                      double new_temperature, double *T, void *stream);
 
+/* get_elements_to_activate followed by add_material_start + add_material_end
+ * (source/material_deposition.cc:376-435, source/ThermalPhysics.templates.hh:
+ * 595-825) in one call, on the device: the same cells, state transfer,
+ * new-material temperature and mass as adm_cells_to_activate +
+ * adm_add_material, without copying the phase state to the host.
+ * boxes: host, n_boxes x {min x, y, z, max x, y, z}; box_cos, box_sin: host,
+ * n_boxes; T: device, n_dofs (may be NULL); box_of_cell: host out, n_cells
+ * (may be NULL): the box that activated each cell, or -1. Enqueued on
+ * stream; returns after one small device-to-host copy. */
+int adm_activate(void *ctx, int32_t n_boxes, const double *boxes,
+                 const double *box_cos, const double *box_sin,
+                 double new_temperature, double *T, int32_t *box_of_cell,
+                 void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Ensemble                                                                  */
 /* ------------------------------------------------------------------------ */
@@ -506,6 +520,11 @@ int adm_ensemble_add_material(void *ens, int32_t member,
                               const int32_t *box_of_cell,
                               const double *box_cos, const double *box_sin,
                               double new_temperature, double *T, void *stream);
+/* adm_activate for one member (T: the whole ensemble vector, or NULL). */
+int adm_ensemble_activate(void *ens, int32_t member, int32_t n_boxes,
+                          const double *boxes, const double *box_cos,
+                          const double *box_sin, double new_temperature,
+                          double *T, int32_t *box_of_cell, void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Data assimilation (EnKF)                                                  */

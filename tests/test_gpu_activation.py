@@ -90,9 +90,13 @@ def test_ensemble_activation_per_member():
     for step in range(12):
         for j in range(3):
             if step == 4 * j:  # member j deposits at step 4 j
-                b = ens.elements_to_activate(j, boxes)
-                np.testing.assert_array_equal(b, oracles[j].cells_to_activate(boxes))
-                ens.add_material(j, b, [1.0], [0.0], 320.0, sol)
+                want_b = oracles[j].cells_to_activate(boxes)
+                if j == 1:  # the fused device call
+                    b = ens.activate(j, boxes, [1.0], [0.0], 320.0, sol)
+                else:
+                    b = ens.elements_to_activate(j, boxes)
+                    ens.add_material(j, b, [1.0], [0.0], 320.0, sol)
+                np.testing.assert_array_equal(b, want_b)
                 oracles[j].add_material(b, [1.0], [0.0], 320.0, Tref[j])
         t_new = ens.evolve(t, dt, 1, sol)
         for j in range(3):
@@ -104,3 +108,93 @@ def test_ensemble_activation_per_member():
         assert rel(got[j], Tref[j]) < STEP_TOL, j
     np.testing.assert_array_equal(
         ens.get_inverse_mass_matrix().cpu().numpy()[2], oracles[2].inverse_mass())
+
+
+def test_activate_partial_layers_and_rebuild():
+    """adm_activate (cells, state transfer, face lists and mass on the
+    device) over two depositions that leave a ragged top (half a layer,
+    then the rest of it and a patch of the next, with rotated deposition
+    directions): cells, new-node temperatures, mass and phase state equal
+    the oracle's, steps agree to 1e-10, and the operator equals, bit for
+    bit, one built from scratch with the same active cells, directions and
+    cell states (face list, face state and face-node lists as the host
+    builds them)."""
+    import dataclasses
+    import torch
+    from adamantine_amd.thermal import ThermalOperatorDevice, ThermalPhysicsDevice
+    p, hz = layered_problem()
+    print("activate: 12x8x8 Q2, ragged layers", flush=True)
+    X, Y = p.extent[0], p.extent[1]
+    ph = ThermalPhysicsDevice(p)
+    ph.setup()
+    o = Oracle(p)
+    rs = np.random.RandomState(11)
+    T0 = 300.0 + 1400.0 * rs.uniform(0.0, 1.0, p.n_dofs)
+    sol = torch.tensor(T0, device="cuda")
+    Tref = T0.copy()
+    dt, n, t, t_ref = 1e-6, 6, 0.0, 0.0
+    depositions = [
+        (np.array([[0.0, 0.0, 5.2 * hz, 0.5 * X, Y, 5.8 * hz]]), [0.3]),
+        (np.array([[0.0, 0.0, 5.2 * hz, X, Y, 5.8 * hz],
+                   [0.3 * X, 0.2 * Y, 6.2 * hz, 0.8 * X, 0.7 * Y, 6.8 * hz]]),
+         [1.1, -0.4]),
+    ]
+    for boxes, ang in depositions:
+        t = ph.evolve(t, dt, n, sol)
+        Tref, t_ref = o.evolve(FORWARD_EULER, t_ref, dt, n, Tref)
+        cs, sn = np.cos(ang), np.sin(ang)
+        want_b = o.cells_to_activate(boxes)
+        before = sol.cpu().numpy()
+        b = ph.activate(boxes, cs, sn, 350.0, sol)
+        o.add_material(want_b, cs, sn, 350.0, Tref)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(b, want_b)
+        assert (b >= 0).sum() > 0
+        after = sol.cpu().numpy()
+        np.testing.assert_array_equal(after != before,
+                                      (Tref == 350.0) & (before != 350.0))
+        assert rel(after, Tref) < STEP_TOL
+        np.testing.assert_array_equal(
+            ph.operator.get_inverse_mass_matrix().cpu().numpy(),
+            o.inverse_mass())
+        lq, pw = ph.operator.qp_state()
+        olq, opw = o.qp_state()
+        np.testing.assert_array_equal(pw, opw)
+        np.testing.assert_array_equal(lq, olq)
+    # the same operator built from scratch, right after the activation
+    active, cos_, sin_ = ph.operator.cell_data()
+    state = ph.set_state_to_material_properties()
+    q = dataclasses.replace(p, cell_active=active.astype(np.uint8),
+                            cell_state=state, deposition_cos=cos_,
+                            deposition_sin=sin_)
+    fresh = ThermalOperatorDevice(q)
+    # the state transfer set every point to its cell mean (the fresh
+    # operator's points hold the mean of those means, which can be an ulp
+    # away), and the faces' points to their cells'
+    lq, pw = ph.operator.qp_state()
+    flq, fpw = fresh.qp_state()
+    fs = ph.operator.face_state()
+    np.testing.assert_allclose(pw, fpw, rtol=1e-15, atol=0)
+    np.testing.assert_allclose(lq, flq, rtol=1e-15, atol=0)
+    np.testing.assert_allclose(fs, fresh.face_state(), rtol=1e-15, atol=0)
+    fresh.set_qp_state(lq, pw)
+    fresh.set_face_state(fs)
+    # one application of each at the oracle's temperature (which moves the
+    # phase state of all three the same way)
+    Tv = torch.tensor(Tref, device="cuda")
+    for op in (ph.operator, fresh):
+        op.set_time(t)
+    o.set_time(t)
+    d1 = ph.operator.initialize_dof_vector()
+    d2 = fresh.initialize_dof_vector()
+    ph.operator.vmult(d1, Tv)
+    fresh.vmult(d2, Tv)
+    want = o.vmult(Tref)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(d1.cpu().numpy(), d2.cpu().numpy())
+    assert rel(d1.cpu().numpy(), want) < 1e-12
+    # and the steps after the activation
+    t = ph.evolve(t, dt, n, sol)
+    Tref, _ = o.evolve(FORWARD_EULER, t_ref, dt, n, Tref)
+    torch.cuda.synchronize()
+    assert rel(sol.cpu().numpy(), Tref) < STEP_TOL

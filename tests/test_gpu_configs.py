@@ -261,8 +261,9 @@ def test_deposition_ensemble_at_size():
     n_act = 0
     for step in range(190, 240):
         for j in np.nonzero(w.activation_step[:, 1] == step)[0]:
-            b = ens.elements_to_activate(int(j), w.layer_boxes[1:2])
-            ens.add_material(int(j), b, [1.0], [0.0], 300.0, sol)
+            b = ens.activate(int(j), w.layer_boxes[1:2], [1.0], [0.0],
+                             300.0, sol)
+            assert (b >= 0).sum() > 0
             n_act += 1
         t = ens.evolve(t, w.dt, 1, sol)
     torch.cuda.synchronize()
