@@ -176,3 +176,79 @@ class _MisfitProxy:
 
 
 _Misfit = _MisfitProxy
+
+
+class ActivationEvent(C.Structure):
+    """adm_activation_event: before step `step` of the window, member
+    `member` activates the cells its boxes meet (adm_activate)."""
+    _fields_ = [("member", C.c_int32), ("step", C.c_int32),
+                ("n_boxes", C.c_int32), ("boxes", _dp), ("box_cos", _dp),
+                ("box_sin", _dp), ("new_temperature", C.c_double)]
+
+
+@dataclass
+class EnsembleAdjointResult:
+    J: np.ndarray            # [n_members]
+    grad: np.ndarray         # [n_members, 3], PARAMETERS order
+    T_final: object          # device [n_members, n_dofs]
+    lambda0: object          # device [n_members, n_dofs]: dJ_j/dT0_j
+
+
+def ensemble_adjoint_gradient(ensemble, t0: float, dt: float, n_steps: int,
+                              T0, obs_dof, obs, obs_weight=None,
+                              events=(), checkpoint_every: int = 0,
+                              method=None,
+                              stream=None) -> EnsembleAdjointResult:
+    """Every member's misfit J_j = 1/2 sum_k w_k (T_j[obs_dof_k] - obs[j,k])^2
+    at the end of n_steps and its gradient, by one batched forward and one
+    batched backward sweep (adm_ensemble_adjoint_gradient). events: (member,
+    step, boxes [k x 6], box_cos [k], box_sin [k], new_temperature) tuples,
+    the activations the forward run makes before the given steps."""
+    import torch
+    L = load_library()
+    from .ensemble import _bind
+    _bind(L)
+    if not getattr(L, "_ens_adj_bound", False):
+        L.adm_ensemble_adjoint_gradient.argtypes = [
+            C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+            C.c_void_p, C.c_int32, C.POINTER(C.c_int64), _dp, _dp, C.c_int32,
+            C.POINTER(ActivationEvent), C.c_void_p, _dp, _dp, C.c_void_p,
+            C.c_void_p]
+        L._ens_adj_bound = True
+    N = ensemble.n_members
+    dof = np.ascontiguousarray(obs_dof, dtype=np.int64)
+    y = np.ascontiguousarray(obs, dtype=np.float64)
+    if dof.ndim != 1 or y.shape != (N, dof.size):
+        raise ValueError("obs_dof: [n_obs]; obs: [n_members, n_obs]")
+    w = None
+    if obs_weight is not None:
+        w = np.ascontiguousarray(obs_weight, dtype=np.float64)
+        if w.shape != dof.shape:
+            raise ValueError("obs_weight must match obs_dof")
+    ensemble._check_vec(T0)
+    keep = []
+    ev = (ActivationEvent * max(len(events), 1))()
+    for i, (member, step, boxes, cs, sn, new_t) in enumerate(events):
+        b = np.ascontiguousarray(boxes, dtype=np.float64).reshape(-1, 6)
+        c = np.ascontiguousarray(cs, dtype=np.float64).reshape(-1)
+        s_ = np.ascontiguousarray(sn, dtype=np.float64).reshape(-1)
+        if c.size != b.shape[0] or s_.size != b.shape[0]:
+            raise ValueError("one cos and one sin per box")
+        keep += [b, c, s_]
+        ev[i] = ActivationEvent(int(member), int(step), b.shape[0],
+                                b.ctypes.data_as(_dp), c.ctypes.data_as(_dp),
+                                s_.ctypes.data_as(_dp), float(new_t))
+    T_final = torch.empty_like(T0)
+    lam = torch.empty_like(T0)
+    J = np.zeros(N)
+    g = np.zeros((N, 3))
+    m = ensemble.method if method is None else method
+    check(L.adm_ensemble_adjoint_gradient(
+        ensemble._h, int(m), float(t0), float(dt), int(n_steps),
+        int(checkpoint_every), _dev_ptr(T0), dof.size,
+        dof.ctypes.data_as(C.POINTER(C.c_int64)), y.ctypes.data_as(_dp),
+        None if w is None else w.ctypes.data_as(_dp), len(events), ev,
+        _dev_ptr(T_final), J.ctypes.data_as(_dp), g.ctypes.data_as(_dp),
+        _dev_ptr(lam), _stream_ptr(stream)))
+    ensemble._inverse_mass = None
+    return EnsembleAdjointResult(J, g, T_final, lam)

@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/adm.h"
@@ -334,6 +335,51 @@ __global__ void node_fill_kernel(ActGeom g, const uint32_t *__restrict__ meta,
   }
 }
 
+// Tbar = 0 on the nodes the last activation added
+__global__ void fresh_zero_kernel(ActGeom g, const uint32_t *__restrict__ meta,
+                                  double *__restrict__ Tbar) {
+  const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= g.n_dofs) return;
+  NodeCells nc;
+  node_cells(g, node, nc);
+  bool before = false, after = false;
+  for (int k = 0; k < nc.n; ++k) {
+    const uint32_t m = meta[nc.cell[k]];
+    if (!(m & kActive)) continue;
+    after = true;
+    if (!(m & kNewlyActive)) before = true;
+  }
+  if (after && !before) Tbar[node] = 0.0;
+}
+
+// pbar of a cell's points <- (sum of them + sum over the cell's faces'
+// points of fpbar) / nq: the adjoint of p_q = mean and face p = mean.
+// Faces are in increasing cell order; a binary search finds the cell's.
+__global__ void transfer_adjoint_kernel(ActGeom g,
+                                        const int32_t *__restrict__ slot,
+                                        const FaceRef *__restrict__ faces,
+                                        int64_t n_faces,
+                                        const double *__restrict__ fpbar,
+                                        double *__restrict__ pbar) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.n_cells) return;
+  if (slot[c] < 0) return;
+  double *pb = pbar + (int64_t)slot[c] * g.nq;
+  double s = 0.0;
+  for (int q = 0; q < g.nq; ++q) s += pb[q];
+  if (fpbar && n_faces > 0) {
+    int64_t lo = 0, hi = n_faces;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) / 2;
+      if (faces[mid].cell < c) lo = mid + 1; else hi = mid;
+    }
+    for (int64_t f = lo; f < n_faces && faces[f].cell == c; ++f)
+      for (int q = 0; q < g.nfq; ++q) s += fpbar[f * g.nfq + q];
+  }
+  const double v = s / g.nq;
+  for (int q = 0; q < g.nq; ++q) pb[q] = v;
+}
+
 template <class T>
 void grow(T *&ptr, size_t &cap, size_t n) {
   if (n <= cap && ptr) return;
@@ -359,6 +405,100 @@ ActWork::~ActWork() {
   if (h_totals) (void)hipHostFree(h_totals);
   if (h_boc) (void)hipHostFree(h_boc);
   if (h_boxes) (void)hipHostFree(h_boxes);
+}
+
+GeomSnap::~GeomSnap() {
+  (void)hipFree(meta);
+  (void)hipFree(cos);
+  (void)hipFree(sin);
+  (void)hipFree(faces);
+  (void)hipFree(fn_node);
+  (void)hipFree(fn_offset);
+  (void)hipFree(fn_slot);
+}
+
+void Context::save_geometry(GeomSnap &g, hipStream_t s) const {
+  auto keep = [&](auto *&dst, const auto *src, size_t count) {
+    using T = std::remove_const_t<std::remove_pointer_t<decltype(src)>>;
+    (void)hipFree(dst);
+    dst = nullptr;
+    ADM_HIP_ACT(hipMalloc(&dst, sizeof(T) * std::max<size_t>(count, 1)));
+    if (count)
+      ADM_HIP_ACT(hipMemcpyAsync(dst, src, sizeof(T) * count,
+                                 hipMemcpyDeviceToDevice, s));
+  };
+  int64_t n_slots = 0;
+  if (n_fnodes > 0)
+    ADM_HIP_ACT(hipMemcpy(&n_slots, d_fn_offset + n_fnodes, sizeof(int32_t),
+                          hipMemcpyDeviceToHost));
+  n_slots &= 0xffffffff;
+  keep(g.meta, d_meta, (size_t)n_cells);
+  keep(g.cos, d_cos, (size_t)n_cells);
+  keep(g.sin, d_sin, (size_t)n_cells);
+  keep(g.faces, d_faces, (size_t)n_faces);
+  keep(g.fn_node, d_fn_node, (size_t)n_fnodes);
+  keep(g.fn_offset, d_fn_offset, n_fnodes ? (size_t)n_fnodes + 1 : 0);
+  keep(g.fn_slot, d_fn_slot, (size_t)n_slots);
+  g.n_faces = n_faces;
+  g.n_fnodes = n_fnodes;
+  g.n_slots = n_slots;
+  g.nz_active = nz_active;
+  g.active = active;
+  g.dep_cos = dep_cos;
+  g.dep_sin = dep_sin;
+}
+
+void Context::restore_geometry(const GeomSnap &g, hipStream_t s) {
+  const int NF = (p + 1) * (p + 1);
+  grow(d_faces, face_cap, (size_t)g.n_faces);
+  grow(d_face_out, face_out_cap, (size_t)g.n_faces * NF);
+  if (n_states == 3) grow(d_face_powder, face_powder_cap, (size_t)g.n_faces * nfq);
+  grow(d_fn_node, fn_node_cap, (size_t)g.n_fnodes);
+  grow(d_fn_offset, fn_offset_cap, (size_t)g.n_fnodes + 1);
+  grow(d_fn_slot, fn_slot_cap, (size_t)g.n_slots);
+  auto put = [&](auto *dst, const auto *src, size_t count) {
+    if (count)
+      ADM_HIP_ACT(hipMemcpyAsync(dst, src, sizeof(*src) * count,
+                                 hipMemcpyDeviceToDevice, s));
+  };
+  put(d_meta, g.meta, (size_t)n_cells);
+  put(d_cos, g.cos, (size_t)n_cells);
+  put(d_sin, g.sin, (size_t)n_cells);
+  put(d_faces, g.faces, (size_t)g.n_faces);
+  put(d_fn_node, g.fn_node, (size_t)g.n_fnodes);
+  put(d_fn_offset, g.fn_offset, g.n_fnodes ? (size_t)g.n_fnodes + 1 : 0);
+  put(d_fn_slot, g.fn_slot, (size_t)g.n_slots);
+  n_faces = g.n_faces;
+  n_fnodes = g.n_fnodes;
+  nz_active = g.nz_active;
+  active = g.active;
+  dep_cos = g.dep_cos;
+  dep_sin = g.dep_sin;
+  faces_stale = true;
+}
+
+void Context::activation_adjoint(double *Tbar, double *pbar,
+                                 const double *fpbar, hipStream_t s) {
+  ActGeom g;
+  for (int d = 0; d < 3; ++d) {
+    g.nc[d] = nc[d];
+    g.nn[d] = nn[d];
+  }
+  g.p = p;
+  g.nq = nq;
+  g.nfq = nfq;
+  g.n_states = n_states;
+  g.n_cells = n_cells;
+  g.n_dofs = n_dofs;
+  if (Tbar)
+    hipLaunchKernelGGL(fresh_zero_kernel,
+                       dim3((unsigned)((n_dofs + 255) / 256)), dim3(256), 0,
+                       s, g, d_meta, Tbar);
+  if (pbar && n_states == 3)
+    hipLaunchKernelGGL(transfer_adjoint_kernel,
+                       dim3((unsigned)((n_cells + 255) / 256)), dim3(256), 0,
+                       s, g, d_powder_slot, d_faces, n_faces, fpbar, pbar);
+  ADM_HIP_ACT(hipGetLastError());
 }
 
 void Context::sync_faces_host() {

@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "../../include/adm.h"
@@ -54,6 +55,17 @@ struct AdjArgs {
                               // NULL: zero after, not stored
   double *partials;           // [3]: d/d eta_0, d/d k-factor, d/d R_0
   double inv_eta0;            // 1 / absorption efficiency of beam 0 (or 0)
+  // ensemble launch (c.members != NULL): member blockIdx.y's vectors at
+  // blockIdx.y * c.member_stride, its powder arrays at pw_off[blockIdx.y],
+  // its partials at partials + 3 blockIdx.y, 1/eta_0 from inv_eta
+  const int64_t *pw_off;
+  const double *inv_eta;
+};
+
+// The face term's arguments of an ensemble launch: member blockIdx.y's
+// face powder arrays at fpw_off[blockIdx.y].
+struct AdjFaceArgs {
+  const int64_t *fpw_off;
 };
 
 // Mixed property value v with its partials: dT at fixed carried powder and
@@ -198,15 +210,31 @@ __device__ __forceinline__ double beam_value_dR(const adm_beam_state &s,
 
 // (dA/dT)^T mu over the cells: same lane layout as cell_kernel (one x-line
 // per lane, (p+1)^2 lanes per cell, one row segment per wavefront), two
-// fields (T, mu) evaluated, atomic accumulation, parameter partials reduced
-// per wavefront.
+// fields (T, mu) evaluated, plain read-add-write of the cells of one parity
+// colour per launch (atomic adds when a launch covers every cell),
+// parameter partials reduced per wavefront.
 template <int P, int NS, int PK>
 __global__ void __launch_bounds__(256)
     adjoint_cell_kernel(const AdjArgs aa) {
   constexpr int N = P + 1, NL = N * N, N3 = N * N * N;
   constexpr int CPW = CellLanes<P>::cells_per_wave;
   const CellArgs &a = aa.c;
-  const KConst &kc = *(const KConst *)as_global(a.kc);
+  // one operator, or member blockIdx.y of an ensemble
+  const MemberArgs *mb = a.members ? a.members + blockIdx.y : nullptr;
+  const uint32_t *meta_p = mb ? mb->meta : a.meta;
+  const int32_t *pslot_p = mb ? mb->powder_slot : a.powder_slot;
+  const double *dcos_p = mb ? mb->dep_cos : a.dep_cos;
+  const double *dsin_p = mb ? mb->dep_sin : a.dep_sin;
+  const int64_t voff = mb ? (int64_t)blockIdx.y * a.member_stride : 0;
+  const int64_t poff = mb ? aa.pw_off[blockIdx.y] : 0;
+  const double *T_p = aa.T + voff;
+  const double *mu_p = aa.mu + voff;
+  double *out_p = aa.out + voff;
+  const double *pwo_p = aa.powder_old ? aa.powder_old + poff : nullptr;
+  double *pwb_p = aa.powder_bar ? aa.powder_bar + poff : nullptr;
+  double *part_p = aa.partials + (mb ? 3 * (int64_t)blockIdx.y : 0);
+  const double inv_eta0 = mb ? aa.inv_eta[blockIdx.y] : aa.inv_eta0;
+  const KConst &kc = *(const KConst *)as_global(mb ? mb->kc : a.kc);
   const MatConst &m = kc.mat;
   extern __shared__ double lds[];
   __shared__ adm_beam_state sbeams[ADM_MAX_BEAMS];
@@ -225,6 +253,11 @@ __global__ void __launch_bounds__(256)
       }
     }
     __syncthreads();
+    if (mb && tid < ADM_MAX_BEAMS)
+      sbeams[tid].alpha = member_alpha(sbeams[tid], mb->beam_gain[tid],
+                                       a.beam_pmod[tid], a.beam_den[tid],
+                                       kc.log_01);
+    if (mb) __syncthreads();
   }
   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
   const int j = lane / NL, L = lane % NL;
@@ -245,7 +278,7 @@ __global__ void __launch_bounds__(256)
   uint32_t meta = 0;
   if (ok) {
     c = rc.cc[0] + (int64_t)a.nc[0] * (rc.cc[1] + (int64_t)a.nc[1] * rc.cc[2]);
-    meta = a.meta[c];
+    meta = meta_p[c];
     ok = (meta & kActive) != 0;
   }
   if (ok) {
@@ -259,8 +292,8 @@ __global__ void __launch_bounds__(256)
       double u0[N], u1[N], t0[N], t1[N];
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        u0[i] = aa.T[base + i];
-        u1[i] = aa.mu[base + i];
+        u0[i] = T_p[base + i];
+        u1[i] = mu_p[base + i];
       }
 #pragma unroll
       for (int q = 0; q < N; ++q) {
@@ -343,15 +376,15 @@ __global__ void __launch_bounds__(256)
     const double sol = m.prop[mid][ADM_SOLIDUS], liq = m.prop[mid][ADM_LIQUIDUS];
     const double latent = m.latent_factor[mid];
     const int64_t slot =
-        (meta & kHasPowder) ? (int64_t)a.powder_slot[c] : -1;
-    const double *pw = slot >= 0 ? aa.powder_old + slot * N3 + own : nullptr;
-    double *pb = (slot >= 0 && aa.powder_bar) ? aa.powder_bar + slot * N3 + own
+        (meta & kHasPowder) ? (int64_t)pslot_p[c] : -1;
+    const double *pw = slot >= 0 ? pwo_p + slot * N3 + own : nullptr;
+    double *pb = (slot >= 0 && pwb_p) ? pwb_p + slot * N3 + own
                                               : nullptr;
     const bool rotated = (meta & kRotated) != 0;
     double cs = 1.0, sn = 0.0;
     if (rotated) {
-      cs = a.dep_cos[c];
-      sn = a.dep_sin[c];
+      cs = dcos_p[c];
+      sn = dsin_p[c];
     }
     const double *mB = lds + mid * kMergedLds;
     const double *mC = mB + kMergedMax;
@@ -416,7 +449,7 @@ __global__ void __launch_bounds__(256)
         }
         c_T += mu * ircT * qs * jxw;
         if constexpr (NS == 3) c_p += mu * ircp * qs * jxw;
-        pe += mu * irc * q0 * aa.inv_eta0 * jxw;
+        pe += mu * irc * q0 * inv_eta0 * jxw;
         pr += mu * irc * beam_value_dR(sbeams[0], q0, px, py, pz, kc.log_01) * jxw;
       }
       if constexpr (NS == 3) {
@@ -488,7 +521,11 @@ __global__ void __launch_bounds__(256)
       double acc = kc.S[0][i] * t[0];
 #pragma unroll
       for (int q = 1; q < N; ++q) acc = __builtin_fma(kc.S[q][i], t[q], acc);
-      unsafeAtomicAdd(aa.out + base + i, acc);
+      // colour launches: no other cell of the launch holds the node
+      if (a.color >= 0)
+        out_p[base + i] += acc;
+      else
+        unsafeAtomicAdd(out_p + base + i, acc);
     }
   }
   // parameter partials: wavefront sum, one atomic per wavefront
@@ -499,9 +536,9 @@ __global__ void __launch_bounds__(256)
     pr += __shfl_xor(pr, off);
   }
   if (lane == 0 && (pe != 0.0 || ps != 0.0 || pr != 0.0)) {
-    unsafeAtomicAdd(aa.partials + 0, pe);
-    unsafeAtomicAdd(aa.partials + 1, ps);
-    unsafeAtomicAdd(aa.partials + 2, pr);
+    unsafeAtomicAdd(part_p + 0, pe);
+    unsafeAtomicAdd(part_p + 1, ps);
+    unsafeAtomicAdd(part_p + 2, pr);
   }
 }
 
@@ -515,14 +552,26 @@ __global__ void __launch_bounds__(256)
                         const double *__restrict__ mu,
                         const double *__restrict__ face_powder_old,
                         double *__restrict__ face_powder_bar,
-                        double *__restrict__ out) {
+                        double *__restrict__ out, const AdjFaceArgs af) {
   constexpr int N = P + 1, NF = N * N;
   const int64_t fi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (fi >= a.n_faces) return;
-  const KConst &kc = *(const KConst *)as_global(a.kc);
+  // one operator, or member blockIdx.y of an ensemble
+  const MemberArgs *mb = a.members ? a.members + blockIdx.y : nullptr;
+  if (fi >= (mb ? mb->n_faces : a.n_faces)) return;
+  if (mb) {
+    const int64_t voff = (int64_t)blockIdx.y * a.member_stride;
+    const int64_t foff = af.fpw_off[blockIdx.y];
+    T += voff;
+    mu += voff;
+    out += voff;
+    if (face_powder_old) face_powder_old += foff;
+    if (face_powder_bar) face_powder_bar += foff;
+  }
+  const KConst &kc = *(const KConst *)as_global(mb ? mb->kc : a.kc);
   const MatConst &m = kc.mat;
-  const int64_t c = a.faces[fi].cell;
-  const int info = a.faces[fi].info;
+  const FaceRef *faces = mb ? mb->faces : a.faces;
+  const int64_t c = faces[fi].cell;
+  const int info = faces[fi].info;
   const int f = info & 7, bt = (info >> 3) & 31, mid = (info >> 8) & 3;
   const int d = f / 2, side = f % 2;
   const int cc[3] = {(int)(c % a.nc[0]), (int)((c / a.nc[0]) % a.nc[1]),
@@ -659,6 +708,13 @@ __global__ void scaled_mass_kernel(double *__restrict__ mu,
 struct DevBuf {
   double *p = nullptr;
   size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
   void alloc(size_t n_) {
     if (n_ == 0) return;
     ADM_HIP_ADJ(hipMalloc(&p, sizeof(double) * n_));
@@ -680,10 +736,25 @@ inline void eval_time_reset(Context &c, double t) {
 // adjoints of the carried cell and face powder (pb, fpb; NULL: zero) are
 // taken from after to before the evaluation, and the parameter partials are
 // added to partials[3].
+// An ensemble launch: member j (grid y) reads its vectors at j * stride,
+// its powder arrays at pw_off[j] / fpw_off[j] (device), its 1/eta_0 at
+// inv_eta[j] (device) and writes its partials at partials + 3 j.
+struct VjpBatch {
+  const MemberArgs *members;
+  int n_members;
+  int64_t stride;
+  double pmod[ADM_MAX_BEAMS], den[ADM_MAX_BEAMS];
+  const int64_t *pw_off, *fpw_off;
+  const double *inv_eta;
+  int64_t max_faces;
+  int nz_active;  // the largest over the members
+};
+
 inline void launch_vjp(Context &c, const double *Y, const double *pw,
                        const double *fpw, double *pb, double *fpb,
                        const double *mu, double *out, double *partials,
-                       double inv_eta0, hipStream_t s) {
+                       double inv_eta0, hipStream_t s,
+                       const VjpBatch *batch = nullptr) {
   const int pk = c.property_kind();
   if (pk == 1)
     throw AdmError(ADM_ERR_UNSUPPORTED,
@@ -692,11 +763,7 @@ inline void launch_vjp(Context &c, const double *Y, const double *pw,
   AdjArgs aa;
   std::memset(&aa, 0, sizeof(aa));
   aa.c = c.cell_args();
-  aa.c.color = -1;
   const int cpw = cells_per_wave(c.p);
-  for (int d = 0; d < 3; ++d) aa.c.lext[d] = c.nc[d];
-  aa.c.nseg = (c.nc[0] + cpw - 1) / cpw;
-  aa.c.n_rows = aa.c.nseg * c.nc[1] * c.nc[2];
   aa.T = Y;
   aa.mu = mu;
   aa.out = out;
@@ -704,21 +771,60 @@ inline void launch_vjp(Context &c, const double *Y, const double *pw,
   aa.powder_bar = pb;
   aa.partials = partials;
   aa.inv_eta0 = inv_eta0;
-  const unsigned blocks = (unsigned)((aa.c.n_rows + 3) / 4);
+  FaceArgs fa = c.face_args();
+  AdjFaceArgs af{nullptr};
+  unsigned ny = 1;
+  if (batch) {
+    aa.c.members = batch->members;
+    aa.c.member_stride = batch->stride;
+    for (int b = 0; b < ADM_MAX_BEAMS; ++b) {
+      aa.c.beam_pmod[b] = batch->pmod[b];
+      aa.c.beam_den[b] = batch->den[b];
+    }
+    aa.pw_off = batch->pw_off;
+    aa.inv_eta = batch->inv_eta;
+    fa.members = batch->members;
+    fa.member_stride = batch->stride;
+    fa.n_faces = batch->max_faces;
+    af.fpw_off = batch->fpw_off;
+    ny = (unsigned)batch->n_members;
+  }
+  // the cell layers that can hold an active cell
+  const int nz = batch ? batch->nz_active : c.nz_active;
+  const int lim[3] = {c.nc[0], c.nc[1], std::min(nz, c.nc[2])};
   const size_t lds =
       (pk == 2 ? sizeof(double) * kMergedLds * c.kc.mat.n_materials : 0) +
       sizeof(double) * 4 * cpw *
           4 * (c.p + 1) * (c.p + 1) * (c.p + 1);
-  FaceArgs fa = c.face_args();
   const unsigned face_blocks = (unsigned)((fa.n_faces + 255) / 256);
+  // eight launches, one per parity colour: cells of one colour share no
+  // node, so each adds its nodes' terms with a plain read-add-write
+  AdjArgs ac[8];
+  unsigned blocks[8];
+  for (int col = 0; col < 8; ++col) {
+    ac[col] = aa;
+    ac[col].c.color = col;
+    int64_t cells = 1;
+    for (int d = 0; d < 3; ++d) {
+      const int b = (col >> d) & 1;
+      ac[col].c.lext[d] = lim[d] > b ? (lim[d] - b + 1) / 2 : 0;
+      cells *= ac[col].c.lext[d];
+    }
+    ac[col].c.nseg = (ac[col].c.lext[0] + cpw - 1) / cpw;
+    ac[col].c.n_rows = ac[col].c.nseg * ac[col].c.lext[1] * ac[col].c.lext[2];
+    blocks[col] = cells ? (unsigned)((ac[col].c.n_rows + 3) / 4) : 0u;
+  }
 #define ADM_ADJ(P_, NS_, PK_)                                              \
   if (c.p == P_ && c.n_states == NS_ && pk == PK_) {                       \
-    hipLaunchKernelGGL((adjoint_cell_kernel<P_, NS_, PK_>), dim3(blocks),  \
-                       dim3(256), lds, s, aa);                              \
+    for (int col = 0; col < 8; ++col)                                      \
+      if (blocks[col])                                                     \
+        hipLaunchKernelGGL((adjoint_cell_kernel<P_, NS_, PK_>),            \
+                           dim3(blocks[col], ny), dim3(256), lds, s,       \
+                           ac[col]);                                       \
     if (fa.n_faces)                                                        \
       hipLaunchKernelGGL((adjoint_face_kernel<P_, NS_, PK_>),              \
-                         dim3(face_blocks), dim3(256), 0, s, fa, Y, mu, fpw, \
-                         fpb, out);                                         \
+                         dim3(face_blocks, ny), dim3(256), 0, s, fa, Y, mu, \
+                         fpw, fpb, out, af);                                \
     launched = true;                                                       \
   }
   bool launched = false;
@@ -759,6 +865,47 @@ inline void launch_vjp(Context &c, const double *Y, const double *pw,
   ADM_HIP_ADJ(hipGetLastError());
 }
 
+// Butcher tableau of the explicit steppers (explicit_step's stage order)
+struct Tableau {
+  int ns = 1;
+  double a[4][4], b[4], c[4];
+  explicit Tableau(int method) {
+    std::memset(a, 0, sizeof(a));
+    std::memset(b, 0, sizeof(b));
+    std::memset(c, 0, sizeof(c));
+    if (method == ADM_FORWARD_EULER) {
+      ns = 1;
+      b[0] = 1.0;
+    } else if (method == ADM_RK_FOURTH_ORDER) {
+      // classic RK4 (TimeStepping::RK_CLASSIC_FOURTH_ORDER)
+      ns = 4;
+      const double bb[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+      const double cc[4] = {0.0, 0.5, 0.5, 1.0};
+      for (int i = 0; i < 4; ++i) {
+        b[i] = bb[i];
+        c[i] = cc[i];
+      }
+      a[1][0] = 0.5;
+      a[2][1] = 0.5;
+      a[3][2] = 1.0;
+    } else if (method == ADM_RK_THIRD_ORDER) {
+      // Kutta's third order (TimeStepping::RK_THIRD_ORDER)
+      ns = 3;
+      const double bb[3] = {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0};
+      const double cc[3] = {0.0, 0.5, 1.0};
+      for (int i = 0; i < 3; ++i) {
+        b[i] = bb[i];
+        c[i] = cc[i];
+      }
+      a[1][0] = 0.5;
+      a[2][0] = -1.0;
+      a[2][1] = 2.0;
+    } else {
+      throw AdmError(ADM_ERR_INVALID, "unknown time-stepping method");
+    }
+  }
+};
+
 struct AdjointRun {
   Context &c;
   hipStream_t s;
@@ -776,38 +923,11 @@ struct AdjointRun {
     n = c.n_dofs;
     npw = c.n_powder_cells * c.nq;
     nfpw = (c.n_states == 3) ? c.n_faces * c.nfq : 0;
-    std::memset(a, 0, sizeof(a));
-    if (method == ADM_FORWARD_EULER) {
-      n_stages = 1;
-      b[0] = 1.0;
-      cst[0] = 0.0;
-    } else if (method == ADM_RK_FOURTH_ORDER) {
-      // classic RK4 (TimeStepping::RK_CLASSIC_FOURTH_ORDER)
-      n_stages = 4;
-      const double bb[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
-      const double cc[4] = {0.0, 0.5, 0.5, 1.0};
-      for (int i = 0; i < 4; ++i) {
-        b[i] = bb[i];
-        cst[i] = cc[i];
-      }
-      a[1][0] = 0.5;
-      a[2][1] = 0.5;
-      a[3][2] = 1.0;
-    } else if (method == ADM_RK_THIRD_ORDER) {
-      // Kutta's third order (TimeStepping::RK_THIRD_ORDER)
-      n_stages = 3;
-      const double bb[3] = {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0};
-      const double cc[3] = {0.0, 0.5, 1.0};
-      for (int i = 0; i < 3; ++i) {
-        b[i] = bb[i];
-        cst[i] = cc[i];
-      }
-      a[1][0] = 0.5;
-      a[2][0] = -1.0;
-      a[2][1] = 2.0;
-    } else {
-      throw AdmError(ADM_ERR_INVALID, "unknown time-stepping method");
-    }
+    const Tableau tb(method);
+    n_stages = tb.ns;
+    std::memcpy(a, tb.a, sizeof(a));
+    std::memcpy(b, tb.b, sizeof(b));
+    std::memcpy(cst, tb.c, sizeof(cst));
     for (int i = 0; i < n_stages; ++i) K[i].alloc(n);
     Y.alloc(n);
     R.alloc(n);
@@ -878,6 +998,421 @@ struct AdjointRun {
                partials.p, inv_eta0, s);
   }
 };
+
+
+// ---------------------------------------------------------------------------
+// ensemble driver: every member's gradient from one batched forward and one
+// batched backward sweep, with material activations inside the window
+// ---------------------------------------------------------------------------
+
+struct EnsEvent {
+  int member, step;
+  std::vector<double> boxes, cos, sin;
+  double new_temperature;
+};
+
+// the beams of the members at time t (scan-path search from the start)
+inline void ens_time(Ensemble &E, double t) {
+  Context &c0 = *E.members[0];
+  for (size_t b = 0; b < c0.old_segment.size(); ++b) c0.old_segment[b] = 0;
+  E.set_time(t);
+}
+
+struct EnsAdjoint {
+  Ensemble &E;
+  hipStream_t s;
+  int N;
+  int64_t n, nt;  // per member, all members
+  Tableau tab;
+  DevBuf K[4], Y, R;
+  // powder of the cells (fixed slots) and of the faces (capacity per member)
+  std::vector<int64_t> pw_off, fpw_off;
+  int64_t npw = 0, nfpw = 0;
+  DevBuf d_off, d_inv_eta;  // [pw_off | fpw_off] as int64, 1/eta_0
+  VjpBatch batch;
+
+  EnsAdjoint(Ensemble &e, int method, hipStream_t s_)
+      : E(e), s(s_), N(e.n_members), n(e.n_dofs), nt(e.n_dofs * e.n_members),
+        tab(method) {
+    for (int i = 0; i < tab.ns; ++i) K[i].alloc(nt);
+    Y.alloc(nt);
+    R.alloc(nt);
+    ADM_HIP_ADJ(hipMemsetAsync(R.p, 0, sizeof(double) * nt, s));
+    pw_off.assign(N + 1, 0);
+    for (int j = 0; j < N; ++j)
+      pw_off[j + 1] = pw_off[j] + E.members[j]->n_powder_cells * E.members[j]->nq;
+    npw = pw_off[N];
+  }
+
+  unsigned grid(int64_t m) const { return (unsigned)((m + 255) / 256); }
+
+  // device tables once the face capacities are known
+  void finish_tables(const std::vector<int64_t> &fcap) {
+    fpw_off.assign(N + 1, 0);
+    for (int j = 0; j < N; ++j) fpw_off[j + 1] = fpw_off[j] + fcap[j];
+    nfpw = fpw_off[N];
+    d_off.alloc(2 * (size_t)N);
+    std::vector<double> off(2 * (size_t)N);
+    for (int j = 0; j < N; ++j) {
+      std::memcpy(&off[j], &pw_off[j], sizeof(int64_t));
+      std::memcpy(&off[N + j], &fpw_off[j], sizeof(int64_t));
+    }
+    ADM_HIP_ADJ(hipMemcpy(d_off.p, off.data(), sizeof(double) * off.size(),
+                          hipMemcpyHostToDevice));
+    std::vector<double> ie(N);
+    for (int j = 0; j < N; ++j) {
+      const Context &c = *E.members[j];
+      const double eta =
+          c.beams.empty() ? 0.0 : c.beams[0].desc.absorption_efficiency;
+      ie[j] = eta != 0.0 ? 1.0 / eta : 0.0;
+    }
+    d_inv_eta.alloc(N);
+    ADM_HIP_ADJ(hipMemcpy(d_inv_eta.p, ie.data(), sizeof(double) * N,
+                          hipMemcpyHostToDevice));
+    batch.n_members = N;
+    batch.stride = n;
+    batch.pw_off = reinterpret_cast<const int64_t *>(d_off.p);
+    batch.fpw_off = reinterpret_cast<const int64_t *>(d_off.p) + N;
+    batch.inv_eta = d_inv_eta.p;
+  }
+
+  // members' powder <-> a packed copy (cells: fixed slots; faces: the
+  // current face count of each member, at its offset)
+  void copy_powder(double *pw, double *fpw, bool save) {
+    for (int j = 0; j < N; ++j) {
+      Context &c = *E.members[j];
+      const int64_t a = pw_off[j + 1] - pw_off[j];
+      if (pw && a)
+        ADM_HIP_ADJ(hipMemcpyAsync(save ? pw + pw_off[j] : c.d_powder,
+                                   save ? c.d_powder : pw + pw_off[j],
+                                   sizeof(double) * a,
+                                   hipMemcpyDeviceToDevice, s));
+      const int64_t f = c.n_states == 3 ? c.n_faces * c.nfq : 0;
+      if (fpw && f)
+        ADM_HIP_ADJ(hipMemcpyAsync(save ? fpw + fpw_off[j] : c.d_face_powder,
+                                   save ? c.d_face_powder : fpw + fpw_off[j],
+                                   sizeof(double) * f,
+                                   hipMemcpyDeviceToDevice, s));
+    }
+  }
+
+  struct Rec {
+    double *Y, *pw, *fpw;
+  };
+  // one batched step of every member (the stage sequence of
+  // AdjointRun::forward_step)
+  void forward_step(double t, double dt, double *T, const Rec *rec) {
+    for (int i = 0; i < tab.ns; ++i) {
+      const double *Yi = T;
+      bool first = true;
+      for (int j = 0; j < i; ++j) {
+        if (tab.a[i][j] == 0.0) continue;
+        if (first)
+          hipLaunchKernelGGL(axpy_copy_kernel, dim3(grid(nt)), dim3(256), 0, s,
+                             Y.p, T, dt * tab.a[i][j], K[j].p, nt);
+        else
+          hipLaunchKernelGGL(axpy_kernel, dim3(grid(nt)), dim3(256), 0, s, Y.p,
+                             K[j].p, dt * tab.a[i][j], nt);
+        first = false;
+        Yi = Y.p;
+      }
+      if (rec) {
+        ADM_HIP_ADJ(hipMemcpyAsync(rec[i].Y, Yi, sizeof(double) * nt,
+                                   hipMemcpyDeviceToDevice, s));
+        copy_powder(rec[i].pw, rec[i].fpw, true);
+      }
+      ens_time(E, t + tab.c[i] * dt);
+      E.apply(Yi, R.p, 0, s);
+      hipLaunchKernelGGL(stage_value_kernel, dim3(grid(nt)), dim3(256), 0, s,
+                         K[i].p, R.p, E.sb.inv_mass, nt);
+    }
+    for (int i = 0; i < tab.ns; ++i)
+      hipLaunchKernelGGL(axpy_kernel, dim3(grid(nt)), dim3(256), 0, s, T,
+                         K[i].p, dt * tab.b[i], nt);
+    ADM_HIP_ADJ(hipGetLastError());
+  }
+
+  void vjp(const Rec &rec, double t_eval, const double *mu, double *out,
+           double *pbar, double *fpbar, double *partials) {
+    ens_time(E, t_eval);
+    batch.members = E.d_members;
+    for (int b = 0; b < ADM_MAX_BEAMS; ++b) {
+      batch.pmod[b] = E.pmod[b];
+      batch.den[b] = E.den[b];
+    }
+    batch.max_faces = E.max_faces;
+    batch.nz_active = 0;
+    for (const auto &m : E.members)
+      batch.nz_active = std::max(batch.nz_active, m->nz_active);
+    launch_vjp(*E.members[0], rec.Y, rec.pw, rec.fpw, pbar, fpbar, mu, out,
+               partials, 0.0, s, &batch);
+  }
+
+  void activate(const EnsEvent &e, double *T) {
+    Context &c = *E.members[e.member];
+    c.activate((int)e.cos.size(), e.boxes.data(), nullptr, e.cos.data(),
+               e.sin.data(), e.new_temperature, T + (int64_t)e.member * n,
+               nullptr, s);
+    E.refresh_members();
+    c.compute_inverse_mass(E.sb.inv_mass + (int64_t)e.member * n, s);
+  }
+};
+
+int ensemble_adjoint(Ensemble &E, int method, double t0, double dt,
+                     int n_steps, int checkpoint_every, const double *T0,
+                     int n_obs, const int64_t *obs_dof, const double *obs,
+                     const double *obs_weight, std::vector<EnsEvent> ev,
+                     double *T_final, double *J, double *grad,
+                     double *lambda0, hipStream_t s) {
+  const int N = E.n_members;
+  const int64_t n = E.n_dofs, nt = n * N;
+  const int K = checkpoint_every > 0 ? checkpoint_every : n_steps;
+  for (const auto &m : E.members)
+    if (m->n_ext >= 0)
+      throw AdmError(ADM_ERR_UNSUPPORTED, "the adjoint works on box nodes");
+  for (const EnsEvent &e : ev)
+    if (!E.members[e.member]->device_activation_fits())
+      throw AdmError(ADM_ERR_UNSUPPORTED,
+                     "mesh too large for the device activation");
+  E.ensure_step_buffers(ADM_FORWARD_EULER);
+  std::stable_sort(ev.begin(), ev.end(), [](const EnsEvent &a,
+                                            const EnsEvent &b) {
+    return a.step < b.step;
+  });
+  EnsAdjoint run(E, method, s);
+  const int S = run.tab.ns;
+  // segments: every K steps and at every activation step
+  std::vector<int> bnd;
+  for (int k = 0; k < n_steps; ++k) {
+    bool b = k % K == 0;
+    for (const EnsEvent &e : ev) b |= e.step == k;
+    if (b) bnd.push_back(k);
+  }
+  const int n_seg = (int)bnd.size();
+  bnd.push_back(n_steps);
+  int max_len = 0;
+  for (int i = 0; i < n_seg; ++i) max_len = std::max(max_len, bnd[i + 1] - bnd[i]);
+  std::vector<double> times((size_t)n_steps + 1);
+  {
+    double t = t0;
+    for (int k = 0; k <= n_steps; ++k) {
+      times[k] = t;
+      t = t + dt;
+    }
+  }
+  // forward sweep: activations, checkpoints, steps
+  DevBuf T, cpT, cpP;
+  T.alloc(nt);
+  cpT.alloc((size_t)nt * n_seg);
+  cpP.alloc((size_t)run.npw * n_seg);
+  std::vector<std::vector<DevBuf>> cpF(n_seg);
+  std::vector<int64_t> fcap(N, 0);
+  for (int j = 0; j < N; ++j)
+    fcap[j] = E.members[j]->n_states == 3
+                  ? E.members[j]->n_faces * E.members[j]->nfq : 0;
+  std::vector<std::unique_ptr<GeomSnap>> snap(ev.size());
+  ADM_HIP_ADJ(hipMemcpyAsync(T.p, T0, sizeof(double) * nt,
+                             hipMemcpyDeviceToDevice, s));
+  size_t next_ev = 0;
+  auto cp_faces = [&](std::vector<DevBuf> &dst, bool save) {
+    if (save) dst.resize(N);
+    for (int j = 0; j < N; ++j) {
+      Context &c = *E.members[j];
+      const int64_t f = c.n_states == 3 ? c.n_faces * c.nfq : 0;
+      if (save) dst[j].alloc((size_t)f);
+      if (f)
+        ADM_HIP_ADJ(hipMemcpyAsync(save ? dst[j].p : c.d_face_powder,
+                                   save ? c.d_face_powder : dst[j].p,
+                                   sizeof(double) * f,
+                                   hipMemcpyDeviceToDevice, s));
+    }
+  };
+  for (int i = 0; i < n_seg; ++i) {
+    const int k0 = bnd[i], k1 = bnd[i + 1];
+    while (next_ev < ev.size() && ev[next_ev].step == k0) {
+      const EnsEvent &e = ev[next_ev];
+      snap[next_ev].reset(new GeomSnap());
+      ADM_HIP_ADJ(hipStreamSynchronize(s));
+      E.members[e.member]->save_geometry(*snap[next_ev], s);
+      run.activate(e, T.p);
+      Context &c = *E.members[e.member];
+      if (c.n_states == 3)
+        fcap[e.member] = std::max(fcap[e.member], c.n_faces * c.nfq);
+      ++next_ev;
+    }
+    ADM_HIP_ADJ(hipMemcpyAsync(cpT.p + (int64_t)i * nt, T.p,
+                               sizeof(double) * nt, hipMemcpyDeviceToDevice,
+                               s));
+    // cell powder at its fixed offsets (face offsets are not needed here)
+    for (int j = 0; j < N; ++j) {
+      Context &c = *E.members[j];
+      const int64_t a = run.pw_off[j + 1] - run.pw_off[j];
+      if (a)
+        ADM_HIP_ADJ(hipMemcpyAsync(cpP.p + (int64_t)i * run.npw + run.pw_off[j],
+                                   c.d_powder, sizeof(double) * a,
+                                   hipMemcpyDeviceToDevice, s));
+    }
+    cp_faces(cpF[i], true);
+    for (int k = k0; k < k1; ++k) run.forward_step(times[k], dt, T.p, nullptr);
+  }
+  if (next_ev != ev.size())
+    throw AdmError(ADM_ERR_INVALID, "an activation step is out of the window");
+  run.finish_tables(fcap);
+  // the state at the final time (put back at the end)
+  std::vector<std::unique_ptr<GeomSnap>> fin_geom(N);
+  std::vector<bool> moved(N, false);
+  for (const EnsEvent &e : ev) moved[e.member] = true;
+  ADM_HIP_ADJ(hipStreamSynchronize(s));
+  for (int j = 0; j < N; ++j)
+    if (moved[j]) {
+      fin_geom[j].reset(new GeomSnap());
+      E.members[j]->save_geometry(*fin_geom[j], s);
+    }
+  DevBuf finP, finF;
+  finP.alloc(run.npw);
+  finF.alloc(run.nfpw);
+  run.copy_powder(finP.p, finF.p, true);
+  std::vector<DevBuf> finL(N), finD(N);
+  for (int j = 0; j < N; ++j) {
+    Context &c = *E.members[j];
+    if (!c.d_liquid) continue;
+    finL[j].alloc((size_t)c.n_cells * c.nq);
+    ADM_HIP_ADJ(hipMemcpyAsync(finL[j].p, c.d_liquid,
+                               sizeof(double) * c.n_cells * c.nq,
+                               hipMemcpyDeviceToDevice, s));
+    finD[j].alloc((size_t)(c.n_cells + 7) / 8);
+    ADM_HIP_ADJ(hipMemcpyAsync(finD[j].p, c.d_liquid_dirty, (size_t)c.n_cells,
+                               hipMemcpyDeviceToDevice, s));
+  }
+  if (T_final)
+    ADM_HIP_ADJ(hipMemcpyAsync(T_final, T.p, sizeof(double) * nt,
+                               hipMemcpyDeviceToDevice, s));
+  // misfits and their gradients
+  std::vector<double> Tf((size_t)nt);
+  ADM_HIP_ADJ(hipMemcpyAsync(Tf.data(), T.p, sizeof(double) * nt,
+                             hipMemcpyDeviceToHost, s));
+  ADM_HIP_ADJ(hipStreamSynchronize(s));
+  std::vector<double> tb((size_t)nt, 0.0);
+  for (int j = 0; j < N; ++j) {
+    double Jv = 0.0;
+    for (int k = 0; k < n_obs; ++k) {
+      const double w = obs_weight ? obs_weight[k] : 1.0;
+      const double r = Tf[(size_t)j * n + obs_dof[k]] - obs[(size_t)j * n_obs + k];
+      Jv += 0.5 * w * r * r;
+      tb[(size_t)j * n + obs_dof[k]] += w * r;
+    }
+    J[j] = Jv;
+  }
+  DevBuf Tb, mu, Ybar, partials, pbar, fpbar;
+  Tb.alloc(nt);
+  mu.alloc(nt);
+  Ybar.alloc(nt);
+  partials.alloc(3 * (size_t)N);
+  pbar.alloc(run.npw);
+  fpbar.alloc(run.nfpw);
+  ADM_HIP_ADJ(hipMemcpyAsync(Tb.p, tb.data(), sizeof(double) * nt,
+                             hipMemcpyHostToDevice, s));
+  ADM_HIP_ADJ(hipMemsetAsync(partials.p, 0, sizeof(double) * 3 * N, s));
+  if (run.npw) ADM_HIP_ADJ(hipMemsetAsync(pbar.p, 0, sizeof(double) * run.npw, s));
+  if (run.nfpw)
+    ADM_HIP_ADJ(hipMemsetAsync(fpbar.p, 0, sizeof(double) * run.nfpw, s));
+  // per-segment recording
+  DevBuf recY, recP, recF;
+  recY.alloc((size_t)nt * max_len * S);
+  recP.alloc((size_t)run.npw * max_len * S);
+  recF.alloc((size_t)run.nfpw * max_len * S);
+  std::vector<EnsAdjoint::Rec> rec((size_t)max_len * S);
+  for (int64_t k = 0; k < (int64_t)max_len * S; ++k) {
+    rec[k].Y = recY.p + k * nt;
+    rec[k].pw = run.npw ? recP.p + k * run.npw : nullptr;
+    rec[k].fpw = run.nfpw ? recF.p + k * run.nfpw : nullptr;
+  }
+  const unsigned gn = run.grid(nt);
+  int ev_i = (int)ev.size() - 1;
+  for (int i = n_seg - 1; i >= 0; --i) {
+    const int k0 = bnd[i], k1 = bnd[i + 1];
+    ADM_HIP_ADJ(hipMemcpyAsync(T.p, cpT.p + (int64_t)i * nt,
+                               sizeof(double) * nt, hipMemcpyDeviceToDevice,
+                               s));
+    for (int j = 0; j < N; ++j) {
+      Context &c = *E.members[j];
+      const int64_t a = run.pw_off[j + 1] - run.pw_off[j];
+      if (a)
+        ADM_HIP_ADJ(hipMemcpyAsync(c.d_powder,
+                                   cpP.p + (int64_t)i * run.npw + run.pw_off[j],
+                                   sizeof(double) * a,
+                                   hipMemcpyDeviceToDevice, s));
+    }
+    cp_faces(cpF[i], false);
+    for (int k = k0; k < k1; ++k)
+      run.forward_step(times[k], dt, T.p, &rec[(size_t)(k - k0) * S]);
+    for (int k = k1 - 1; k >= k0; --k) {
+      const EnsAdjoint::Rec *rk = &rec[(size_t)(k - k0) * S];
+      for (int st = 0; st < S; ++st)
+        hipLaunchKernelGGL(scale_kernel, dim3(gn), dim3(256), 0, s,
+                           run.K[st].p, Tb.p, dt * run.tab.b[st], nt);
+      for (int st = S - 1; st >= 0; --st) {
+        hipLaunchKernelGGL(scaled_mass_kernel, dim3(gn), dim3(256), 0, s, mu.p,
+                           run.K[st].p, E.sb.inv_mass, 1.0, nt);
+        ADM_HIP_ADJ(hipMemsetAsync(Ybar.p, 0, sizeof(double) * nt, s));
+        run.vjp(rk[st], times[k] + run.tab.c[st] * dt, mu.p, Ybar.p, pbar.p,
+                fpbar.p, partials.p);
+        hipLaunchKernelGGL(axpy_kernel, dim3(gn), dim3(256), 0, s, Tb.p,
+                           Ybar.p, 1.0, nt);
+        for (int jj = 0; jj < st; ++jj)
+          if (run.tab.a[st][jj] != 0.0)
+            hipLaunchKernelGGL(axpy_kernel, dim3(gn), dim3(256), 0, s,
+                               run.K[jj].p, Ybar.p, dt * run.tab.a[st][jj], nt);
+      }
+    }
+    // the activations at the start of the segment, last first
+    for (; ev_i >= 0 && ev[ev_i].step == k0; --ev_i) {
+      const EnsEvent &e = ev[ev_i];
+      const int j = e.member;
+      Context &c = *E.members[j];
+      c.activation_adjoint(Tb.p + (int64_t)j * n,
+                           run.npw ? pbar.p + run.pw_off[j] : nullptr,
+                           run.nfpw ? fpbar.p + run.fpw_off[j] : nullptr, s);
+      const int64_t f = run.fpw_off[j + 1] - run.fpw_off[j];
+      if (f)
+        ADM_HIP_ADJ(hipMemsetAsync(fpbar.p + run.fpw_off[j], 0,
+                                   sizeof(double) * f, s));
+      ADM_HIP_ADJ(hipStreamSynchronize(s));
+      c.restore_geometry(*snap[ev_i], s);
+      E.refresh_members();
+      c.compute_inverse_mass(E.sb.inv_mass + (int64_t)j * n, s);
+    }
+  }
+  // back to the final state
+  ADM_HIP_ADJ(hipStreamSynchronize(s));
+  for (int j = 0; j < N; ++j) {
+    Context &c = *E.members[j];
+    if (moved[j]) {
+      c.restore_geometry(*fin_geom[j], s);
+      c.compute_inverse_mass(E.sb.inv_mass + (int64_t)j * n, s);
+    }
+    if (c.d_liquid) {
+      ADM_HIP_ADJ(hipMemcpyAsync(c.d_liquid, finL[j].p,
+                                 sizeof(double) * c.n_cells * c.nq,
+                                 hipMemcpyDeviceToDevice, s));
+      ADM_HIP_ADJ(hipMemcpyAsync(c.d_liquid_dirty, finD[j].p,
+                                 (size_t)c.n_cells, hipMemcpyDeviceToDevice,
+                                 s));
+    }
+  }
+  E.refresh_members();
+  run.copy_powder(finP.p, finF.p, false);
+  ens_time(E, times[n_steps]);
+  std::vector<double> pp(3 * (size_t)N);
+  ADM_HIP_ADJ(hipMemcpyAsync(pp.data(), partials.p, sizeof(double) * 3 * N,
+                             hipMemcpyDeviceToHost, s));
+  if (lambda0)
+    ADM_HIP_ADJ(hipMemcpyAsync(lambda0, Tb.p, sizeof(double) * nt,
+                               hipMemcpyDeviceToDevice, s));
+  ADM_HIP_ADJ(hipStreamSynchronize(s));
+  for (int k = 0; k < 3 * N; ++k) grad[k] = pp[k];
+  return ADM_OK;
+}
 
 }  // namespace adm
 
@@ -1050,6 +1585,46 @@ int adm_adjoint_gradient(void *ptr, int method, double t0, double dt,
     return adm::set_error(ADM_ERR_INVALID, e.what());
   }
   return ADM_OK;
+}
+
+int adm_ensemble_adjoint_gradient(
+    void *ptr, int method, double t0, double dt, int n_steps,
+    int checkpoint_every, const double *T0, int32_t n_obs,
+    const int64_t *obs_dof, const double *obs, const double *obs_weight,
+    int32_t n_events, const adm_activation_event *events, double *T_final,
+    double *J, double *grad, double *lambda0, void *stream) {
+  try {
+    if (!ptr) return adm::set_error(ADM_ERR_STATE, "null ensemble");
+    adm::Ensemble &E = *static_cast<adm::Ensemble *>(ptr);
+    if (!T0 || !obs_dof || !obs || !J || !grad || n_obs < 1 || n_steps < 1 ||
+        n_events < 0 || (n_events && !events))
+      return adm::set_error(ADM_ERR_INVALID, "bad adjoint arguments");
+    for (int k = 0; k < n_obs; ++k)
+      if (obs_dof[k] < 0 || obs_dof[k] >= E.n_dofs)
+        return adm::set_error(ADM_ERR_INVALID, "observation DoF out of range");
+    std::vector<adm::EnsEvent> ev(n_events);
+    for (int i = 0; i < n_events; ++i) {
+      const adm_activation_event &e = events[i];
+      if (e.member < 0 || e.member >= E.n_members || e.step < 0 ||
+          e.step >= n_steps || e.n_boxes < 0 ||
+          (e.n_boxes && (!e.boxes || !e.box_cos || !e.box_sin)))
+        return adm::set_error(ADM_ERR_INVALID, "bad activation event");
+      ev[i].member = e.member;
+      ev[i].step = e.step;
+      ev[i].boxes.assign(e.boxes, e.boxes + 6 * (size_t)e.n_boxes);
+      ev[i].cos.assign(e.box_cos, e.box_cos + e.n_boxes);
+      ev[i].sin.assign(e.box_sin, e.box_sin + e.n_boxes);
+      ev[i].new_temperature = e.new_temperature;
+    }
+    return adm::ensemble_adjoint(E, method, t0, dt, n_steps, checkpoint_every,
+                                 T0, n_obs, obs_dof, obs, obs_weight,
+                                 std::move(ev), T_final, J, grad, lambda0,
+                                 static_cast<hipStream_t>(stream));
+  } catch (const AdmError &e) {
+    return adm::set_error(e.code, e.what());
+  } catch (const std::exception &e) {
+    return adm::set_error(ADM_ERR_INVALID, e.what());
+  }
 }
 
 int adm_operator_vjp(void *ptr, double t, const double *T, const double *mu,

@@ -70,6 +70,25 @@ struct ActWork {
   ~ActWork();
 };
 
+// A context's activation geometry: what adm_activate changes besides the
+// temperature and the phase state (the adjoint across an activation goes
+// back to the geometry before it).
+struct GeomSnap {
+  uint32_t *meta = nullptr;
+  double *cos = nullptr, *sin = nullptr;
+  FaceRef *faces = nullptr;
+  int64_t *fn_node = nullptr;
+  int32_t *fn_offset = nullptr, *fn_slot = nullptr;
+  int64_t n_faces = 0, n_fnodes = 0, n_slots = 0;
+  int nz_active = 0;
+  std::vector<uint8_t> active;
+  std::vector<double> dep_cos, dep_sin;
+  GeomSnap() = default;
+  GeomSnap(const GeomSnap &) = delete;
+  GeomSnap &operator=(const GeomSnap &) = delete;
+  ~GeomSnap();
+};
+
 struct Context {
   explicit Context(const adm_problem_desc &d);
   ~Context();
@@ -103,6 +122,17 @@ struct Context {
   bool device_activation_fits() const;
   // faces <- d_faces after a device activation
   void sync_faces_host();
+  // the activation geometry, saved and put back (the inverse mass is the
+  // caller's: it recomputes it)
+  void save_geometry(GeomSnap &g, hipStream_t s) const;
+  void restore_geometry(const GeomSnap &g, hipStream_t s);
+  // adjoint of the last activation, with the geometry after it (the cells
+  // it activated carry kNewlyActive): Tbar = 0 on the nodes it added, and
+  // the carried powder's adjoint through the cell-mean transfer, pbar
+  // [slots][nq] (in: after, out: before) plus the adjoint of the new faces'
+  // powder fpbar [n_faces][nfq] (read)
+  void activation_adjoint(double *Tbar, double *pbar, const double *fpbar,
+                          hipStream_t s);
   CellArgs cell_args() const;
   FaceArgs face_args() const;
   void build_merged_tables();

@@ -310,6 +310,94 @@ def adjoint_bench():
                            "beam_radius"]}
 
 
+def ensemble_adjoint_bench(n_members: int = 32, start: int = 180,
+                           n: int = 50, every: int = 10, loop_sample: int = 4):
+    """The deposition workload's gradients (32 members of the 96x96x40 Q2
+    box, forward Euler 2e-7 s): steps start .. start+n-1 with the members'
+    second-layer depositions that fall in them, every member's misfit
+    gradient from one batched adjoint run (adm_ensemble_adjoint_gradient,
+    checkpoints every `every` steps and at each deposition), against the
+    batched forward run of the same steps and against one-member adjoint
+    runs of `loop_sample` members (the member loop, scaled to all)."""
+    import numpy as np
+    import torch
+    from adamantine_amd.adjoint import ensemble_adjoint_gradient
+    from adamantine_amd.ensemble import EnsembleDevice
+    from adamantine_amd.workloads import deposition_workload
+    w = deposition_workload(n_members=n_members)
+    acts = w.activation_step[:, 1]
+    events = [(j, int(acts[j]) - start, w.layer_boxes[1:2], [1.0], [0.0],
+               300.0) for j in range(n_members) if start <= acts[j] < start + n]
+
+    def at_start(members):
+        ens = EnsembleDevice(w.base, members)
+        ens.compute_inverse_mass_matrix()
+        sol = ens.initialize_dof_vector(300.0)
+        ens.evolve(0.0, w.dt, start, sol)
+        return ens, sol
+    t_start = 0.0
+    for _ in range(start):
+        t_start = t_start + w.dt
+    # observed: nodes of the layer-0 top along the track
+    nn = w.base.n_nodes
+    iz = 2 * 4  # top of cell layer 3, Q2
+    ix = np.arange(10, nn[0] - 10, 2)
+    obs_dof = ix + nn[0] * (nn[1] // 2 + nn[1] * iz)
+    ens, sol = at_start(w.members)
+    obs = sol[:, torch.tensor(obs_dof, device=sol.device)].cpu().numpy() + 5.0
+    # batched forward of the same steps
+    T = sol.clone()
+    torch.cuda.synchronize()
+    f0 = time.perf_counter()
+    t = t_start
+    for k in range(n):
+        for (j, step, boxes, cs, sn, nt) in events:
+            if step == k:
+                ens.activate(j, boxes, cs, sn, nt, T)
+        t = ens.evolve(t, w.dt, 1, T)
+    torch.cuda.synchronize()
+    fwd_s = time.perf_counter() - f0
+    del ens, T
+    torch.cuda.empty_cache()
+    ens, sol = at_start(w.members)
+    torch.cuda.synchronize()
+    g0 = time.perf_counter()
+    res = ensemble_adjoint_gradient(ens, t_start, w.dt, n, sol, obs_dof, obs,
+                                    events=events, checkpoint_every=every)
+    torch.cuda.synchronize()
+    grad_s = time.perf_counter() - g0
+    del ens, sol, res
+    torch.cuda.empty_cache()
+    # the member loop, on a sample of members
+    loop_s = 0.0
+    picks = [j for j in range(n_members) if any(e[0] == j for e in events)]
+    picks = (picks + [j for j in range(n_members) if j not in picks])
+    picks = picks[:loop_sample]
+    for j in picks:
+        e1, s1 = at_start([w.members[j]])
+        ev1 = [(0,) + e[1:] for e in events if e[0] == j]
+        torch.cuda.synchronize()
+        l0 = time.perf_counter()
+        ensemble_adjoint_gradient(e1, t_start, w.dt, n, s1, obs_dof,
+                                  obs[j:j + 1], events=ev1,
+                                  checkpoint_every=every)
+        torch.cuda.synchronize()
+        loop_s += time.perf_counter() - l0
+        del e1, s1
+    loop_all = loop_s / len(picks) * n_members
+    return {"workload": "32 members of the 96x96x40 Q2 box, forward Euler "
+                        "2e-07 s, steps %d..%d (%d second-layer depositions "
+                        "inside), %d observed nodes, checkpoints every %d"
+                        % (start, start + n - 1, len(events), obs_dof.size,
+                           every),
+            "forward_ms": fwd_s * 1e3, "gradient_ms": grad_s * 1e3,
+            "gradient_over_forward": grad_s / fwd_s,
+            "member_loop_gradient_ms": loop_all * 1e3,
+            "speedup_vs_member_loop": loop_all / grad_s,
+            "loop_sample": "%d members, scaled to %d" % (len(picks),
+                                                         n_members)}
+
+
 def deposition_bench(n_members: int = 32, start: int = 180, n: int = 125):
     """The deposition workload (32 members of the 96x96x40 Q2 box, forward
     Euler 2e-7 s): steps start .. start+n-1, in which every member deposits
@@ -498,6 +586,8 @@ def main():
     result["adjoint"] = adjoint_bench()
     torch.cuda.empty_cache()
     result["deposition"] = deposition_bench()
+    torch.cuda.empty_cache()
+    result["ensemble_adjoint"] = ensemble_adjoint_bench()
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(problem, args.cpu_seconds)
     print(json.dumps(result), flush=True)

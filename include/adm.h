@@ -624,6 +624,45 @@ int adm_adjoint_gradient(void *ctx, int method, double t0, double dt,
                          double *T_final, double *J, double *grad,
                          double *lambda0, void *stream);
 
+/* A material activation inside an adjoint window: before step `step`
+ * (0 .. n_steps-1) of the window, member `member` runs adm_activate with
+ * these boxes (host arrays). */
+typedef struct adm_activation_event {
+  int32_t member;
+  int32_t step;
+  int32_t n_boxes;
+  const double *boxes;     /* n_boxes x {min x, y, z, max x, y, z} */
+  const double *box_cos;   /* n_boxes */
+  const double *box_sin;   /* n_boxes */
+  double new_temperature;
+} adm_activation_event;
+
+/* adm_adjoint_gradient for every member of an ensemble at once (no
+ * reference counterpart; the reference's ensemble loop is
+ * application/adamantine.hh:1961-2433): one batched forward sweep (the
+ * members' steps in one launch per stage, activations per member as listed
+ * in events), one batched backward sweep (the transposed cell and face
+ * operators of all members in one launch per stage), checkpoints every
+ * checkpoint_every steps and at every activation step. Across an
+ * activation the adjoint follows add_material: zero on the nodes it added,
+ * the carried powder's adjoint through the cell-mean transfer, and the
+ * geometry (active cells, faces, mass) before it.
+ *   T0, T_final, lambda0  device, n_members x n_dofs (T_final, lambda0 may
+ *                         be NULL)
+ *   obs_dof (host, n_obs): the observed nodes, shared; obs (host, n_members
+ *   x n_obs): each member's data; obs_weight (host, n_obs, or NULL)
+ *   J (host out, n_members): each member's misfit; grad (host out,
+ *   n_members x 3): d J_j / d(member j's absorption efficiency of beam 0,
+ *   its conductivity factor, the shared radius of beam 0).
+ * At return every member is in its final state (geometry and phase state),
+ * as after the forward run. */
+int adm_ensemble_adjoint_gradient(
+    void *ens, int method, double t0, double dt, int n_steps,
+    int checkpoint_every, const double *T0, int32_t n_obs,
+    const int64_t *obs_dof, const double *obs, const double *obs_weight,
+    int32_t n_events, const adm_activation_event *events, double *T_final,
+    double *J, double *grad, double *lambda0, void *stream);
+
 /* The linearised operator by itself (no reference counterpart): with the
  * beams at time t and the context's current phase state, out (device,
  * overwritten) = (dA/dT)^T mu at T, A being adm_vmult's operator; grad
