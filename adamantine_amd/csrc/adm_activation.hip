@@ -305,13 +305,15 @@ __global__ void node_fill_kernel(ActGeom g, const uint32_t *__restrict__ meta,
                                  int64_t *__restrict__ fn_node,
                                  int32_t *__restrict__ fn_offset,
                                  int32_t *__restrict__ fn_slot,
+                                 int32_t *__restrict__ fn_rank,
                                  int64_t n_fnodes) {
   const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (node >= g.n_dofs) return;
   const uint64_t pk = packed[node];
-  if (!(pk >> 32)) return;
   const uint64_t sc = scan[node];
   const int64_t rank = (int64_t)(sc >> 32);
+  fn_rank[node] = (pk >> 32) ? (int32_t)rank : -1;
+  if (!(pk >> 32)) return;
   const int32_t off = (int32_t)(sc & 0xffffffffull);
   fn_node[rank] = node;
   fn_offset[rank] = off;
@@ -415,6 +417,7 @@ GeomSnap::~GeomSnap() {
   (void)hipFree(fn_node);
   (void)hipFree(fn_offset);
   (void)hipFree(fn_slot);
+  (void)hipFree(fn_rank);
 }
 
 void Context::save_geometry(GeomSnap &g, hipStream_t s) const {
@@ -439,6 +442,7 @@ void Context::save_geometry(GeomSnap &g, hipStream_t s) const {
   keep(g.fn_node, d_fn_node, (size_t)n_fnodes);
   keep(g.fn_offset, d_fn_offset, n_fnodes ? (size_t)n_fnodes + 1 : 0);
   keep(g.fn_slot, d_fn_slot, (size_t)n_slots);
+  keep(g.fn_rank, d_fn_rank, (size_t)n_dofs);
   g.n_faces = n_faces;
   g.n_fnodes = n_fnodes;
   g.n_slots = n_slots;
@@ -468,6 +472,7 @@ void Context::restore_geometry(const GeomSnap &g, hipStream_t s) {
   put(d_fn_node, g.fn_node, (size_t)g.n_fnodes);
   put(d_fn_offset, g.fn_offset, g.n_fnodes ? (size_t)g.n_fnodes + 1 : 0);
   put(d_fn_slot, g.fn_slot, (size_t)g.n_slots);
+  put(d_fn_rank, g.fn_rank, (size_t)n_dofs);
   n_faces = g.n_faces;
   n_fnodes = g.n_fnodes;
   nz_active = g.nz_active;
@@ -620,7 +625,7 @@ void Context::activate(int n_boxes, const double *boxes,
                      n_states == 3 ? d_face_powder : nullptr);
   hipLaunchKernelGGL(node_fill_kernel, dim3(bn), dim3(256), 0, s, g, d_meta,
                      w.start, w.packed, w.scan, d_fn_node, d_fn_offset,
-                     d_fn_slot, n_nodes);
+                     d_fn_slot, d_fn_rank, n_nodes);
   ADM_HIP_ACT(hipGetLastError());
   n_faces = nf;
   n_fnodes = n_nodes;

@@ -480,6 +480,21 @@ Context::Context(const adm_problem_desc &d) {
         acc += (long double)kc.Dc[q][k] * (long double)kc.S[k][i];
       kc.DcS[q][i] = (double)acc;
     }
+  // the plane kernel's Q2 form relies on these equalities of the 1-D
+  // tables (they hold for the Gauss and Gauss-Lobatto rules); without them
+  // Q2 goes through the line kernel
+  if (n1 == 3) {
+    const auto &S = kc.S, &D = kc.Dc;
+    const bool sym = S[1][0] == 0.0 && S[1][1] == 1.0 && S[1][2] == 0.0 &&
+                     S[2][0] == S[0][2] && S[2][2] == S[0][0] &&
+                     D[1][1] == 0.0 && D[1][2] == -D[1][0] &&
+                     D[2][0] == -D[0][2] && D[2][1] == -D[0][1] &&
+                     D[2][2] == -D[0][0];
+    if (!sym) {
+      use_plane = false;
+      plane_tables_ok = false;
+    }
+  }
   det = 1.0;
   for (int k = 0; k < 3; ++k) {
     kc.h[k] = m.extent[k] / nc[k];
@@ -623,6 +638,7 @@ Context::Context(const adm_problem_desc &d) {
   ADM_HIP(hipMalloc(&d_cos, sizeof(double) * n_cells));
   ADM_HIP(hipMalloc(&d_sin, sizeof(double) * n_cells));
   ADM_HIP(hipMalloc(&d_wgl, sizeof(double) * kMaxN));
+  ADM_HIP(hipMalloc(&d_fn_rank, sizeof(int32_t) * n_dofs));
   ADM_HIP(hipMemcpy(d_wgl, wgl, sizeof(double) * n1, hipMemcpyHostToDevice));
   if (n_states > 1) {
     ADM_HIP(hipMalloc(&d_liquid, sizeof(double) * n_cells * nq));
@@ -650,6 +666,7 @@ Context::~Context() {
   ADM_FREE(d_fn_node);
   ADM_FREE(d_fn_offset);
   ADM_FREE(d_fn_slot);
+  ADM_FREE(d_fn_rank);
   ADM_FREE(d_beam_table);
   ADM_FREE(d_slot);
   ADM_FREE(d_patch);
@@ -832,6 +849,7 @@ void Context::build_face_nodes() {
   d_fn_slot = nullptr;
   n_fnodes = 0;
   face_out_cap = fn_node_cap = fn_offset_cap = fn_slot_cap = 0;
+  ADM_HIP(hipMemset(d_fn_rank, 0xff, sizeof(int32_t) * n_dofs));
   if (faces.empty()) return;
   const int N = p + 1, NF = N * N;
   const int64_t ns[3] = {1, (int64_t)nn[0], (int64_t)nn[0] * nn[1]};
@@ -875,6 +893,12 @@ void Context::build_face_nodes() {
   fn_node_cap = node.size();
   fn_offset_cap = off.size();
   fn_slot_cap = slot.size();
+  {
+    std::vector<int32_t> rank((size_t)n_dofs, -1);
+    for (size_t k = 0; k < node.size(); ++k) rank[(size_t)node[k]] = (int32_t)k;
+    ADM_HIP(hipMemcpy(d_fn_rank, rank.data(), sizeof(int32_t) * n_dofs,
+                      hipMemcpyHostToDevice));
+  }
   ADM_HIP(hipMemcpy(d_fn_node, node.data(), sizeof(int64_t) * node.size(),
                     hipMemcpyHostToDevice));
   ADM_HIP(hipMemcpy(d_fn_offset, off.data(), sizeof(int32_t) * off.size(),
@@ -1174,6 +1198,7 @@ FaceArgs Context::face_args() const {
   a.fn_offset = d_fn_offset;
   a.fn_slot = d_fn_slot;
   a.n_fnodes = n_fnodes;
+  a.fn_rank = d_fn_rank;
   return a;
 }
 
@@ -1196,7 +1221,37 @@ void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
   const size_t lds = pk == 2 ? sizeof(double) * kMergedLds * kc.mat.n_materials
                              : 0;
   const unsigned M = n_members;
-  if (use_plane && p <= 2) {
+  // the face term's values into its slots (face_out)
+  auto launch_faces = [&]() {
+    if (!fa.n_faces) return;
+#define ADM_DISPATCH(P_, NS_)                                           \
+  if (p == P_ && n_states == NS_) {                                      \
+    if (pk == 0)                                                         \
+      launch_face<P_, NS_, 0>(fa, src, dst, M, s);                       \
+    else if (pk == 1)                                                    \
+      launch_face<P_, NS_, 1>(fa, src, dst, M, s);                       \
+    else                                                                 \
+      launch_face<P_, NS_, 2>(fa, src, dst, M, s);                       \
+  }
+    ADM_DISPATCH(1, 1)
+    ADM_DISPATCH(1, 2)
+    ADM_DISPATCH(1, 3)
+    ADM_DISPATCH(2, 1)
+    ADM_DISPATCH(2, 2)
+    ADM_DISPATCH(2, 3)
+    ADM_DISPATCH(3, 1)
+    ADM_DISPATCH(3, 2)
+    ADM_DISPATCH(3, 3)
+    ADM_DISPATCH(4, 1)
+    ADM_DISPATCH(4, 2)
+    ADM_DISPATCH(4, 3)
+    ADM_DISPATCH(5, 1)
+    ADM_DISPATCH(5, 2)
+    ADM_DISPATCH(5, 3)
+#undef ADM_DISPATCH
+  };
+  const bool plane = use_plane && p <= 2;
+  if (plane) {
     // one launch, one PX x PY cell patch of a cell layer per wavefront
     const size_t plds =
         pk == 2 ? sizeof(double) * kPlaneMergedLds * kc.mat.n_materials : 0;
@@ -1247,20 +1302,22 @@ void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
       ADM_DISPATCH(2, 3)
 #undef ADM_DISPATCH
     }
-    // second pass: every node's sum from the patch buffer into dst
+    // the face term's slots, then the second pass: every node's sum from
+    // the patch buffer and its face slots into dst
+    launch_faces();
     const dim3 ggrid((unsigned)((n_dofs + 255) / 256), M);
     if (p == 1)
       hipLaunchKernelGGL(plane_gather_kernel<1>, ggrid, dim3(256), 0, s,
                          ac.plane_E, ac.plane_E_stride, dst,
                          a.members ? a.member_stride : (int64_t)0, nn[0],
                          nn[1], nn[2], ac.plane_npx, ac.plane_npy, lim[2],
-                         a.add);
+                         a.add, fa);
     else
       hipLaunchKernelGGL(plane_gather_kernel<2>, ggrid, dim3(256), 0, s,
                          ac.plane_E, ac.plane_E_stride, dst,
                          a.members ? a.member_stride : (int64_t)0, nn[0],
                          nn[1], nn[2], ac.plane_npx, ac.plane_npy, lim[2],
-                         a.add);
+                         a.add, fa);
   }
   // the line kernel runs once per parity color (cells of one color share no
   // node: plain read-add-write of dst), or once with atomics
@@ -1310,33 +1367,9 @@ void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
     ADM_DISPATCH(5, 3)
 #undef ADM_DISPATCH
   }
-  if (fa.n_faces) {
-#define ADM_DISPATCH(P_, NS_)                                           \
-  if (p == P_ && n_states == NS_) {                                      \
-    if (pk == 0)                                                         \
-      launch_face<P_, NS_, 0>(fa, src, dst, M, s);                       \
-    else if (pk == 1)                                                    \
-      launch_face<P_, NS_, 1>(fa, src, dst, M, s);                       \
-    else                                                                 \
-      launch_face<P_, NS_, 2>(fa, src, dst, M, s);                       \
-  }
-    ADM_DISPATCH(1, 1)
-    ADM_DISPATCH(1, 2)
-    ADM_DISPATCH(1, 3)
-    ADM_DISPATCH(2, 1)
-    ADM_DISPATCH(2, 2)
-    ADM_DISPATCH(2, 3)
-    ADM_DISPATCH(3, 1)
-    ADM_DISPATCH(3, 2)
-    ADM_DISPATCH(3, 3)
-    ADM_DISPATCH(4, 1)
-    ADM_DISPATCH(4, 2)
-    ADM_DISPATCH(4, 3)
-    ADM_DISPATCH(5, 1)
-    ADM_DISPATCH(5, 2)
-    ADM_DISPATCH(5, 3)
-#undef ADM_DISPATCH
-    if (fa.n_fnodes)
+  if (!plane) {
+    launch_faces();
+    if (fa.n_faces && fa.n_fnodes)
       hipLaunchKernelGGL(face_gather_kernel,
                          dim3((unsigned)((fa.n_fnodes + 255) / 256), M),
                          dim3(256), 0, s, fa, dst);
@@ -1650,6 +1683,7 @@ void Ensemble::refresh_members() {
     m.fn_offset = c.d_fn_offset;
     m.fn_slot = c.d_fn_slot;
     m.n_fnodes = c.n_fnodes;
+    m.fn_rank = c.d_fn_rank;
     max_fnodes = std::max(max_fnodes, m.n_fnodes);
     for (size_t b = 0; b < c.beams.size(); ++b)
       m.beam_gain[b] =
@@ -1870,8 +1904,10 @@ int adm_vmult(void *ptr, const double *src, double *dst, int add,
 }
 
 // checks a launch configuration for a context of FE degree p
-static int check_launch_config(int p, int64_t n_dofs, int32_t mode,
+static int check_launch_config(const adm::Context &c, int32_t mode,
                                int32_t waves) {
+  const int p = c.p;
+  const int64_t n_dofs = c.n_dofs;
   if (waves < 2 || waves > 4)
     return adm::set_error(ADM_ERR_INVALID, "waves_per_block must be 2..4");
   if (mode < ADM_LAUNCH_LINE_ATOMIC || mode > ADM_LAUNCH_PLANE)
@@ -1882,6 +1918,9 @@ static int check_launch_config(int p, int64_t n_dofs, int32_t mode,
   if (mode == ADM_LAUNCH_PLANE && n_dofs >= (int64_t)INT32_MAX)
     return adm::set_error(ADM_ERR_UNSUPPORTED,
                           "the plane kernel takes fewer than 2^31 nodes");
+  if (mode == ADM_LAUNCH_PLANE && !c.plane_tables_ok)
+    return adm::set_error(ADM_ERR_UNSUPPORTED,
+                          "the plane kernel needs the symmetric Q2 tables");
   return ADM_OK;
 }
 
@@ -1907,7 +1946,7 @@ int adm_vmult_noalloc(void *ptr, const double *src, double *dst, int add,
 int adm_set_launch_config(void *ptr, int32_t mode, int32_t waves_per_block) {
   ADM_API_BEGIN
   ADM_CTX(ptr)
-  if (int e = check_launch_config(ctx.p, ctx.n_dofs, mode, waves_per_block))
+  if (int e = check_launch_config(ctx, mode, waves_per_block))
     return e;
   ADM_HIP(hipDeviceSynchronize());
   ctx.set_launch_mode(mode);
@@ -1960,7 +1999,9 @@ static void tune_launch(adm::Context &cfg,
   for (int mi = 0; mi < 3; ++mi)
     for (int w = 2; w <= 4; ++w, ++k) {
       const int mode = modes[mi];
-      if (mode == ADM_LAUNCH_PLANE && (cfg.p > 2 || cfg.n_dofs >= (int64_t)INT32_MAX)) {
+      if (mode == ADM_LAUNCH_PLANE &&
+          (cfg.p > 2 || cfg.n_dofs >= (int64_t)INT32_MAX ||
+           !cfg.plane_tables_ok)) {
         if (ms_out) ms_out[k] = -1.0;
         continue;
       }
@@ -2254,8 +2295,7 @@ int adm_ensemble_set_launch_config(void *ptr, int32_t mode,
                                    int32_t waves_per_block) {
   ADM_API_BEGIN
   ADM_ENS(ptr)
-  if (int e = check_launch_config(ens.members[0]->p, ens.n_dofs, mode,
-                                  waves_per_block))
+  if (int e = check_launch_config(*ens.members[0], mode, waves_per_block))
     return e;
   ADM_HIP(hipDeviceSynchronize());
   ens.members[0]->set_launch_mode(mode);

@@ -78,7 +78,7 @@ struct GeomSnap {
   double *cos = nullptr, *sin = nullptr;
   FaceRef *faces = nullptr;
   int64_t *fn_node = nullptr;
-  int32_t *fn_offset = nullptr, *fn_slot = nullptr;
+  int32_t *fn_offset = nullptr, *fn_slot = nullptr, *fn_rank = nullptr;
   int64_t n_faces = 0, n_fnodes = 0, n_slots = 0;
   int nz_active = 0;
   std::vector<uint8_t> active;
@@ -195,6 +195,7 @@ struct Context {
   int64_t *d_fn_node = nullptr;
   int32_t *d_fn_offset = nullptr;
   int32_t *d_fn_slot = nullptr;
+  int32_t *d_fn_rank = nullptr;  // [n_dofs]: index in fn_node, or -1
 
   std::vector<BeamData> beams;
   std::vector<int> old_segment;
@@ -206,6 +207,8 @@ struct Context {
   // plane kernel (FE degrees 1 and 2; the default there unless
   // ADM_CELL_KERNEL=line is in the environment at creation)
   bool use_plane = false;
+  // the 1-D tables have the symmetries the plane kernel's Q2 form uses
+  bool plane_tables_ok = true;
   // ADM_LAUNCH_* mode of the fields above
   int launch_mode() const {
     return use_plane ? ADM_LAUNCH_PLANE

@@ -123,6 +123,7 @@ struct MemberArgs {
   const int32_t *fn_offset;
   const int32_t *fn_slot;
   int64_t n_fnodes;
+  const int32_t *fn_rank;
 };
 
 // Arguments of the cell kernel (by value).
@@ -195,6 +196,9 @@ struct FaceArgs {
   const int32_t *fn_offset;
   const int32_t *fn_slot;
   int64_t n_fnodes;     // ensemble: the largest member count
+  // per box node, its index in fn_node or -1: the plane kernel's gather
+  // pass adds a node's face slots with its patch sums
+  const int32_t *fn_rank;
 };
 
 // ---------------------------------------------------------------------------

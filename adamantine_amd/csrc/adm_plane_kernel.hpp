@@ -214,16 +214,51 @@ __global__ void __launch_bounds__(256)
   }
   // uniform constants, in scalar registers for the whole patch loop:
   // in-plane shape values, collocation derivative, DcS, geometry and the
-  // kernel arguments the loop reads
+  // kernel arguments the loop reads. For Q2 the host has checked that the
+  // middle Gauss point is the middle node (S row 1 = (0, 1, 0)), that S is
+  // symmetric under reflection except S[2][1] and that Dc is antisymmetric
+  // with Dc[1][1] = 0, bit for bit: S and Dc then take 8 scalar values
+  // instead of 18, and the middle rows' products by 0 and 1 are skipped,
+  // which leaves every sum as the generic chain gives it (x + 0 = x for the
+  // finite temperatures here).
   double S[N][N], Dc[N][N], DS[N][N];
+  if constexpr (P == 2) {
+    const double s00 = in_sgpr(kc.S[0][0]), s01 = in_sgpr(kc.S[0][1]),
+                 s02 = in_sgpr(kc.S[0][2]), s21 = in_sgpr(kc.S[2][1]);
+    const double d00 = in_sgpr(kc.Dc[0][0]), d01 = in_sgpr(kc.Dc[0][1]),
+                 d02 = in_sgpr(kc.Dc[0][2]), d10 = in_sgpr(kc.Dc[1][0]);
+    S[0][0] = s00;
+    S[0][1] = s01;
+    S[0][2] = s02;
+    S[1][0] = -0.0;
+    S[1][1] = 1.0;
+    S[1][2] = 0.0;
+    S[2][0] = s02;
+    S[2][1] = s21;
+    S[2][2] = s00;
+    Dc[0][0] = d00;
+    Dc[0][1] = d01;
+    Dc[0][2] = d02;
+    Dc[1][0] = d10;
+    Dc[1][1] = 0.0;
+    Dc[1][2] = -d10;
+    Dc[2][0] = -d02;
+    Dc[2][1] = -d01;
+    Dc[2][2] = -d00;
 #pragma unroll
-  for (int q = 0; q < N; ++q)
+    for (int q = 0; q < N; ++q)
 #pragma unroll
-    for (int k = 0; k < N; ++k) {
-      S[q][k] = in_sgpr(kc.S[q][k]);
-      Dc[q][k] = in_sgpr(kc.Dc[q][k]);
-      DS[q][k] = in_sgpr(kc.DcS[q][k]);
-    }
+      for (int k = 0; k < N; ++k) DS[q][k] = in_sgpr(kc.DcS[q][k]);
+  } else {
+#pragma unroll
+    for (int q = 0; q < N; ++q)
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        S[q][k] = in_sgpr(kc.S[q][k]);
+        Dc[q][k] = in_sgpr(kc.Dc[q][k]);
+        DS[q][k] = in_sgpr(kc.DcS[q][k]);
+      }
+  }
   const double ih0 = in_sgpr(kc.inv_h[0]), ih1 = in_sgpr(kc.inv_h[1]),
                ih2 = in_sgpr(kc.inv_h[2]);
   const double det = in_sgpr(kc.det);
@@ -349,6 +384,10 @@ __global__ void __launch_bounds__(256)
     for (int yy = 0; yy < N; ++yy)
 #pragma unroll
       for (int qx = 0; qx < N; ++qx) {
+        if (P == 2 && qx == 1) {
+          t[qx + N * yy] = u[1 + N * yy];
+          continue;
+        }
         double acc = S[qx][0] * u[N * yy];
 #pragma unroll
         for (int k = 1; k < N; ++k)
@@ -359,6 +398,10 @@ __global__ void __launch_bounds__(256)
     for (int qy = 0; qy < N; ++qy)
 #pragma unroll
       for (int qx = 0; qx < N; ++qx) {
+        if (P == 2 && qy == 1) {
+          v[qx + N * qy] = t[qx + N];
+          continue;
+        }
         double acc = S[qy][0] * t[qx];
 #pragma unroll
         for (int k = 1; k < N; ++k)
@@ -475,8 +518,10 @@ __global__ void __launch_bounds__(256)
         double gyq = Dc[qy][0] * Tq[qx];
 #pragma unroll
         for (int k = 1; k < N; ++k) {
-          gxq = __builtin_fma(Dc[qx][k], Tq[k + N * qy], gxq);
-          gyq = __builtin_fma(Dc[qy][k], Tq[qx + N * k], gyq);
+          if (!(P == 2 && qx == 1 && k == 1))
+            gxq = __builtin_fma(Dc[qx][k], Tq[k + N * qy], gxq);
+          if (!(P == 2 && qy == 1 && k == 1))
+            gyq = __builtin_fma(Dc[qy][k], Tq[qx + N * k], gyq);
         }
         double r_[NS];
         double l = 0.0;
@@ -569,6 +614,13 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int q = 1; q < N; ++q) {
           t0 = __builtin_fma(DS[q][xx], w0[q], t0);
+          if (P == 2 && q == 1) {
+            if (xx == 1) {
+              t1 = t1 + w1[q];
+              t2 = t2 + w2[q];
+            }
+            continue;
+          }
           t1 = __builtin_fma(S[q][xx], w1[q], t1);
           t2 = __builtin_fma(S[q][xx], w2[q], t2);
         }
@@ -578,6 +630,10 @@ __global__ void __launch_bounds__(256)
           if (qy == 0) {
             Ap[o] = __builtin_fma(DS[qy][yy], t1, S[qy][yy] * t0);
             Bp[o] = S[qy][yy] * t2;
+          } else if (P == 2 && qy == 1) {
+            if (yy == 1) Ap[o] = Ap[o] + t0;
+            Ap[o] = __builtin_fma(DS[qy][yy], t1, Ap[o]);
+            if (yy == 1) Bp[o] = Bp[o] + t2;
           } else {
             Ap[o] = __builtin_fma(S[qy][yy], t0, Ap[o]);
             Ap[o] = __builtin_fma(DS[qy][yy], t1, Ap[o]);
@@ -760,7 +816,8 @@ __global__ void __launch_bounds__(256)
     plane_gather_kernel(const double *__restrict__ E, int64_t E_stride,
                         double *__restrict__ dst, int64_t member_stride,
                         int32_t nn0, int32_t nn1, int32_t nn2, int32_t npx,
-                        int32_t npy, int32_t nz, int32_t add) {
+                        int32_t npy, int32_t nz, int32_t add,
+                        const FaceArgs fa) {
   using L = PlaneLanes<P>;
   constexpr int WX = P * L::PX, WY = P * L::PY;
   const int node = blockIdx.x * blockDim.x + threadIdx.x;
@@ -770,7 +827,10 @@ __global__ void __launch_bounds__(256)
   const int row = node / nn0, X = node - row * nn0;
   const int Z = row / nn1, Y = row - Z * nn1;
   const int64_t rowlen = (int64_t)npx * L::patch_nx;
-  int yoff[2], zoff[2], xoff[2], nyc = 0, nzc = 0, nxc = 0;
+  // offsets of the blocks holding the node (unused entries stay 0: in
+  // bounds of E)
+  int yoff[2] = {0, 0}, zoff[2] = {0, 0}, xoff[2] = {0, 0};
+  int nyc = 0, nzc = 0, nxc = 0;
   const int py = Y / WY, ly = Y - py * WY;
   if (ly == 0 && py > 0) yoff[nyc++] = (py - 1) * L::patch_ny + WY;
   if (py < npy) yoff[nyc++] = py * L::patch_ny + ly;
@@ -780,14 +840,45 @@ __global__ void __launch_bounds__(256)
   const int px = X / WX, lx = X - px * WX;
   if (lx == 0 && px > 0) xoff[nxc++] = (px - 1) * L::patch_nx + WX;
   if (px < npx) xoff[nxc++] = px * L::patch_nx + lx;
-  double sum = 0.0;
-  for (int kz = 0; kz < nzc; ++kz)
-    for (int ky = 0; ky < nyc; ++ky) {
+  // up to 2 x 2 x 2 blocks: every load issued before the sums, which add
+  // the blocks present in (z, y, x) order
+  double v[8];
+#pragma unroll
+  for (int kz = 0; kz < 2; ++kz)
+#pragma unroll
+    for (int ky = 0; ky < 2; ++ky) {
       const double *r =
-          E + ((int64_t)zoff[kz] * npy * L::patch_ny + yoff[ky]) * rowlen;
-      for (int kx = 0; kx < nxc; ++kx) sum += r[xoff[kx]];
+          E + ((int64_t)zoff[kz < nzc ? kz : 0] * npy * L::patch_ny +
+               yoff[ky < nyc ? ky : 0]) *
+                  rowlen;
+#pragma unroll
+      for (int kx = 0; kx < 2; ++kx) {
+        const bool here = kz < nzc && ky < nyc && kx < nxc;
+        v[4 * kz + 2 * ky + kx] = here ? r[xoff[kx < nxc ? kx : 0]] : 0.0;
+      }
     }
-  dst[node] = add ? dst[node] + sum : sum;
+  double sum = 0.0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    if (((k >> 2) < nzc) && (((k >> 1) & 1) < nyc) && ((k & 1) < nxc))
+      sum += v[k];
+  double r = add ? dst[node] + sum : sum;
+  // the face term's slots at this node, summed in increasing face order
+  // and added after the cell term (as a separate face gather pass would)
+  const MemberArgs *mb = fa.members ? fa.members + blockIdx.y : nullptr;
+  const int32_t *rank_p = mb ? mb->fn_rank : fa.fn_rank;
+  if (rank_p && (mb ? mb->n_faces : fa.n_faces) > 0) {
+    const int32_t k = rank_p[node];
+    if (k >= 0) {
+      const double *fo = mb ? mb->face_out : fa.face_out;
+      const int32_t *off = mb ? mb->fn_offset : fa.fn_offset;
+      const int32_t *slot = mb ? mb->fn_slot : fa.fn_slot;
+      double fs = 0.0;
+      for (int32_t e = off[k]; e < off[k + 1]; ++e) fs += fo[slot[e]];
+      r += fs;
+    }
+  }
+  dst[node] = r;
 }
 
 }  // namespace adm
