@@ -26,7 +26,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 #include <memory>
 #include <vector>
 
@@ -210,9 +212,9 @@ __device__ __forceinline__ double beam_value_dR(const adm_beam_state &s,
 
 // (dA/dT)^T mu over the cells: same lane layout as cell_kernel (one x-line
 // per lane, (p+1)^2 lanes per cell, one row segment per wavefront), two
-// fields (T, mu) evaluated, plain read-add-write of the cells of one parity
-// colour per launch (atomic adds when a launch covers every cell),
-// parameter partials reduced per wavefront.
+// fields (T, mu) evaluated, atomic accumulation (or plain read-add-write
+// of the cells of one parity colour per launch), parameter partials
+// reduced per wavefront.
 template <int P, int NS, int PK>
 __global__ void __launch_bounds__(256)
     adjoint_cell_kernel(const AdjArgs aa) {
@@ -797,17 +799,26 @@ inline void launch_vjp(Context &c, const double *Y, const double *pw,
       sizeof(double) * 4 * cpw *
           4 * (c.p + 1) * (c.p + 1) * (c.p + 1);
   const unsigned face_blocks = (unsigned)((fa.n_faces + 255) / 256);
-  // eight launches, one per parity colour: cells of one colour share no
-  // node, so each adds its nodes' terms with a plain read-add-write
+  // one launch with atomic adds, or (ADM_ADJOINT_LAUNCH=colored in the
+  // environment) eight launches, one per parity colour, each adding its
+  // nodes' terms with a plain read-add-write (cells of one colour share no
+  // node). Measured: equal on the 32-member 96x96x40 workload (1.06 s per
+  // 50-step gradient either way), 1.5x slower on one 64x32x16 operator
+  // (the colour launches are too small to fill the chip).
+  bool colored = false;
+  if (const char *e = std::getenv("ADM_ADJOINT_LAUNCH"))
+    colored = std::string(e) == "colored";
+  const int n_launch = colored ? 8 : 1;
   AdjArgs ac[8];
   unsigned blocks[8];
-  for (int col = 0; col < 8; ++col) {
+  for (int col = 0; col < n_launch; ++col) {
     ac[col] = aa;
-    ac[col].c.color = col;
+    ac[col].c.color = colored ? col : -1;
     int64_t cells = 1;
     for (int d = 0; d < 3; ++d) {
       const int b = (col >> d) & 1;
-      ac[col].c.lext[d] = lim[d] > b ? (lim[d] - b + 1) / 2 : 0;
+      ac[col].c.lext[d] =
+          colored ? (lim[d] > b ? (lim[d] - b + 1) / 2 : 0) : lim[d];
       cells *= ac[col].c.lext[d];
     }
     ac[col].c.nseg = (ac[col].c.lext[0] + cpw - 1) / cpw;
@@ -816,7 +827,7 @@ inline void launch_vjp(Context &c, const double *Y, const double *pw,
   }
 #define ADM_ADJ(P_, NS_, PK_)                                              \
   if (c.p == P_ && c.n_states == NS_ && pk == PK_) {                       \
-    for (int col = 0; col < 8; ++col)                                      \
+    for (int col = 0; col < n_launch; ++col)                               \
       if (blocks[col])                                                     \
         hipLaunchKernelGGL((adjoint_cell_kernel<P_, NS_, PK_>),            \
                            dim3(blocks[col], ny), dim3(256), lds, s,       \

@@ -32,7 +32,7 @@ LAUNCH_MODE_NAMES = {0: "line_atomic", 1: "line_colored", 2: "plane"}
 MEASURED_TRAFFIC_BYTES = {
     "line_colored": 355.6e6,   # profiles/hbm_traffic_plain_large.md
     "line_atomic": 176.0e6,    # profiles/hbm_traffic_plain_large.md
-    "plane": 116.8e6,          # profiles/hbm_traffic_plane_plain_large.md
+    "plane": 127.1e6,          # profiles/hbm_traffic_plane_plain_large.md
 }
 
 
@@ -338,11 +338,13 @@ def ensemble_adjoint_bench(n_members: int = 32, start: int = 180,
     t_start = 0.0
     for _ in range(start):
         t_start = t_start + w.dt
-    # observed: nodes of the layer-0 top along the track
+    # observed: 300 nodes of the layer-0 top (3 rows of 100 along the
+    # track), the thermocouple count of the deposition workload
     nn = w.base.n_nodes
     iz = 2 * 4  # top of cell layer 3, Q2
-    ix = np.arange(10, nn[0] - 10, 2)
-    obs_dof = ix + nn[0] * (nn[1] // 2 + nn[1] * iz)
+    ix = np.linspace(10, nn[0] - 11, 100).astype(np.int64)
+    obs_dof = np.concatenate([ix + nn[0] * (nn[1] // 2 + dy + nn[1] * iz)
+                              for dy in (-4, 0, 4)])
     ens, sol = at_start(w.members)
     obs = sol[:, torch.tensor(obs_dof, device=sol.device)].cpu().numpy() + 5.0
     # batched forward of the same steps
