@@ -1213,7 +1213,12 @@ void Context::apply(const double *src, double *dst, int add, hipStream_t s,
 // the arguments carry the member table and the grid's y index is the member.
 void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
                      double *dst, unsigned n_members, hipStream_t s,
-                     int nz_rows, double *patch) const {
+                     int nz_rows, double *patch, const StageArgs *st) const {
+  if (st && !(use_plane && p <= 2))
+    throw AdmError(ADM_ERR_INVALID, "a fused stage needs the plane kernel");
+  StageArgs stage;
+  std::memset(&stage, 0, sizeof(stage));
+  if (st) stage = *st;
   // cell layers above the highest active one hold no active cell
   const int lim[3] = {nc[0], nc[1],
                       nz_rows >= 0 ? std::min(nz_rows, nc[2]) : nc[2]};
@@ -1311,13 +1316,13 @@ void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
                          ac.plane_E, ac.plane_E_stride, dst,
                          a.members ? a.member_stride : (int64_t)0, nn[0],
                          nn[1], nn[2], ac.plane_npx, ac.plane_npy, lim[2],
-                         a.add, fa);
+                         a.add, fa, stage);
     else
       hipLaunchKernelGGL(plane_gather_kernel<2>, ggrid, dim3(256), 0, s,
                          ac.plane_E, ac.plane_E_stride, dst,
                          a.members ? a.member_stride : (int64_t)0, nn[0],
                          nn[1], nn[2], ac.plane_npx, ac.plane_npy, lim[2],
-                         a.add, fa);
+                         a.add, fa, stage);
   }
   // the line kernel runs once per parity color (cells of one color share no
   // node: plain read-add-write of dst), or once with atomics
@@ -1450,21 +1455,48 @@ StepBuffers::~StepBuffers() {
 // into the cleared residual r; the vector kernels then apply the inverse
 // mass and deal.II's stage combinations (forward Euler as sadd; classic RK4
 // and RK3 as TimeStepping::ExplicitRungeKutta orders them).
-template <class Rhs>
+//
+// fused(t_stage, x, st), when given, runs the operator on x and applies the
+// stage update st in the plane path's gather pass (the same arithmetic as
+// the stage kernels, without the residual vector's round trip).
+template <class Rhs, class Fused>
 static void explicit_step(int method, double t, double dt, double *T,
-                          StepBuffers &b, Rhs &&rhs, hipStream_t s) {
+                          StepBuffers &b, Rhs &&rhs, Fused &&fused,
+                          bool use_fused, hipStream_t s) {
   const int block = 256;
   const int64_t n = b.n;
   const unsigned grid = (unsigned)((n + block - 1) / block);
+  StageArgs st;
+  std::memset(&st, 0, sizeof(st));
+  st.y = T;
+  st.inv_mass = b.inv_mass;
+  st.n = n;
   if (method == ADM_FORWARD_EULER) {
-    rhs(t, T, b.R);
-    hipLaunchKernelGGL(fe_update_kernel, dim3(grid), dim3(block), 0, s, T, b.R,
-                       b.inv_mass, dt, n);
+    if (use_fused) {
+      st.kind = kStageFE;
+      st.c0 = dt;
+      fused(t, T, st);
+    } else {
+      rhs(t, T, b.R);
+      hipLaunchKernelGGL(fe_update_kernel, dim3(grid), dim3(block), 0, s, T,
+                         b.R, b.inv_mass, dt, n);
+    }
   } else if (method == ADM_RK_FOURTH_ORDER) {
     static const double bw[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
     static const double c[4] = {0.0, 0.5, 0.5, 1.0};
     static const double anext[4] = {0.5, 0.5, 1.0, 0.0};
     for (int i = 0; i < 4; ++i) {
+      if (use_fused) {
+        st.kind = kStageRK4;
+        st.acc = b.acc;
+        st.Y = b.Y;
+        st.c0 = dt * bw[i];
+        st.c1 = dt * anext[i];
+        st.first = i == 0 ? 1 : 0;
+        st.last = i == 3 ? 1 : 0;
+        fused(t + c[i] * dt, i == 0 ? T : b.Y, st);
+        continue;
+      }
       rhs(t + c[i] * dt, i == 0 ? T : b.Y, b.R);
       hipLaunchKernelGGL(rk4_stage_kernel, dim3(grid), dim3(block), 0, s, T,
                          b.R, b.inv_mass, b.acc, b.Y, dt * bw[i],
@@ -1475,8 +1507,21 @@ static void explicit_step(int method, double t, double dt, double *T,
     static const double bw[3] = {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0};
     static const double c[3] = {0.0, 0.5, 1.0};
     for (int i = 0; i < 3; ++i) {
-      rhs(t + c[i] * dt, i == 0 ? T : b.Y, b.R);
       const double *an = i + 1 < 3 ? a[i + 1] : a[0];
+      if (use_fused) {
+        st.kind = kStageRK;
+        st.F = b.F;
+        st.Y = b.Y;
+        st.stage = i;
+        st.n_stages = 3;
+        for (int j = 0; j < 3; ++j) {
+          st.a[j] = dt * an[j];
+          st.b[j] = dt * bw[j];
+        }
+        fused(t + c[i] * dt, i == 0 ? T : b.Y, st);
+        continue;
+      }
+      rhs(t + c[i] * dt, i == 0 ? T : b.Y, b.R);
       hipLaunchKernelGGL(rk_stage_kernel, dim3(grid), dim3(block), 0, s, T,
                          b.R, b.inv_mass, b.F, i, 3, dt * an[0], dt * an[1],
                          dt * an[2], b.Y, dt * bw[0], dt * bw[1], dt * bw[2],
@@ -1502,7 +1547,13 @@ void Context::step(int method, double t, double dt, double *T, hipStream_t s) {
         set_time(ts);
         apply(x, r, 0, s);
       },
-      s);
+      [&](double ts, const double *x, const StageArgs &st) {
+        set_time(ts);
+        CellArgs a = cell_args();
+        a.add = 0;
+        launch(a, face_args(), x, nullptr, 1, s, nz_active, nullptr, &st);
+      },
+      plane_path(), s);
 }
 
 // Stage time fractions of explicit_step's methods.
@@ -1581,7 +1632,7 @@ void Context::evolve_graph(int method, double t, double dt, int n_steps,
           hipLaunchKernelGGL(advance_slot_kernel, dim3(1), dim3(64), 0, cs,
                              d_slot);
         },
-        cs);
+        [](double, const double *, const StageArgs &) {}, false, cs);
   } catch (...) {
     // end the capture before the stream goes away
     if (hipStreamEndCapture(cs, &graph) == hipSuccess && graph)
@@ -1741,7 +1792,8 @@ void Ensemble::set_time(double t) {
   }
 }
 
-void Ensemble::apply(const double *src, double *dst, int add, hipStream_t s) {
+void Ensemble::apply(const double *src, double *dst, int add, hipStream_t s,
+                     const StageArgs *st) {
   Context &c0 = *members[0];
   CellArgs a = c0.cell_args();
   a.add = add;
@@ -1758,7 +1810,7 @@ void Ensemble::apply(const double *src, double *dst, int add, hipStream_t s) {
   fa.n_fnodes = max_fnodes;
   int nz = 0;
   for (const auto &m : members) nz = std::max(nz, m->nz_active);
-  c0.launch(a, fa, src, dst, (unsigned)n_members, s, nz);
+  c0.launch(a, fa, src, dst, (unsigned)n_members, s, nz, nullptr, st);
   ADM_HIP(hipGetLastError());
 }
 
@@ -1784,7 +1836,11 @@ void Ensemble::step(int method, double t, double dt, double *T,
         set_time(ts);
         apply(x, r, 0, s);
       },
-      s);
+      [&](double ts, const double *x, const StageArgs &st) {
+        set_time(ts);
+        apply(x, nullptr, 0, s, &st);
+      },
+      members[0]->plane_path(), s);
 }
 
 }  // namespace adm

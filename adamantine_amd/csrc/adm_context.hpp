@@ -143,9 +143,12 @@ struct Context {
              double *patch = nullptr);
   // nz_rows: cell layers (from z = 0) that can hold an active cell; the
   // launches cover only those (-1: all)
+  // st: a stage update the plane path's gather pass applies instead of
+  // writing dst (dst unused then)
   void launch(const CellArgs &a, const FaceArgs &fa, const double *src,
               double *dst, unsigned n_members, hipStream_t s,
-              int nz_rows = -1, double *patch = nullptr) const;
+              int nz_rows = -1, double *patch = nullptr,
+              const StageArgs *st = nullptr) const;
   void fill(double *x, double v, int64_t n, hipStream_t s);
   // the plane kernel's patch buffer, grown to n doubles if smaller
   double *ensure_patch_buffer(size_t n) const;
@@ -268,7 +271,8 @@ struct Ensemble {
   Ensemble &operator=(const Ensemble &) = delete;
 
   void set_time(double t);
-  void apply(const double *src, double *dst, int add, hipStream_t s);
+  void apply(const double *src, double *dst, int add, hipStream_t s,
+             const StageArgs *st = nullptr);
   void compute_inverse_mass(double *out, hipStream_t s);
   void ensure_step_buffers(int method);
   void step(int method, double t, double dt, double *T, hipStream_t s);

@@ -558,4 +558,88 @@ __host__ __device__ __forceinline__ bool beam_may_touch(
   return !(d2 > 5.0 * s.radius_squared * (1.0 + 1e-6));
 }
 
+// ---------------------------------------------------------------------------
+// explicit-stepper stage updates (deal.II's stage and accumulation order),
+// applied to the operator result r of entry i: by the stand-alone step
+// kernels (r read from the residual vector) and by the plane path's gather
+// pass (r as it is formed, without storing it)
+// ---------------------------------------------------------------------------
+
+// Forward Euler (source/ThermalPhysics.templates.hh:871-876): y = 1*y + dt*f
+__device__ __forceinline__ void fe_stage(double *__restrict__ y,
+                                         const double *__restrict__ inv_mass,
+                                         double dt, int64_t i, double r) {
+  const double f = r * inv_mass[i];
+  y[i] = 1. * y[i] + dt * f;
+}
+
+// Classic RK4 stage, f_i = r*Minv: acc = acc + (dt b_i) f_i (acc starts as
+// y); next stage input Y = y + (dt a_{i+1,i}) f_i (the other a_{i+1,j} of
+// RK4 are zero, adding +-0); the last stage writes y = acc.
+__device__ __forceinline__ void rk4_stage(double *__restrict__ y,
+                                          const double *__restrict__ inv_mass,
+                                          double *__restrict__ acc,
+                                          double *__restrict__ Y, double bcoef,
+                                          double acoef, int first, int last,
+                                          int64_t i, double r) {
+  const double f = r * inv_mass[i];
+  const double yi = y[i];
+  double s = first ? yi : acc[i];
+  s = 1. * s + bcoef * f;
+  if (last) {
+    y[i] = s;
+  } else {
+    acc[i] = s;
+    Y[i] = 1. * yi + acoef * f;
+  }
+}
+
+// Generic explicit RK stage (RK3, whose a_{3,1} != 0): stores f_i in
+// F[stage][n] and forms the next stage input from all stored stages, or
+// the new y after the last.
+__device__ __forceinline__ void rk_stage(double *__restrict__ y,
+                                         const double *__restrict__ inv_mass,
+                                         double *__restrict__ F, int stage,
+                                         int n_stages, const double *a,
+                                         double *__restrict__ Y,
+                                         const double *b, int64_t n, int64_t i,
+                                         double r) {
+  const double f = r * inv_mass[i];
+  F[(int64_t)stage * n + i] = f;
+  if (stage + 1 < n_stages) {
+    double s = y[i];
+    for (int j = 0; j <= stage; ++j) s = 1. * s + a[j] * F[(int64_t)j * n + i];
+    Y[i] = s;
+  } else {
+    double s = y[i];
+    for (int j = 0; j < n_stages; ++j) s = 1. * s + b[j] * F[(int64_t)j * n + i];
+    y[i] = s;
+  }
+}
+
+// A stage update for the gather pass to apply instead of writing dst.
+struct StageArgs {
+  int32_t kind;  // kStageNone, kStageFE, kStageRK4, kStageRK
+  int32_t first, last, stage, n_stages;
+  double *y;
+  const double *inv_mass;
+  double *acc, *Y, *F;
+  double c0, c1;      // FE: dt; RK4: dt b_i, dt a_{i+1,i}
+  double a[3], b[3];  // generic stage: dt a_{i+1,j}, dt b_j
+  int64_t n;          // entries of y (every member)
+};
+constexpr int32_t kStageNone = 0, kStageFE = 1, kStageRK4 = 2, kStageRK = 3;
+
+__device__ __forceinline__ void stage_apply(const StageArgs &st, int64_t i,
+                                            double r) {
+  if (st.kind == kStageFE)
+    fe_stage(st.y, st.inv_mass, st.c0, i, r);
+  else if (st.kind == kStageRK4)
+    rk4_stage(st.y, st.inv_mass, st.acc, st.Y, st.c0, st.c1, st.first,
+              st.last, i, r);
+  else
+    rk_stage(st.y, st.inv_mass, st.F, st.stage, st.n_stages, st.a, st.Y, st.b,
+             st.n, i, r);
+}
+
 }  // namespace adm

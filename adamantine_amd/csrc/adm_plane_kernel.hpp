@@ -817,7 +817,7 @@ __global__ void __launch_bounds__(256)
                         double *__restrict__ dst, int64_t member_stride,
                         int32_t nn0, int32_t nn1, int32_t nn2, int32_t npx,
                         int32_t npy, int32_t nz, int32_t add,
-                        const FaceArgs fa) {
+                        const FaceArgs fa, const StageArgs st) {
   using L = PlaneLanes<P>;
   constexpr int WX = P * L::PX, WY = P * L::PY;
   const int node = blockIdx.x * blockDim.x + threadIdx.x;
@@ -878,7 +878,12 @@ __global__ void __launch_bounds__(256)
       r += fs;
     }
   }
-  dst[node] = r;
+  // the result, or (a time step on the plane path) the stage update it
+  // feeds, without storing it
+  if (st.kind == kStageNone)
+    dst[node] = r;
+  else
+    stage_apply(st, (int64_t)blockIdx.y * member_stride + node, r);
 }
 
 }  // namespace adm

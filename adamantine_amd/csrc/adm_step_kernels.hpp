@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "adm_device.hpp"
+
 namespace adm {
 
 // Stage counter of a captured step loop: set before the replays, advanced
@@ -48,22 +50,19 @@ __global__ void scatter_value_kernel(double *__restrict__ T,
   if (k < n) T[idx[k]] = v;
 }
 
-// Forward Euler (source/ThermalPhysics.templates.hh:871-876):
-// T = 1*T + dt*(R*Minv); the residual buffer is cleared for the next stage.
+// Forward Euler (source/ThermalPhysics.templates.hh:871-876) on the
+// residual R; R is cleared for the next stage.
 __global__ void fe_update_kernel(double *__restrict__ T, double *__restrict__ R,
                                  const double *__restrict__ inv_mass, double dt,
                                  int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double f = R[i] * inv_mass[i];
-  T[i] = 1. * T[i] + dt * f;
+  fe_stage(T, inv_mass, dt, i, R[i]);
   R[i] = 0.0;
 }
 
-// Classic RK4 stage of deal.II's ExplicitRungeKutta with the stage vector
-// f_i = R*Minv: acc = acc + (dt b_i) f_i (acc starts as y); next stage input
-// Y = y + (dt a_{i+1,i}) f_i (the other a_{i+1,j} of RK4 are zero, adding
-// +-0); the last stage writes y = acc.
+// Classic RK4 stage of deal.II's ExplicitRungeKutta (rk4_stage) on the
+// residual R; R is cleared for the next stage.
 __global__ void rk4_stage_kernel(double *__restrict__ y, double *__restrict__ R,
                                  const double *__restrict__ inv_mass,
                                  double *__restrict__ acc,
@@ -72,21 +71,12 @@ __global__ void rk4_stage_kernel(double *__restrict__ y, double *__restrict__ R,
                                  int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double f = R[i] * inv_mass[i];
-  const double yi = y[i];
-  double s = first ? yi : acc[i];
-  s = 1. * s + bcoef * f;
-  if (last) {
-    y[i] = s;
-  } else {
-    acc[i] = s;
-    Y[i] = 1. * yi + acoef * f;
-  }
+  const double r = R[i];
   R[i] = 0.0;
+  rk4_stage(y, inv_mass, acc, Y, bcoef, acoef, first, last, i, r);
 }
 
-// Generic explicit RK stage (used for RK3, whose a_{3,1} != 0): stores
-// f_i and forms the next stage input from all stored stages.
+// Generic explicit RK stage (rk_stage, used for RK3) on the residual R.
 __global__ void rk_stage_kernel(double *__restrict__ y, double *__restrict__ R,
                                 const double *__restrict__ inv_mass,
                                 double *__restrict__ F, int stage, int n_stages,
@@ -95,20 +85,10 @@ __global__ void rk_stage_kernel(double *__restrict__ y, double *__restrict__ R,
                                 double b2, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double f = R[i] * inv_mass[i];
-  F[(int64_t)stage * n + i] = f;
+  const double r = R[i];
   R[i] = 0.0;
-  if (stage + 1 < n_stages) {
-    double s = y[i];
-    const double a[3] = {a0, a1, a2};
-    for (int j = 0; j <= stage; ++j) s = 1. * s + a[j] * F[(int64_t)j * n + i];
-    Y[i] = s;
-  } else {
-    const double b[3] = {b0, b1, b2};
-    double s = y[i];
-    for (int j = 0; j < n_stages; ++j) s = 1. * s + b[j] * F[(int64_t)j * n + i];
-    y[i] = s;
-  }
+  const double a[3] = {a0, a1, a2}, b[3] = {b0, b1, b2};
+  rk_stage(y, inv_mass, F, stage, n_stages, a, Y, b, n, i, r);
 }
 
 }  // namespace adm
