@@ -826,6 +826,13 @@ __global__ void __launch_bounds__(256)
   dst += (int64_t)blockIdx.y * member_stride;
   const int row = node / nn0, X = node - row * nn0;
   const int Z = row / nn1, Y = row - Z * nn1;
+  // above the top node plane of the launch's cell layers no cell or face
+  // adds a term: the result is 0 (or dst as it is), and a stage update
+  // leaves every vector as it is
+  if (Z > P * nz) {
+    if (st.kind == kStageNone && !add) dst[node] = 0.0;
+    return;
+  }
   const int64_t rowlen = (int64_t)npx * L::patch_nx;
   // offsets of the blocks holding the node (unused entries stay 0: in
   // bounds of E)
