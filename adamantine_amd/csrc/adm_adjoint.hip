@@ -35,6 +35,7 @@
 #include "../../include/adm.h"
 #include "adm_context.hpp"
 #include "adm_kernels.hpp"
+#include "adm_plane_kernel.hpp"
 
 namespace adm {
 
@@ -77,16 +78,23 @@ struct Mix {
   double r[NS], rT[NS], rp[NS];
 };
 
-// value, T- and powder-partials of a mixed property from the merged tables
-template <int NS>
-__device__ __forceinline__ void merged_vd(const double *B, const double *C,
-                                          double T, int prop,
-                                          const Mix<NS> &x, double &v,
-                                          double &vT, double &vp) {
+// index of the merged-table interval holding T: #{B_k <= T} (binary
+// search over the padded breakpoints)
+__device__ __forceinline__ int merged_search(const double *B, double T) {
   int k = 0;
 #pragma unroll
   for (int step = kMergedMax / 2; step >= 1; step >>= 1)
     k += (B[k + step - 1] <= T) ? step : 0;
+  return k;
+}
+
+// value, T- and powder-partials of a mixed property from the merged tables,
+// in interval k of T
+template <int NS>
+__device__ __forceinline__ void merged_vd(const double *B, const double *C,
+                                          double T, int k, int prop,
+                                          const Mix<NS> &x, double &v,
+                                          double &vT, double &vp) {
   const double d = T - B[k > 0 ? k - 1 : 0];
   const double *c = C + ((prop * kMergedMax + k) * ADM_MAX_STATES) * 2;
   v = 0.0;
@@ -165,13 +173,15 @@ __device__ __forceinline__ void ratios_lin(double T, double p_old, double sol,
   }
 }
 
+// k: the merged-table interval of T (merged tables only)
 template <int NS, int PK>
 __device__ __forceinline__ void prop_vd(const MatConst &m, int mid,
                                         const double *mB, const double *mC,
-                                        double T, int prop, const Mix<NS> &x,
-                                        double &v, double &vT, double &vp) {
+                                        double T, int k, int prop,
+                                        const Mix<NS> &x, double &v,
+                                        double &vT, double &vp) {
   if constexpr (PK == 2)
-    merged_vd<NS>(mB, mC, T, prop, x, v, vT, vp);
+    merged_vd<NS>(mB, mC, T, k, prop, x, v, vT, vp);
   else
     poly_vd<NS>(m, mid, T, prop, x, v, vT, vp);
 }
@@ -182,12 +192,12 @@ template <int NS, int PK>
 __device__ __forceinline__ void inv_rho_cp_lin(const MatConst &m, int mid,
                                                const double *mB,
                                                const double *mC, double T,
-                                               double latent,
+                                               int k, double latent,
                                                const Mix<NS> &x, double &irc,
                                                double &ircT, double &ircp) {
   double rho, rhoT, rhop, cp, cpT, cpp;
-  prop_vd<NS, PK>(m, mid, mB, mC, T, ADM_DENSITY, x, rho, rhoT, rhop);
-  prop_vd<NS, PK>(m, mid, mB, mC, T, ADM_SPECIFIC_HEAT, x, cp, cpT, cpp);
+  prop_vd<NS, PK>(m, mid, mB, mC, T, k, ADM_DENSITY, x, rho, rhoT, rhop);
+  prop_vd<NS, PK>(m, mid, mB, mC, T, k, ADM_SPECIFIC_HEAT, x, cp, cpT, cpp);
   if constexpr (NS > 1)
     if (x.r[1] * x.r[0] > 0.0) cp += latent;
   irc = 1.0 / (rho * cp);
@@ -248,10 +258,12 @@ __global__ void __launch_bounds__(256)
       reinterpret_cast<uint64_t *>(sbeams)[tid] =
           reinterpret_cast<const uint64_t *>(a.beams)[tid];
     if constexpr (PK == 2) {
-      const int total = n_mat * kMergedLds;
+      const int total = n_mat * kPlaneMergedLds;
       for (int i = tid; i < total; i += blockDim.x) {
-        const int mm = i / kMergedLds, k = i % kMergedLds;
-        lds[i] = (&m.merged[mm].B[0])[k];
+        const int mm = i / kPlaneMergedLds, k = i % kPlaneMergedLds;
+        lds[i] = k < kMergedLds ? (&m.merged[mm].B[0])[k]
+                                : reinterpret_cast<const double *>(
+                                      &m.bins[mm])[k - kMergedLds];
       }
     }
     __syncthreads();
@@ -265,7 +277,7 @@ __global__ void __launch_bounds__(256)
   const int j = lane / NL, L = lane % NL;
   const int la = L % N, lb = L / N;
   const int own = N * la + NL * lb;
-  double *xb = lds + (PK == 2 ? n_mat * kMergedLds : 0) +
+  double *xb = lds + (PK == 2 ? n_mat * kPlaneMergedLds : 0) +
                (wave * CPW + (j < CPW ? j : 0)) * (4 * N3);
   double *xb2 = xb + N3, *xb3 = xb + 2 * N3, *xb4 = xb + 3 * N3;
   const int r = blockIdx.x * 4 + wave;
@@ -388,8 +400,11 @@ __global__ void __launch_bounds__(256)
       cs = dcos_p[c];
       sn = dsin_p[c];
     }
-    const double *mB = lds + mid * kMergedLds;
+    const double *mB = lds + mid * kPlaneMergedLds;
     const double *mC = mB + kMergedMax;
+    const MergedBins &mbin =
+        *reinterpret_cast<const MergedBins *>(mB + kMergedLds);
+    const bool iso = m.isotropic[mid] != 0;
     const double py = kc.origin[1] + (rc.cc[1] + kc.xq[la]) * kc.h[1];
     const double pz = kc.origin[2] + (rc.cc[2] + kc.xq[lb]) * kc.h[2];
     double cT[N], cw[3][N];
@@ -401,14 +416,23 @@ __global__ void __launch_bounds__(256)
       bool clamped;
       ratios_lin<NS>(T, pw ? pw[q] : 0.0, sol, liq, x, dl, clamped);
       double irc, ircT, ircp;
-      inv_rho_cp_lin<NS, PK>(m, mid, mB, mC, T, latent, x, irc, ircT, ircp);
+      // the merged-table interval of T, once for every property
+      const int kt = PK == 2 ? merged_bin_interval(mB, mbin, T) : 0;
+      inv_rho_cp_lin<NS, PK>(m, mid, mB, mC, T, kt, latent, x, irc, ircT,
+                             ircp);
       double kx, kxT, kxp, ky, kyT, kyp, kz, kzT, kzp;
-      prop_vd<NS, PK>(m, mid, mB, mC, T, ADM_THERMAL_CONDUCTIVITY_X, x, kx,
-                      kxT, kxp);
-      prop_vd<NS, PK>(m, mid, mB, mC, T, ADM_THERMAL_CONDUCTIVITY_Y, x, ky,
-                      kyT, kyp);
-      prop_vd<NS, PK>(m, mid, mB, mC, T, ADM_THERMAL_CONDUCTIVITY_Z, x, kz,
-                      kzT, kzp);
+      prop_vd<NS, PK>(m, mid, mB, mC, T, kt, ADM_THERMAL_CONDUCTIVITY_X, x,
+                      kx, kxT, kxp);
+      if (iso) {
+        ky = kz = kx;
+        kyT = kzT = kxT;
+        kyp = kzp = kxp;
+      } else {
+        prop_vd<NS, PK>(m, mid, mB, mC, T, kt, ADM_THERMAL_CONDUCTIVITY_Y, x,
+                        ky, kyT, kyp);
+        prop_vd<NS, PK>(m, mid, mB, mC, T, kt, ADM_THERMAL_CONDUCTIVITY_Z, x,
+                        kz, kzT, kzp);
+      }
       const double g0 = G[0][0][q], g1 = G[0][1][q], g2 = G[0][2][q];
       const double m0 = G[1][0][q], m1 = G[1][1][q], m2 = G[1][2][q];
       // mu-gradient . K g for the conductivities (kx, ky, kz) with the x-y
@@ -609,14 +633,16 @@ __global__ void __launch_bounds__(256)
     bool clamped;
     ratios_lin<NS>(Tq, fp ? fp[q] : 0.0, sol, liq, x, dl, clamped);
     double irc, ircT, ircp;
-    inv_rho_cp_lin<NS, PK>(m, mid, mB, mC, Tq, latent, x, irc, ircT, ircp);
+    const int kt = PK == 2 ? merged_search(mB, Tq) : 0;
+    inv_rho_cp_lin<NS, PK>(m, mid, mB, mC, Tq, kt, latent, x, irc, ircT,
+                           ircp);
     // s = h (T - T_conv) + eps sigma (T + T_rad)(T^2 + T_rad^2)(T - T_rad)
     double s1 = 0.0, s1T = 0.0, s1p = 0.0;
     if (bt & ADM_BC_CONVECTIVE) {
       const double cT = m.prop[mid][ADM_CONVECTION_TEMPERATURE_INFTY];
       double h, hT, hp;
-      prop_vd<NS, PK>(m, mid, mB, mC, Tq, ADM_CONVECTION_HEAT_TRANSFER_COEF, x,
-                      h, hT, hp);
+      prop_vd<NS, PK>(m, mid, mB, mC, Tq, kt,
+                      ADM_CONVECTION_HEAT_TRANSFER_COEF, x, h, hT, hp);
       s1 += h * (Tq - cT);
       s1T += hT * (Tq - cT) + h;
       s1p += hp * (Tq - cT);
@@ -624,7 +650,7 @@ __global__ void __launch_bounds__(256)
     if (bt & ADM_BC_RADIATIVE) {
       const double rT = m.prop[mid][ADM_RADIATION_TEMPERATURE_INFTY];
       double e, eT, ep;
-      prop_vd<NS, PK>(m, mid, mB, mC, Tq, ADM_EMISSIVITY, x, e, eT, ep);
+      prop_vd<NS, PK>(m, mid, mB, mC, Tq, kt, ADM_EMISSIVITY, x, e, eT, ep);
       const double a1 = Tq + rT, a2 = Tq * Tq + rT * rT;
       const double g = kStefanBoltzmann * a1 * a2 * (Tq - rT);
       const double gT = kStefanBoltzmann *
@@ -795,7 +821,8 @@ inline void launch_vjp(Context &c, const double *Y, const double *pw,
   const int nz = batch ? batch->nz_active : c.nz_active;
   const int lim[3] = {c.nc[0], c.nc[1], std::min(nz, c.nc[2])};
   const size_t lds =
-      (pk == 2 ? sizeof(double) * kMergedLds * c.kc.mat.n_materials : 0) +
+      (pk == 2 ? sizeof(double) * kPlaneMergedLds * c.kc.mat.n_materials
+               : 0) +
       sizeof(double) * 4 * cpw *
           4 * (c.p + 1) * (c.p + 1) * (c.p + 1);
   const unsigned face_blocks = (unsigned)((fa.n_faces + 255) / 256);
