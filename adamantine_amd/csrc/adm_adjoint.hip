@@ -1467,6 +1467,9 @@ int adm_adjoint_gradient(void *ptr, int method, double t0, double dt,
   try {
     if (!ptr) return adm::set_error(ADM_ERR_STATE, "null context");
     adm::Context &c = *static_cast<adm::Context *>(ptr);
+    if (c.dim != 3)
+      return adm::set_error(ADM_ERR_UNSUPPORTED,
+                            "the adjoint is three-dimensional");
     if (!T0 || !obs_dof || !obs || !J || !grad || n_obs < 1 || n_steps < 1)
       return adm::set_error(ADM_ERR_INVALID, "bad adjoint arguments");
     for (int k = 0; k < n_obs; ++k)
@@ -1634,6 +1637,9 @@ int adm_ensemble_adjoint_gradient(
   try {
     if (!ptr) return adm::set_error(ADM_ERR_STATE, "null ensemble");
     adm::Ensemble &E = *static_cast<adm::Ensemble *>(ptr);
+    if (E.members.empty() || E.members[0]->dim != 3)
+      return adm::set_error(ADM_ERR_UNSUPPORTED,
+                            "the adjoint is three-dimensional");
     if (!T0 || !obs_dof || !obs || !J || !grad || n_obs < 1 || n_steps < 1 ||
         n_events < 0 || (n_events && !events))
       return adm::set_error(ADM_ERR_INVALID, "bad adjoint arguments");
@@ -1670,6 +1676,9 @@ int adm_operator_vjp(void *ptr, double t, const double *T, const double *mu,
   try {
     if (!ptr) return adm::set_error(ADM_ERR_STATE, "null context");
     adm::Context &c = *static_cast<adm::Context *>(ptr);
+    if (c.dim != 3)
+      return adm::set_error(ADM_ERR_UNSUPPORTED,
+                            "the adjoint is three-dimensional");
     if (!T || !mu || !out)
       return adm::set_error(ADM_ERR_INVALID, "null vector");
     hipStream_t s = static_cast<hipStream_t>(stream);

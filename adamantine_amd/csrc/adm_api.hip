@@ -20,6 +20,7 @@
 #include "../../include/adm.h"
 #include "adm_context.hpp"
 #include "adm_kernels.hpp"
+#include "adm_kernels_2d.hpp"
 #include "adm_launch.hpp"
 #include "adm_plane_kernel.hpp"
 #include "adm_step_kernels.hpp"
@@ -417,16 +418,20 @@ Context::Context(const adm_problem_desc &d) {
       n_cu = cus;
   }
   const adm_mesh_desc &m = d.mesh;
-  if (m.dim != 3)
-    throw AdmError(ADM_ERR_UNSUPPORTED, "the device path is three-dimensional");
+  if (m.dim != 2 && m.dim != 3)
+    throw AdmError(ADM_ERR_INVALID, "dim must be 2 or 3");
+  dim = m.dim;
+  // the plane kernel is three-dimensional
+  if (dim == 2) use_plane = false;
   if (m.fe_degree < 1 || m.fe_degree > 5)
     throw AdmError(ADM_ERR_UNSUPPORTED,
                    "fe_degree must be 1 to 5 on the device path");
   if (m.quadrature != 0 && m.quadrature != 1)
     throw AdmError(ADM_ERR_INVALID, "quadrature must be 0 (Gauss) or 1 "
                                     "(Gauss-Lobatto)");
+  // dim = 2 uses axes 0 (length) and 2 (height) of the descriptor
   for (int k = 0; k < 3; ++k)
-    if (m.n_cells[k] < 1 || !(m.extent[k] > 0.0))
+    if ((dim == 3 || k != 1) && (m.n_cells[k] < 1 || !(m.extent[k] > 0.0)))
       throw AdmError(ADM_ERR_INVALID, "invalid mesh size");
   const adm_material_desc &mat = d.material;
   if (mat.n_materials < 1 || mat.n_materials > ADM_MAX_MATERIALS)
@@ -442,13 +447,14 @@ Context::Context(const adm_problem_desc &d) {
 
   p = m.fe_degree;
   n1 = p + 1;
-  nq = n1 * n1 * n1;
-  nfq = n1 * n1;
+  nq = dim == 3 ? n1 * n1 * n1 : n1 * n1;
+  nfq = dim == 3 ? n1 * n1 : n1;
   n_cells = 1;
   n_dofs = 1;
   for (int k = 0; k < 3; ++k) {
-    nc[k] = m.n_cells[k];
-    nn[k] = p * nc[k] + 1;
+    const bool flat = dim == 2 && k == 1;
+    nc[k] = flat ? 1 : m.n_cells[k];
+    nn[k] = flat ? 1 : p * nc[k] + 1;
     n_cells *= nc[k];
     n_dofs *= nn[k];
   }
@@ -497,6 +503,11 @@ Context::Context(const adm_problem_desc &d) {
   }
   det = 1.0;
   for (int k = 0; k < 3; ++k) {
+    if (dim == 2 && k == 1) {
+      kc.h[k] = kc.inv_h[k] = 1.0;
+      kc.origin[k] = 0.0;
+      continue;
+    }
     kc.h[k] = m.extent[k] / nc[k];
     kc.inv_h[k] = 1.0 / kc.h[k];
     kc.origin[k] = m.origin[k];
@@ -507,20 +518,21 @@ Context::Context(const adm_problem_desc &d) {
   for (int q = 0; q < nq; ++q) {
     double w = 1.0;
     int r = q;
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < dim; ++k) {
       w *= wq[r % n1];
       r /= n1;
     }
     kc.jxw[q] = det * w;
   }
   for (int nd = 0; nd < 3; ++nd) {
+    if (!has_face(2 * nd)) continue;
     double fdet = 1.0;
     for (int k = 0; k < 3; ++k)
-      if (k != nd) fdet *= kc.h[k];
+      if (k != nd && (dim == 3 || k != 1)) fdet *= kc.h[k];
     for (int q = 0; q < nfq; ++q) {
       double w = 1.0;
       int r = q;
-      for (int k = 0; k < 2; ++k) {
+      for (int k = 0; k < dim - 1; ++k) {
         w *= wq[r % n1];
         r /= n1;
       }
@@ -742,11 +754,12 @@ uint32_t Context::face_mask(int64_t c) const {
                      (int)(c / ((int64_t)nc[0] * nc[1]))};
   uint32_t mask = 0;
   for (int f = 0; f < 6; ++f) {
+    if (!has_face(f)) continue;
     const int dd = f / 2, side = f % 2;
     const int nb = cc[dd] + (side ? 1 : -1);
     unsigned bid;
     if (nb < 0 || nb >= nc[dd]) {
-      bid = (unsigned)f;
+      bid = (unsigned)face_id(f);
     } else {
       int cn[3] = {cc[0], cc[1], cc[2]};
       cn[dd] = nb;
@@ -786,7 +799,7 @@ void Context::rebuild_cells() {
                            (int)(c / ((int64_t)nc[0] * nc[1]))};
         const int dd = f / 2, side = f % 2;
         const bool domain = side ? (cc[dd] == nc[dd] - 1) : (cc[dd] == 0);
-        const unsigned bid = domain ? (unsigned)f : 0xffffffffu;
+        const unsigned bid = domain ? (unsigned)face_id(f) : 0xffffffffu;
         const int nb = kc.n_boundary_ids;
         const int bt = bid < (unsigned)(nb - 1) ? kc.boundary_type[bid]
                                                 : kc.boundary_type[nb - 1];
@@ -851,7 +864,7 @@ void Context::build_face_nodes() {
   face_out_cap = fn_node_cap = fn_offset_cap = fn_slot_cap = 0;
   ADM_HIP(hipMemset(d_fn_rank, 0xff, sizeof(int32_t) * n_dofs));
   if (faces.empty()) return;
-  const int N = p + 1, NF = N * N;
+  const int N = p + 1, NF = nfq;
   const int64_t ns[3] = {1, (int64_t)nn[0], (int64_t)nn[0] * nn[1]};
   std::vector<std::pair<int64_t, int32_t>> ent;
   ent.reserve(faces.size() * NF);
@@ -862,6 +875,13 @@ void Context::build_face_nodes() {
                            c / ((int64_t)nc[0] * nc[1])};
     const int64_t base = cc[0] * p + ns[1] * cc[1] * p + ns[2] * cc[2] * p +
                          (side ? p : 0) * ns[d];
+    if (dim == 2) {
+      // an edge: its nodes along the other axis of the (x, z) plane
+      const int64_t ts = d == 0 ? ns[2] : 1;
+      for (int q = 0; q < NF; ++q)
+        ent.emplace_back(base + q * ts, (int32_t)(f * NF + q));
+      continue;
+    }
     const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
     for (int q = 0; q < NF; ++q)
       ent.emplace_back(base + (q % N) * ns[t1] + (q / N) * ns[t2],
@@ -1048,7 +1068,7 @@ std::vector<uint8_t> Context::node_activity() const {
     const int64_t cx = c % nc[0], cy = (c / nc[0]) % nc[1],
                   cz = c / ((int64_t)nc[0] * nc[1]);
     for (int k = 0; k <= p; ++k)
-      for (int j = 0; j <= p; ++j)
+      for (int j = 0; j <= (dim == 3 ? p : 0); ++j)
         for (int i = 0; i <= p; ++i)
           na[(size_t)(cx * p + i) +
              (size_t)nn[0] * ((cy * p + j) + (size_t)nn[1] * (cz * p + k))] = 1;
@@ -1069,6 +1089,7 @@ void Context::cells_to_activate(int n_boxes, const double *boxes,
     for (int b = 0; b < n_boxes; ++b) {
       bool hit = true;
       for (int d = 0; d < 3; ++d) {
+        if (dim == 2 && d == 1) continue;  // boxes are (x, -, z) in 2-D
         const double lo = kc.origin[d] + cc[d] * kc.h[d];
         const double hi = kc.origin[d] + (cc[d] + 1) * kc.h[d];
         if (!(lo <= boxes[6 * b + 3 + d] && boxes[6 * b + d] <= hi)) hit = false;
@@ -1102,8 +1123,22 @@ void Context::add_material(const int32_t *box_of_cell, const double *box_cos,
   }
 }
 
+// cells_to_activate and add_material_host in one call (the activation of
+// 2-D meshes)
+void Context::activate_host(int n_boxes, const double *boxes,
+                            const double *box_cos, const double *box_sin,
+                            double new_temperature, double *T,
+                            int32_t *box_of_cell_out, hipStream_t s) {
+  std::vector<int32_t> boc((size_t)n_cells);
+  cells_to_activate(n_boxes, boxes, boc.data());
+  add_material_host(boc.data(), box_cos, box_sin, new_temperature, T, s);
+  if (box_of_cell_out) std::copy(boc.begin(), boc.end(), box_of_cell_out);
+}
+
 // the device activation's counters and scans are 32-bit
 bool Context::device_activation_fits() const {
+  // the device activation's face and node kernels are three-dimensional
+  if (dim == 2) return false;
   const int64_t NF = (int64_t)(p + 1) * (p + 1);
   return n_cells * 6 * NF < (int64_t)INT32_MAX && n_dofs < (int64_t)INT32_MAX;
 }
@@ -1226,6 +1261,10 @@ void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
   const size_t lds = pk == 2 ? sizeof(double) * kMergedLds * kc.mat.n_materials
                              : 0;
   const unsigned M = n_members;
+  if (dim == 2) {
+    launch_2d(a, fa, src, dst, M, lim, s);
+    return;
+  }
   // the face term's values into its slots (face_out)
   auto launch_faces = [&]() {
     if (!fa.n_faces) return;
@@ -1382,6 +1421,88 @@ void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
   ADM_HIP(hipGetLastError());
 }
 
+// dim = 2: the cell kernel once per parity color of (x, z) (plain
+// read-add-write) or once with atomics, then the face kernel and the face
+// gather pass.
+void Context::launch_2d(const CellArgs &a, const FaceArgs &fa,
+                        const double *src, double *dst, unsigned M,
+                        const int (&lim)[3], hipStream_t s) const {
+  const int pk = property_kind();
+  const int n_colors = use_coloring ? 4 : 1;
+  const int cpw = 64 / (p + 1);
+  for (int col = 0; col < n_colors; ++col) {
+    CellArgs ac = a;
+    ac.color = use_coloring ? col : -1;
+    const int bx = use_coloring ? col & 1 : 0;
+    const int bz = use_coloring ? (col >> 1) & 1 : 0;
+    ac.lext[0] = use_coloring ? (lim[0] > bx ? (lim[0] - bx + 1) / 2 : 0)
+                              : lim[0];
+    ac.lext[1] = 1;
+    ac.lext[2] = use_coloring ? (lim[2] > bz ? (lim[2] - bz + 1) / 2 : 0)
+                              : lim[2];
+    if (ac.lext[0] == 0 || ac.lext[2] == 0) continue;
+    ac.nseg = (ac.lext[0] + cpw - 1) / cpw;
+    ac.n_rows = ac.nseg * ac.lext[2];
+#define ADM_DISPATCH(P_, NS_)                                   \
+  if (p == P_ && n_states == NS_) {                              \
+    if (pk == 0)                                                 \
+      launch_cell_2d<P_, NS_, 0>(ac, src, dst, M, s);            \
+    else if (pk == 1)                                            \
+      launch_cell_2d<P_, NS_, 1>(ac, src, dst, M, s);            \
+    else                                                         \
+      launch_cell_2d<P_, NS_, 2>(ac, src, dst, M, s);            \
+  }
+    ADM_DISPATCH(1, 1)
+    ADM_DISPATCH(1, 2)
+    ADM_DISPATCH(1, 3)
+    ADM_DISPATCH(2, 1)
+    ADM_DISPATCH(2, 2)
+    ADM_DISPATCH(2, 3)
+    ADM_DISPATCH(3, 1)
+    ADM_DISPATCH(3, 2)
+    ADM_DISPATCH(3, 3)
+    ADM_DISPATCH(4, 1)
+    ADM_DISPATCH(4, 2)
+    ADM_DISPATCH(4, 3)
+    ADM_DISPATCH(5, 1)
+    ADM_DISPATCH(5, 2)
+    ADM_DISPATCH(5, 3)
+#undef ADM_DISPATCH
+  }
+  if (fa.n_faces) {
+#define ADM_DISPATCH(P_, NS_)                                   \
+  if (p == P_ && n_states == NS_) {                              \
+    if (pk == 0)                                                 \
+      launch_face_2d<P_, NS_, 0>(fa, src, M, s);                 \
+    else if (pk == 1)                                            \
+      launch_face_2d<P_, NS_, 1>(fa, src, M, s);                 \
+    else                                                         \
+      launch_face_2d<P_, NS_, 2>(fa, src, M, s);                 \
+  }
+    ADM_DISPATCH(1, 1)
+    ADM_DISPATCH(1, 2)
+    ADM_DISPATCH(1, 3)
+    ADM_DISPATCH(2, 1)
+    ADM_DISPATCH(2, 2)
+    ADM_DISPATCH(2, 3)
+    ADM_DISPATCH(3, 1)
+    ADM_DISPATCH(3, 2)
+    ADM_DISPATCH(3, 3)
+    ADM_DISPATCH(4, 1)
+    ADM_DISPATCH(4, 2)
+    ADM_DISPATCH(4, 3)
+    ADM_DISPATCH(5, 1)
+    ADM_DISPATCH(5, 2)
+    ADM_DISPATCH(5, 3)
+#undef ADM_DISPATCH
+    if (fa.n_fnodes)
+      hipLaunchKernelGGL(face_gather_kernel,
+                         dim3((unsigned)((fa.n_fnodes + 255) / 256), M),
+                         dim3(256), 0, s, fa, dst);
+  }
+  ADM_HIP(hipGetLastError());
+}
+
 size_t Context::patch_buffer_size(int nz_rows, unsigned n_members) const {
   const int nz = nz_rows >= 0 ? std::min(nz_rows, nc[2]) : nc[2];
   const int px = p == 1 ? PlaneLanes<1>::PX : PlaneLanes<2>::PX;
@@ -1413,6 +1534,20 @@ void Context::fill(double *x, double v, int64_t n, hipStream_t s) {
 void Context::compute_inverse_mass(double *out, hipStream_t s) {
   const int block = 256;
   const unsigned grid = (unsigned)((n_dofs + block - 1) / block);
+  if (dim == 2) {
+#define ADM_MASS_2D(P_)                                                   \
+  if (p == P_)                                                            \
+    hipLaunchKernelGGL(inverse_mass_kernel_2d<P_>, dim3(grid), dim3(block), \
+                       0, s, d_meta, d_wgl, det, nc[0], nc[2], n_dofs, out);
+    ADM_MASS_2D(1)
+    ADM_MASS_2D(2)
+    ADM_MASS_2D(3)
+    ADM_MASS_2D(4)
+    ADM_MASS_2D(5)
+#undef ADM_MASS_2D
+    ADM_HIP(hipGetLastError());
+    return;
+  }
   if (p == 1)
     hipLaunchKernelGGL(inverse_mass_kernel<1>, dim3(grid), dim3(block), 0, s,
                        d_meta, d_wgl, det, nc[0], nc[1], nc[2], n_dofs, out);
@@ -1977,6 +2112,9 @@ static int check_launch_config(const adm::Context &c, int32_t mode,
   if (mode == ADM_LAUNCH_PLANE && !c.plane_tables_ok)
     return adm::set_error(ADM_ERR_UNSUPPORTED,
                           "the plane kernel needs the symmetric Q2 tables");
+  if (mode == ADM_LAUNCH_PLANE && c.dim == 2)
+    return adm::set_error(ADM_ERR_UNSUPPORTED,
+                          "the plane kernel is three-dimensional");
   return ADM_OK;
 }
 
@@ -2057,7 +2195,7 @@ static void tune_launch(adm::Context &cfg,
       const int mode = modes[mi];
       if (mode == ADM_LAUNCH_PLANE &&
           (cfg.p > 2 || cfg.n_dofs >= (int64_t)INT32_MAX ||
-           !cfg.plane_tables_ok)) {
+           !cfg.plane_tables_ok || cfg.dim == 2)) {
         if (ms_out) ms_out[k] = -1.0;
         continue;
       }
@@ -2188,8 +2326,8 @@ int adm_get_face_state(void *ptr, double *face_powder) {
   ADM_CTX(ptr)
   if (!face_powder) return adm::set_error(ADM_ERR_INVALID, "null output");
   ctx.sync_faces_host();
-  const int nf = ctx.nfq;
-  std::fill(face_powder, face_powder + (size_t)ctx.n_cells * 6 * nf, 0.0);
+  const int nf = ctx.nfq, nfc = 2 * ctx.dim;
+  std::fill(face_powder, face_powder + (size_t)ctx.n_cells * nfc * nf, 0.0);
   if (ctx.d_face_powder && !ctx.faces.empty()) {
     ADM_HIP(hipDeviceSynchronize());
     std::vector<double> fpw(ctx.faces.size() * nf);
@@ -2197,9 +2335,9 @@ int adm_get_face_state(void *ptr, double *face_powder) {
                       sizeof(double) * fpw.size(), hipMemcpyDeviceToHost));
     for (size_t f = 0; f < ctx.faces.size(); ++f) {
       const int64_t c = ctx.faces[f].cell;
-      const int fid = ctx.faces[f].info & 7;
+      const int fid = ctx.face_id(ctx.faces[f].info & 7);
       for (int q = 0; q < nf; ++q)
-        face_powder[(c * 6 + fid) * nf + q] = fpw[f * nf + q];
+        face_powder[(c * nfc + fid) * nf + q] = fpw[f * nf + q];
     }
   }
   ADM_API_END
@@ -2210,14 +2348,14 @@ int adm_set_face_state(void *ptr, const double *face_powder) {
   ADM_CTX(ptr)
   if (!face_powder) return adm::set_error(ADM_ERR_INVALID, "null input");
   ctx.sync_faces_host();
-  const int nf = ctx.nfq;
+  const int nf = ctx.nfq, nfc = 2 * ctx.dim;
   if (ctx.d_face_powder && !ctx.faces.empty()) {
     std::vector<double> fpw(ctx.faces.size() * nf);
     for (size_t f = 0; f < ctx.faces.size(); ++f) {
       const int64_t c = ctx.faces[f].cell;
-      const int fid = ctx.faces[f].info & 7;
+      const int fid = ctx.face_id(ctx.faces[f].info & 7);
       for (int q = 0; q < nf; ++q)
-        fpw[f * nf + q] = face_powder[(c * 6 + fid) * nf + q];
+        fpw[f * nf + q] = face_powder[(c * nfc + fid) * nf + q];
     }
     ADM_HIP(hipDeviceSynchronize());
     ADM_HIP(hipMemcpy(ctx.d_face_powder, fpw.data(),
@@ -2495,6 +2633,11 @@ int adm_activate(void *ptr, int32_t n_boxes, const double *boxes,
     return adm::set_error(ADM_ERR_UNSUPPORTED,
                           "activation changes the DoFs: clear the DoF map, "
                           "activate on the box-node vector, set the new map");
+  if (ctx.dim == 2) {
+    ctx.activate_host(n_boxes, boxes, box_cos, box_sin, new_temperature, T,
+                      box_of_cell, static_cast<hipStream_t>(stream));
+    return ADM_OK;
+  }
   if (!ctx.device_activation_fits())
     return adm::set_error(ADM_ERR_UNSUPPORTED,
                           "mesh too large for the device activation");
@@ -2568,12 +2711,19 @@ int adm_ensemble_activate(void *ptr, int32_t member, int32_t n_boxes,
   if (n_boxes < 0 || (n_boxes && (!boxes || !box_cos || !box_sin)))
     return adm::set_error(ADM_ERR_INVALID, "bad box list");
   adm::Context &c = *ens.members[member];
-  if (!c.device_activation_fits())
-    return adm::set_error(ADM_ERR_UNSUPPORTED,
-                          "mesh too large for the device activation");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  c.activate(n_boxes, boxes, nullptr, box_cos, box_sin, new_temperature,
-             T ? T + (int64_t)member * ens.n_dofs : nullptr, box_of_cell, s);
+  if (c.dim == 2) {
+    c.activate_host(n_boxes, boxes, box_cos, box_sin, new_temperature,
+                    T ? T + (int64_t)member * ens.n_dofs : nullptr,
+                    box_of_cell, s);
+  } else {
+    if (!c.device_activation_fits())
+      return adm::set_error(ADM_ERR_UNSUPPORTED,
+                            "mesh too large for the device activation");
+    c.activate(n_boxes, boxes, nullptr, box_cos, box_sin, new_temperature,
+               T ? T + (int64_t)member * ens.n_dofs : nullptr, box_of_cell,
+               s);
+  }
   ens.refresh_members();
   if (ens.sb.inv_mass)
     c.compute_inverse_mass(ens.sb.inv_mass + (int64_t)member * ens.n_dofs, s);

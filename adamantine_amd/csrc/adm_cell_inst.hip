@@ -1,6 +1,7 @@
 // One instantiation of the cell and face kernels, selected by -DADM_P,
 // -DADM_NS and -DADM_PK at compile time.
 #include "adm_kernels.hpp"
+#include "adm_kernels_2d.hpp"
 #include "adm_launch.hpp"
 #if ADM_P <= 2
 #include "adm_plane_kernel.hpp"
@@ -44,6 +45,30 @@ void launch_face(const FaceArgs &a, const double *src, double *dst,
                      dim3(256), 0, s, a, src, dst);
 }
 
+template <int P, int NS, int PK>
+void launch_cell_2d(const CellArgs &a, const double *src, double *dst,
+                    unsigned n_members, hipStream_t s) {
+  constexpr int waves = Cell2dLanes<P>::waves_per_block;
+  const unsigned grid = (unsigned)((a.n_rows + waves - 1) / waves);
+  hipLaunchKernelGGL((cell_kernel_2d<P, NS, PK>), dim3(grid, n_members),
+                     dim3(64 * waves), 0, s, a, src, dst);
+}
+
+template <int P, int NS, int PK>
+void launch_face_2d(const FaceArgs &a, const double *src, unsigned n_members,
+                    hipStream_t s) {
+  const int64_t per = Face2dLanes<P>::faces_per_block;
+  const int64_t grid = (a.n_faces + per - 1) / per;
+  hipLaunchKernelGGL((face_kernel_2d<P, NS, PK>),
+                     dim3((unsigned)grid, n_members), dim3(256), 0, s, a, src);
+}
+
+template void launch_cell_2d<ADM_P, ADM_NS, ADM_PK>(const CellArgs &,
+                                                    const double *, double *,
+                                                    unsigned, hipStream_t);
+template void launch_face_2d<ADM_P, ADM_NS, ADM_PK>(const FaceArgs &,
+                                                    const double *, unsigned,
+                                                    hipStream_t);
 template void launch_cell<ADM_P, ADM_NS, ADM_PK>(const CellArgs &,
                                                  const double *, double *,
                                                  int64_t, size_t, unsigned,

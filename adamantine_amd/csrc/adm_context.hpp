@@ -120,6 +120,11 @@ struct Context {
                 double new_temperature, double *T, int32_t *box_of_cell_out,
                 hipStream_t s);
   bool device_activation_fits() const;
+  // the same on the host, box_of_cell_out (optional) receiving the cells'
+  // boxes
+  void activate_host(int n_boxes, const double *boxes, const double *box_cos,
+                     const double *box_sin, double new_temperature, double *T,
+                     int32_t *box_of_cell_out, hipStream_t s);
   // faces <- d_faces after a device activation
   void sync_faces_host();
   // the activation geometry, saved and put back (the inverse mass is the
@@ -149,6 +154,9 @@ struct Context {
               double *dst, unsigned n_members, hipStream_t s,
               int nz_rows = -1, double *patch = nullptr,
               const StageArgs *st = nullptr) const;
+  void launch_2d(const CellArgs &a, const FaceArgs &fa, const double *src,
+                 double *dst, unsigned n_members, const int (&lim)[3],
+                 hipStream_t s) const;
   void fill(double *x, double v, int64_t n, hipStream_t s);
   // the plane kernel's patch buffer, grown to n doubles if smaller
   double *ensure_patch_buffer(size_t n) const;
@@ -164,6 +172,13 @@ struct Context {
                     hipStream_t s);
 
   int device = 0;
+  // dim = 2: an n_x x 1 x n_z box with one node layer in y (nn[1] == 1),
+  // geometry constants in slots 0 and 2, faces 0, 1, 4, 5 only
+  int dim = 3;
+  bool has_face(int f) const { return dim == 3 || f / 2 != 1; }
+  // the reference's (colorized) boundary id of face f, also the face's
+  // slot in the [cell][2 dim][face point] state layout
+  int face_id(int f) const { return dim == 2 && f >= 4 ? f - 2 : f; }
   int p = 1, n1 = 2, nq = 8, nfq = 4;
   int nc[3] = {0, 0, 0}, nn[3] = {0, 0, 0};
   int64_t n_cells = 0, n_dofs = 0;

@@ -544,7 +544,7 @@ void nearest_dofs(const Context &c, int64_t np, const double *pts,
     const int64_t cx = cell % c.nc[0], cy = (cell / c.nc[0]) % c.nc[1],
                   cz = cell / ((int64_t)c.nc[0] * c.nc[1]);
     for (int k = 0; k <= c.p; ++k)
-      for (int j = 0; j <= c.p; ++j)
+      for (int j = 0; j <= (c.dim == 3 ? c.p : 0); ++j)
         for (int i = 0; i <= c.p; ++i)
           node_active[(size_t)(cx * c.p + i) +
                       (size_t)c.nn[0] * ((cy * c.p + j) +

@@ -27,4 +27,13 @@ template <int P, int NS, int PK>
 void launch_face(const FaceArgs &a, const double *src, double *dst,
                  unsigned n_members, hipStream_t s);
 
+// dim = 2: the cell kernel over a.n_rows row segments (4 per block) and the
+// face kernel writing face_out
+template <int P, int NS, int PK>
+void launch_cell_2d(const CellArgs &a, const double *src, double *dst,
+                    unsigned n_members, hipStream_t s);
+template <int P, int NS, int PK>
+void launch_face_2d(const FaceArgs &a, const double *src, unsigned n_members,
+                    hipStream_t s);
+
 }  // namespace adm

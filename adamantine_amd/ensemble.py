@@ -363,14 +363,14 @@ class EnsembleDevice:
             pw.ctypes.data_as(_dp)))
 
     def face_state(self, member: int):
-        fp = np.zeros((self.n_cells, 6, self.n_face_q))
+        fp = np.zeros((self.n_cells, 2 * self.base.dim, self.n_face_q))
         check(_lib().adm_ensemble_get_face_state(
             self._h, int(member), fp.ctypes.data_as(_dp)))
         return fp
 
     def set_face_state(self, member: int, face_powder):
         fp = np.ascontiguousarray(face_powder, dtype=np.float64)
-        if fp.size != self.n_cells * 6 * self.n_face_q:
+        if fp.size != self.n_cells * 2 * self.base.dim * self.n_face_q:
             raise ValueError("face state of the wrong size")
         check(_lib().adm_ensemble_set_face_state(
             self._h, int(member), fp.ctypes.data_as(_dp)))

@@ -291,17 +291,18 @@ class ThermalOperatorDevice:
 
     def face_state(self):
         """Powder ratio at the points of the boundary faces with a
-        boundary term, [n_cells][6][n_face_q] (zero elsewhere)."""
-        fp = np.zeros((self.n_cells, 6, self.n_face_q))
+        boundary term, [n_cells][2 dim][n_face_q] (zero elsewhere)."""
+        fp = np.zeros((self.n_cells, 2 * self.problem.dim, self.n_face_q))
         check(load_library().adm_get_face_state(self._h,
                                                 fp.ctypes.data_as(_dp)))
         return fp
 
     def set_face_state(self, face_powder):
         fp = _host(face_powder)
-        if fp.size != self.n_cells * 6 * self.n_face_q:
+        need = self.n_cells * 2 * self.problem.dim * self.n_face_q
+        if fp.size != need:
             raise AdmError("face state of %d values where %d are needed" %
-                           (fp.size, self.n_cells * 6 * self.n_face_q))
+                           (fp.size, need))
         check(load_library().adm_set_face_state(self._h,
                                                 fp.ctypes.data_as(_dp)))
 

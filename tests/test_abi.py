@@ -58,7 +58,7 @@ def test_invalid_descriptor_is_an_error_not_a_crash():
     from adamantine_amd import thermal
     from adamantine_amd.problem import Problem
     lib = thermal.load_library()
-    for bad in (Problem(dim=2, n_cells=(4, 1, 5)),
+    for bad in (Problem(dim=1, n_cells=(4, 1, 5)),
                 Problem(fe_degree=3),
                 Problem(n_cells=(0, 4, 4))):
         d, keep = bad.to_c()
