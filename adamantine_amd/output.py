@@ -2,9 +2,9 @@
 source/PostProcessor.hh:218-322).
 
 Like deal.II's DataOut with build_patches(n), every cell is written as its
-own patch of n^3 hexahedra (n = max(1, additional_output_refinement)) on
-points of its own. The temperature is the FE_Q(p) field evaluated at the
-patch's equispaced points; cells without FE_Q (inactive) get zero, as
+own patch of n^3 hexahedra (n^2 quadrilaterals in 2-D; n = max(1,
+additional_output_refinement)) on points of its own. The temperature is
+the FE_Q(p) field evaluated at the patch's equispaced points; cells without FE_Q (inactive) get zero, as
 FE_Nothing does. The cell data are the material state ratios "powder",
 "liquid" and "solid" and "subdomain" (0 here, one process); TIME is the
 field data of VtkFlags(time). Files are
@@ -24,6 +24,7 @@ from .basis import gauss_lobatto, lagrange_matrix
 from .problem import Problem
 
 _VTK_HEXAHEDRON = 12
+_VTK_QUAD = 9
 
 
 def _b64(a: np.ndarray) -> str:
@@ -57,51 +58,64 @@ class PostProcessor:
 
     def _build_patches(self):
         p = self.problem
-        if p.dim != 3:
-            raise NotImplementedError("the device path is three-dimensional")
+        dim = p.dim
+        ax = p.axes()       # (x, y, z), or (x, height) in 2-D
         n, deg = self.n_sub, p.fe_degree
-        nc = p.n_cells
-        h = [p.extent[a] / nc[a] for a in range(3)]
+        nc = [p.n_cells[a] for a in ax]
+        h = [p.extent[a] / p.n_cells[a] for a in ax]
+        org = [p.origin[a] for a in ax]
         xi = np.arange(n + 1) / n
         # equispaced patch points from the FE_Q(p) Gauss-Lobatto support
         # points, one direction at a time
         self._L = lagrange_matrix(gauss_lobatto(deg + 1)[0], xi)
         k = np.arange(n + 1)
         # patch points in x-fastest order, per cell
-        pi = np.stack(np.meshgrid(k, k, k, indexing="ij"), -1)[..., ::-1]
-        pi = pi.reshape(-1, 3)                          # (n+1)^3 x (i, j, k)
+        pi = np.stack(np.meshgrid(*([k] * dim), indexing="ij"), -1)[..., ::-1]
+        pi = pi.reshape(-1, dim)                        # (n+1)^dim x (i, ...)
         cells = np.arange(p.n_cells_total)
-        cc = np.stack([cells % nc[0], (cells // nc[0]) % nc[1],
-                       cells // (nc[0] * nc[1])], 1)
-        pts = np.empty((p.n_cells_total, (n + 1) ** 3, 3))
-        for a in range(3):
-            pts[..., a] = (p.origin[a] + (cc[:, None, a] + xi[pi[None, :, a]])
-                           * h[a])
+        cc = np.stack([(cells // int(np.prod(nc[:d]))) % nc[d]
+                       for d in range(dim)], 1)
+        # points have three components; a 2-D point is (x, height, 0), as
+        # deal.II writes it
+        pts = np.zeros((p.n_cells_total, (n + 1) ** dim, 3))
+        for d in range(dim):
+            pts[..., d] = (org[d] + (cc[:, None, d] + xi[pi[None, :, d]])
+                           * h[d])
         self.points = pts.reshape(-1, 3)
-        # hexahedra of each patch (VTK corner order)
+        # hexahedra (quadrilaterals in 2-D) of each patch, VTK corner order
         s = [1, n + 1, (n + 1) ** 2]
         base = []
-        for kz in range(n):
-            for ky in range(n):
+        if dim == 3:
+            for kz in range(n):
+                for ky in range(n):
+                    for kx in range(n):
+                        o = kx * s[0] + ky * s[1] + kz * s[2]
+                        base.append([o, o + s[0], o + s[0] + s[1], o + s[1],
+                                     o + s[2], o + s[0] + s[2],
+                                     o + s[0] + s[1] + s[2], o + s[1] + s[2]])
+        else:
+            for kz in range(n):
                 for kx in range(n):
-                    o = kx * s[0] + ky * s[1] + kz * s[2]
-                    base.append([o, o + s[0], o + s[0] + s[1], o + s[1],
-                                 o + s[2], o + s[0] + s[2],
-                                 o + s[0] + s[1] + s[2], o + s[1] + s[2]])
+                    o = kx * s[0] + kz * s[1]
+                    base.append([o, o + s[0], o + s[0] + s[1], o + s[1]])
         base = np.array(base, dtype=np.int64)
-        npp = (n + 1) ** 3
+        npp = (n + 1) ** dim
         self.connectivity = (base[None, :, :] +
                              npp * cells[:, None, None]).reshape(-1)
-        self.n_hex = p.n_cells_total * n ** 3
+        self.n_hex = p.n_cells_total * n ** dim
+        self.cell_type = _VTK_HEXAHEDRON if dim == 3 else _VTK_QUAD
+        self.corners = 2 ** dim
         # node index of every (cell, local node) for the gather
         m = deg + 1
         nn = p.n_nodes
         loc = np.arange(m)
-        lz, ly, lx = np.meshgrid(loc, loc, loc, indexing="ij")
-        ix = cc[:, None, 0] * deg + lx.ravel()[None, :]
-        iy = cc[:, None, 1] * deg + ly.ravel()[None, :]
-        iz = cc[:, None, 2] * deg + lz.ravel()[None, :]
-        self._cell_nodes = ix + nn[0] * (iy + nn[1] * iz)
+        grid = np.meshgrid(*([loc] * dim), indexing="ij")[::-1]  # x first
+        node = np.zeros((p.n_cells_total, m ** dim), np.int64)
+        stride = 1
+        for d in range(dim):
+            node += (cc[:, None, d] * deg + grid[d].ravel()[None, :]) * stride
+            stride *= nn[d]
+        self._cell_nodes = node
 
     def patch_temperature(self, temperature, active) -> np.ndarray:
         """The field at the patch points, cell by cell; zero on inactive
@@ -109,11 +123,16 @@ class PostProcessor:
         T = temperature.detach().cpu().numpy() if hasattr(
             temperature, "cpu") else np.asarray(temperature, np.float64)
         m = self.problem.fe_degree + 1
-        u = T[self._cell_nodes].reshape(-1, m, m, m)    # [c][z][y][x]
         L = self._L
-        v = np.einsum("ax,czyx->czya", L, u)
-        v = np.einsum("by,czya->czba", L, v)
-        v = np.einsum("cz,dzba->dcba", L, v)
+        if self.problem.dim == 3:
+            u = T[self._cell_nodes].reshape(-1, m, m, m)    # [c][z][y][x]
+            v = np.einsum("ax,czyx->czya", L, u)
+            v = np.einsum("by,czya->czba", L, v)
+            v = np.einsum("cz,dzba->dcba", L, v)
+        else:
+            u = T[self._cell_nodes].reshape(-1, m, m)       # [c][z][x]
+            v = np.einsum("ax,czx->cza", L, u)
+            v = np.einsum("bz,cza->cba", L, v)
         v = v.reshape(v.shape[0], -1)
         v[~np.asarray(active, bool)] = 0.0
         return v.reshape(-1)
@@ -124,7 +143,7 @@ class PostProcessor:
         temperature, the material state and the subdomain, then
         write_pvtu; returns the .pvtu path."""
         ns = cell_state.shape[1]
-        n3 = self.n_sub ** 3
+        n3 = self.n_sub ** self.problem.dim
         solid = np.repeat(cell_state[:, 0], n3)
         liquid = np.repeat(cell_state[:, 1] if ns > 1 else
                            np.zeros(len(cell_state)), n3)
@@ -133,8 +152,8 @@ class PostProcessor:
         Tp = self.patch_temperature(temperature, active)
         name = "%s.%d.0.vtu" % (self.filename_prefix, time_step)
         path = os.path.join(self.output_dir, name) if self.output_dir else name
-        offsets = 8 * np.arange(1, self.n_hex + 1, dtype=np.int64)
-        types = np.full(self.n_hex, _VTK_HEXAHEDRON, np.uint8)
+        offsets = self.corners * np.arange(1, self.n_hex + 1, dtype=np.int64)
+        types = np.full(self.n_hex, self.cell_type, np.uint8)
         with open(path, "w") as f:
             f.write('<?xml version="1.0"?>\n<VTKFile type="UnstructuredGrid" '
                     'version="1.0" byte_order="LittleEndian" '

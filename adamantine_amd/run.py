@@ -72,16 +72,18 @@ def problem_from_database(db: Dict, base_dir: str = ".",
     if str(_get(g, "import_mesh", "false")) == "true":
         raise NotImplementedError("imported meshes are not on the device path")
     dim = int(_get(g, "dim", 3))
-    if dim != 3:
-        raise NotImplementedError("the device path is three-dimensional")
-    # axis x = length, y = width, z = height (Geometry.cc:188-243)
+    if dim not in (2, 3):
+        raise ValueError("geometry.dim must be 2 or 3")
+    # axis x = length, y = width, z = height (Geometry.cc:188-243); a 2-D
+    # box has no width (the descriptor's y entries are placeholders)
     n_cells = (int(_get(g, "length_divisions", 10)),
-               int(_get(g, "width_divisions", 10)),
+               int(_get(g, "width_divisions", 10)) if dim == 3 else 1,
                int(_get(g, "height_divisions", 10)))
     origin = (float(_get(g, "length_origin", 0.0)),
-              float(_get(g, "width_origin", 0.0)),
+              float(_get(g, "width_origin", 0.0)) if dim == 3 else 0.0,
               float(_get(g, "height_origin", 0.0)))
-    extent = (float(g["length"]), float(g["width"]), float(g["height"]))
+    extent = (float(g["length"]), float(g["width"]) if dim == 3 else 1.0,
+              float(g["height"]))
     mats = db["materials"]
     n_states, p_order = material_template_parameters(mats)
     m = material_from_database(mats, n_states, p_order)
@@ -90,9 +92,9 @@ def problem_from_database(db: Dict, base_dir: str = ".",
     quad = 1 if str(disc.get("quadrature", "gauss")) == "lobatto" else 0
     ts = db["time_stepping"]
     p = Problem(n_cells=n_cells, extent=extent, origin=origin, fe_degree=deg,
-                quadrature=quad, dim=3, material=m,
+                quadrature=quad, dim=dim, material=m,
                 boundary_types=boundary_from_database(db.get("boundary", {}),
-                                                      6),
+                                                      2 * dim),
                 beams=beams_from_database(db["sources"], base_dir,
                                           db.get("units")),
                 method=_METHODS[str(ts["method"])],
@@ -108,6 +110,20 @@ def problem_from_database(db: Dict, base_dir: str = ".",
         st[:, 0] = 1.0
         p.cell_state = st
     return p
+
+
+def deposition_boxes(g: Dict, p: Problem, base_dir: str):
+    """create_material_deposition_boxes with the boxes as rows of (min x,
+    y, z, max x, y, z), the layout of the activation calls; a 2-D box's
+    (x, height) bounds go to x and z."""
+    from .deposition import create_material_deposition_boxes
+    boxes, t, c, s = create_material_deposition_boxes(g, p.beams, base_dir,
+                                                      p.dim)
+    if p.dim == 2:
+        b6 = np.zeros((len(boxes), 6))
+        b6[:, [0, 2, 3, 5]] = np.asarray(boxes).reshape(-1, 4)
+        boxes = b6
+    return boxes, t, c, s
 
 
 def _stddev_keys(tree: Dict, path: str = "") -> List[str]:
@@ -159,7 +175,6 @@ def run(db: Dict, base_dir: str = ".", fe_degree: Optional[int] = None,
     configuration is picked by timing on the initial temperature (the
     phase state is left as it was)."""
     from bisect import bisect_left
-    from .deposition import create_material_deposition_boxes
     from .thermal import ThermalPhysicsDevice
     p = problem_from_database(db, base_dir, fe_degree)
     ph = ThermalPhysicsDevice(p)
@@ -194,8 +209,7 @@ def run(db: Dict, base_dir: str = ".", fe_degree: Optional[int] = None,
         write(n_time_step, time)
     n_time_step += 1
     g = db["geometry"]
-    boxes, dep_t, dep_cos, dep_sin = create_material_deposition_boxes(
-        g, p.beams, base_dir)
+    boxes, dep_t, dep_cos, dep_sin = deposition_boxes(g, p, base_dir)
     dep_t = list(dep_t)
     ts = db["time_stepping"]
     dt = float(ts["time_step"])
@@ -366,10 +380,8 @@ def run_ensemble(db: Dict, base_dir: str = ".",
     # deposition boxes of member 0's sources, shared by all members
     # (application/adamantine.hh:1917-1925)
     from bisect import bisect_left
-    from .deposition import create_material_deposition_boxes
     g = db["geometry"]
-    boxes, dep_t, dep_cos, dep_sin = create_material_deposition_boxes(
-        g, p.beams, base_dir)
+    boxes, dep_t, dep_cos, dep_sin = deposition_boxes(g, p, base_dir)
     dep_t = list(dep_t)
     activation_time = float(_get(g, "deposition_time", 0.0))
     new_temperature = float(_get(db, "materials.new_material_temperature",

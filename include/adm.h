@@ -121,7 +121,12 @@ typedef struct adm_material_desc {
 /* Structured box mesh (source/Geometry.cc:186-260) and FE space
  * (source/ThermalPhysics.templates.hh:364-369). */
 typedef struct adm_mesh_desc {
-  int32_t dim;           /* 3 (the CPU oracle also accepts 2: axes x, z) */
+  int32_t dim;           /* 3, or 2: axes x (length) and z (height); the
+                            n_cells, origin and extent entries [1] are then
+                            unused, boundary ids are 0..3 and the beam
+                            centre's components 0 and 1 are (x, height), as
+                            in the reference's dim = 2 instantiations
+                            (source/instantiation.hh:121-179) */
   int32_t fe_degree;     /* 1..5 */
   int32_t quadrature;    /* 0 = QGauss(p+1), 1 = QGaussLobatto(p+1) */
   int32_t n_cells[3];    /* length_divisions, width_divisions, height_divisions */
@@ -355,7 +360,7 @@ int adm_set_qp_state(void *ctx, const double *liquid, const double *powder);
 
 /* Per-face-quadrature-point powder ratio of the boundary faces
  * (_face_powder_ratio of source/ThermalOperator.hh:225), host out
- * [n_cells][6][n_face_q]; faces without a boundary term read as zero. */
+ * [n_cells][2 dim][n_face_q]; faces without a boundary term read as zero. */
 int adm_get_face_state(void *ctx, double *face_powder);
 /* Sets the face powder ratios from the same layout (the entries of faces
  * without a boundary term are ignored): the restart counterpart of
@@ -415,7 +420,9 @@ int adm_add_material(void *ctx, const int32_t *box_of_cell,
  * boxes: host, n_boxes x {min x, y, z, max x, y, z}; box_cos, box_sin: host,
  * n_boxes; T: device, n_dofs (may be NULL); box_of_cell: host out, n_cells
  * (may be NULL): the box that activated each cell, or -1. Enqueued on
- * stream; returns after one small device-to-host copy. */
+ * stream; returns after one small device-to-host copy. A 2-D context takes
+ * the boxes' x and z bounds and runs the host path of
+ * adm_cells_to_activate + adm_add_material (synchronous). */
 int adm_activate(void *ctx, int32_t n_boxes, const double *boxes,
                  const double *box_cos, const double *box_sin,
                  double new_temperature, double *T, int32_t *box_of_cell,
@@ -616,7 +623,8 @@ int adm_ensemble_transform(void *ens, double *T, const double *Tm,
  * exactly (its own adjoint per quadrature point); at return the context's
  * phase state is the one at the final time, as after adm_evolve. Polynomial
  * or tabulated properties with the merged-table form (not the general-table
- * evaluator). */
+ * evaluator). Three-dimensional contexts only (a 2-D one: ADM_ERR_UNSUPPORTED,
+ * as for adm_ensemble_adjoint_gradient and adm_operator_vjp). */
 int adm_adjoint_gradient(void *ctx, int method, double t0, double dt,
                          int n_steps, int checkpoint_every, const double *T0,
                          int32_t n_obs, const int64_t *obs_dof,

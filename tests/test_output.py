@@ -78,3 +78,44 @@ def test_pvtu_and_pvd(tmp_path):
     assert piece.get("Source") == "thermal_m2.1.0.vtu"
     d = read_vtu(str(tmp_path / "thermal_m2.2.0.vtu"))
     assert (d["temperature"] == 300.0).all() and (d["solid"] == 1.0).all()
+
+
+@pytest.mark.parametrize("deg,refine", [(1, 0), (2, 3)])
+def test_vtu_patches_2d(tmp_path, deg, refine):
+    """2-D: quadrilateral patches on (x, height, 0) points, the field exact
+    where FE_Q(p) holds it."""
+    p = Problem(n_cells=(4, 1, 3), extent=(2.0, 1.0, 1.5),
+                origin=(0.5, 0.0, -0.3), fe_degree=deg, dim=2)
+    X = p.node_coordinates()
+
+    def f2(x):
+        if deg >= 2:
+            return 300.0 + 4.0 * x[:, 0] ** 2 - 2.0 * x[:, 0] * x[:, 1] + x[:, 1]
+        return 300.0 + 2.0 * x[:, 0] - 3.0 * x[:, 1]
+    T = f2(X)
+    active = np.ones(p.n_cells_total, np.uint8)
+    active[0] = 0
+    cs = np.zeros((p.n_cells_total, 2))
+    cs[:, 0] = 1.0
+    pp = PostProcessor({"filename_prefix": "two",
+                        "output_dir": str(tmp_path) + "/",
+                        "additional_output_refinement": refine}, p)
+    pp.write_thermal_output(0, 0.0, T, cs, active)
+    d = read_vtu(str(tmp_path / "two.0.0.vtu"))
+    n = max(1, refine)
+    npts = (n + 1) ** 2
+    pts = d["points"]
+    assert pts.shape == (p.n_cells_total * npts, 3)
+    assert (pts[:, 2] == 0.0).all()
+    assert (d["types"] == 9).all() and d["types"].size == p.n_cells_total * n * n
+    got = d["temperature"].reshape(p.n_cells_total, npts)
+    want = f2(pts[:, :2]).reshape(p.n_cells_total, npts)
+    np.testing.assert_allclose(got[1:], want[1:], rtol=1e-13, atol=1e-10)
+    assert (got[0] == 0.0).all()
+    conn = d["connectivity"].reshape(-1, 4)
+    c = pts[conn][:, :, :2]
+    lo, hi = c.min(1), c.max(1)
+    np.testing.assert_array_equal(c[:, 0], lo)
+    np.testing.assert_array_equal(c[:, 2], hi)
+    cen = pts.reshape(p.n_cells_total, npts, 3).mean(1)[:, :2]
+    np.testing.assert_allclose(cen, p.cell_centers(), rtol=0, atol=1e-12)
