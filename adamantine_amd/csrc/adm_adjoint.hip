@@ -231,8 +231,11 @@ __global__ void __launch_bounds__(256)
   constexpr int N = P + 1, NL = N * N, N3 = N * N * N;
   constexpr int CPW = CellLanes<P>::cells_per_wave;
   const CellArgs &a = aa.c;
-  // one operator, or member blockIdx.y of an ensemble
-  const MemberArgs *mb = a.members ? a.members + blockIdx.y : nullptr;
+  // one operator, or member blockIdx.y of an ensemble (a constant-space
+  // pointer: a generic one selected against the argument block's fields
+  // would make the compiler copy the whole argument block to scratch)
+  const ADM_CONSTANT MemberArgs *mb =
+      a.members ? as_constant(a.members) + blockIdx.y : nullptr;
   const uint32_t *meta_p = mb ? mb->meta : a.meta;
   const int32_t *pslot_p = mb ? mb->powder_slot : a.powder_slot;
   const double *dcos_p = mb ? mb->dep_cos : a.dep_cos;
@@ -582,7 +585,8 @@ __global__ void __launch_bounds__(256)
   constexpr int N = P + 1, NF = N * N;
   const int64_t fi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   // one operator, or member blockIdx.y of an ensemble
-  const MemberArgs *mb = a.members ? a.members + blockIdx.y : nullptr;
+  const ADM_CONSTANT MemberArgs *mb =
+      a.members ? as_constant(a.members) + blockIdx.y : nullptr;
   if (fi >= (mb ? mb->n_faces : a.n_faces)) return;
   if (mb) {
     const int64_t voff = (int64_t)blockIdx.y * a.member_stride;
