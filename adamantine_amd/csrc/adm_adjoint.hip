@@ -58,6 +58,11 @@ struct AdjArgs {
                               // NULL: zero after, not stored
   double *partials;           // [3]: d/d eta_0, d/d k-factor, d/d R_0
   double inv_eta0;            // 1 / absorption efficiency of beam 0 (or 0)
+  // per-block sums of the partials, [member][part_blocks][3]: this launch's
+  // blocks at part_base .. (reduce_partials_kernel adds them to partials in
+  // a fixed order: no same-address atomics, the same sums every run)
+  double *block_part;
+  int32_t part_blocks, part_base;
   // ensemble launch (c.members != NULL): member blockIdx.y's vectors at
   // blockIdx.y * c.member_stride, its powder arrays at pw_off[blockIdx.y],
   // its partials at partials + 3 blockIdx.y, 1/eta_0 from inv_eta
@@ -247,7 +252,10 @@ __global__ void __launch_bounds__(256)
   double *out_p = aa.out + voff;
   const double *pwo_p = aa.powder_old ? aa.powder_old + poff : nullptr;
   double *pwb_p = aa.powder_bar ? aa.powder_bar + poff : nullptr;
-  double *part_p = aa.partials + (mb ? 3 * (int64_t)blockIdx.y : 0);
+  double *bpart_p =
+      aa.block_part +
+      3 * ((mb ? (int64_t)blockIdx.y * aa.part_blocks : 0) + aa.part_base +
+           blockIdx.x);
   const double inv_eta0 = mb ? aa.inv_eta[blockIdx.y] : aa.inv_eta0;
   const KConst &kc = *(const KConst *)as_global(mb ? mb->kc : a.kc);
   const MatConst &m = kc.mat;
@@ -557,18 +565,49 @@ __global__ void __launch_bounds__(256)
         unsafeAtomicAdd(out_p + base + i, acc);
     }
   }
-  // parameter partials: wavefront sum, one atomic per wavefront
+  // parameter partials: wavefront sums, then the block's sum (waves in
+  // order) into its slot. Same-address FP64 atomics from every wavefront
+  // serialise at the memory side and slowed the whole kernel (the node
+  // accumulation's atomics with them) by 1.7x.
+  __shared__ double wsum[4][3];
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
     pe += __shfl_xor(pe, off);
     ps += __shfl_xor(ps, off);
     pr += __shfl_xor(pr, off);
   }
-  if (lane == 0 && (pe != 0.0 || ps != 0.0 || pr != 0.0)) {
-    unsafeAtomicAdd(part_p + 0, pe);
-    unsafeAtomicAdd(part_p + 1, ps);
-    unsafeAtomicAdd(part_p + 2, pr);
+  if (lane == 0) {
+    wsum[wave][0] = pe;
+    wsum[wave][1] = ps;
+    wsum[wave][2] = pr;
   }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double v = wsum[0][threadIdx.x];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v += wsum[w][threadIdx.x];
+    bpart_p[threadIdx.x] = v;
+  }
+}
+
+// partials[3 j + k] += the sum over the blocks of member j (blockIdx.x) of
+// block_part, in a fixed order: lane l adds blocks l, l + 64, ..., then a
+// fixed butterfly over the lanes.
+__global__ void __launch_bounds__(64)
+    reduce_partials_kernel(const double *__restrict__ block_part,
+                           int32_t part_blocks, double *__restrict__ partials) {
+  const int lane = threadIdx.x;
+  const double *bp = block_part + 3 * (int64_t)blockIdx.x * part_blocks;
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int b = lane; b < part_blocks; b += 64)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v[k] += bp[3 * (int64_t)b + k];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v[k] += __shfl_xor(v[k], off);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) partials[3 * blockIdx.x + k] += v[k];
 }
 
 // (dA_face/dT)^T mu over the boundary faces (one lane per face, as
@@ -752,6 +791,14 @@ struct DevBuf {
     ADM_HIP_ADJ(hipMalloc(&p, sizeof(double) * n_));
     n = n_;
   }
+  // at least n_ doubles (the contents are not kept)
+  void grow(size_t n_) {
+    if (n_ <= n) return;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    alloc(n_);
+  }
   ~DevBuf() {
     if (p) (void)hipFree(p);
   }
@@ -785,7 +832,7 @@ struct VjpBatch {
 inline void launch_vjp(Context &c, const double *Y, const double *pw,
                        const double *fpw, double *pb, double *fpb,
                        const double *mu, double *out, double *partials,
-                       double inv_eta0, hipStream_t s,
+                       double inv_eta0, DevBuf &bpart, hipStream_t s,
                        const VjpBatch *batch = nullptr) {
   const int pk = c.property_kind();
   if (pk == 1)
@@ -856,6 +903,17 @@ inline void launch_vjp(Context &c, const double *Y, const double *pw,
     ac[col].c.n_rows = ac[col].c.nseg * ac[col].c.lext[1] * ac[col].c.lext[2];
     blocks[col] = cells ? (unsigned)((ac[col].c.n_rows + 3) / 4) : 0u;
   }
+  // every block's partial sums, launch after launch, then their reduction
+  int32_t part_blocks = 0;
+  for (int col = 0; col < n_launch; ++col) {
+    ac[col].part_base = part_blocks;
+    part_blocks += (int32_t)blocks[col];
+  }
+  bpart.grow((size_t)3 * part_blocks * ny);
+  for (int col = 0; col < n_launch; ++col) {
+    ac[col].block_part = bpart.p;
+    ac[col].part_blocks = part_blocks;
+  }
 #define ADM_ADJ(P_, NS_, PK_)                                              \
   if (c.p == P_ && c.n_states == NS_ && pk == PK_) {                       \
     for (int col = 0; col < n_launch; ++col)                               \
@@ -904,6 +962,9 @@ inline void launch_vjp(Context &c, const double *Y, const double *pw,
   if (!launched)
     throw AdmError(ADM_ERR_UNSUPPORTED,
                    "no adjoint kernel for this degree / state count");
+  if (part_blocks > 0)
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(ny), dim3(64), 0, s,
+                       bpart.p, part_blocks, partials);
   ADM_HIP_ADJ(hipGetLastError());
 }
 
@@ -956,7 +1017,7 @@ struct AdjointRun {
   int64_t npw, nfpw;  // powder values of cells and faces
   int n_stages;
   double a[4][4], b[4], cst[4];  // Butcher tableau of the forward method
-  DevBuf partials;
+  DevBuf partials, bpart;
   DevBuf K[4], Y, R;
   DevBuf pbar, fpbar;  // adjoints of the carried cell and face powder
 
@@ -1037,7 +1098,7 @@ struct AdjointRun {
            double *out, double inv_eta0) {
     eval_time(t_eval);
     launch_vjp(c, rec.Y, rec.pw, rec.fpw, pbar.p, fpbar.p, mu, out,
-               partials.p, inv_eta0, s);
+               partials.p, inv_eta0, bpart, s);
   }
 };
 
@@ -1071,6 +1132,7 @@ struct EnsAdjoint {
   std::vector<int64_t> pw_off, fpw_off;
   int64_t npw = 0, nfpw = 0;
   DevBuf d_off, d_inv_eta;  // [pw_off | fpw_off] as int64, 1/eta_0
+  DevBuf bpart;             // per-block partial sums of the cell term
   VjpBatch batch;
 
   EnsAdjoint(Ensemble &e, int method, hipStream_t s_)
@@ -1187,7 +1249,7 @@ struct EnsAdjoint {
     for (const auto &m : E.members)
       batch.nz_active = std::max(batch.nz_active, m->nz_active);
     launch_vjp(*E.members[0], rec.Y, rec.pw, rec.fpw, pbar, fpbar, mu, out,
-               partials, 0.0, s, &batch);
+               partials, 0.0, bpart, s, &batch);
   }
 
   void activate(const EnsEvent &e, double *T) {
@@ -1686,7 +1748,7 @@ int adm_operator_vjp(void *ptr, double t, const double *T, const double *mu,
     if (!T || !mu || !out)
       return adm::set_error(ADM_ERR_INVALID, "null vector");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    adm::DevBuf partials;
+    adm::DevBuf partials, bpart;
     partials.alloc(3);
     ADM_HIP_ADJ(hipMemsetAsync(partials.p, 0, sizeof(double) * 3, s));
     ADM_HIP_ADJ(hipMemsetAsync(out, 0, sizeof(double) * c.n_dofs, s));
@@ -1694,7 +1756,8 @@ int adm_operator_vjp(void *ptr, double t, const double *T, const double *mu,
     const double eta0 =
         c.beams.empty() ? 0.0 : c.beams[0].desc.absorption_efficiency;
     adm::launch_vjp(c, T, c.d_powder, c.d_face_powder, nullptr, nullptr, mu,
-                    out, partials.p, eta0 != 0.0 ? 1.0 / eta0 : 0.0, s);
+                    out, partials.p, eta0 != 0.0 ? 1.0 / eta0 : 0.0, bpart,
+                    s);
     double pp[3];
     ADM_HIP_ADJ(hipMemcpyAsync(pp, partials.p, sizeof(pp),
                                hipMemcpyDeviceToHost, s));
