@@ -724,26 +724,6 @@ __global__ void __launch_bounds__(256)
 // vector kernels
 // ---------------------------------------------------------------------------
 
-// y = x (+ a * z)
-__global__ void axpy_copy_kernel(double *__restrict__ y,
-                                 const double *__restrict__ x, double a,
-                                 const double *__restrict__ z, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  y[i] = z ? 1. * x[i] + a * z[i] : x[i];
-}
-
-// k = R * inv_mass; R = 0
-__global__ void stage_value_kernel(double *__restrict__ k,
-                                   double *__restrict__ R,
-                                   const double *__restrict__ inv_mass,
-                                   int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  k[i] = R[i] * inv_mass[i];
-  R[i] = 0.0;
-}
-
 // y = 1 * y + a * x
 __global__ void axpy_kernel(double *__restrict__ y,
                             const double *__restrict__ x, double a,
@@ -1018,7 +998,7 @@ struct AdjointRun {
   int n_stages;
   double a[4][4], b[4], cst[4];  // Butcher tableau of the forward method
   DevBuf partials, bpart;
-  DevBuf K[4], Y, R;
+  DevBuf K[4];  // stage adjoints of the backward sweep
   DevBuf pbar, fpbar;  // adjoints of the carried cell and face powder
 
   AdjointRun(Context &c_, int method_, hipStream_t s_)
@@ -1032,9 +1012,6 @@ struct AdjointRun {
     std::memcpy(b, tb.b, sizeof(b));
     std::memcpy(cst, tb.c, sizeof(cst));
     for (int i = 0; i < n_stages; ++i) K[i].alloc(n);
-    Y.alloc(n);
-    R.alloc(n);
-    ADM_HIP_ADJ(hipMemsetAsync(R.p, 0, sizeof(double) * n, s));
     partials.alloc(3);
     pbar.alloc(npw);
     fpbar.alloc(nfpw);
@@ -1054,43 +1031,22 @@ struct AdjointRun {
     double *Y, *pw, *fpw;
   };
   void forward_step(double t, double dt, double *T, const StageRecord *rec) {
-    for (int i = 0; i < n_stages; ++i) {
-      // stage input: Y_i = T + sum_j dt a_ij K_j in j order (the zero
-      // entries skipped, as the stage kernels of the step do)
-      const double *Yi = T;
-      bool first = true;
-      for (int j = 0; j < i; ++j) {
-        if (a[i][j] == 0.0) continue;
-        if (first)
-          hipLaunchKernelGGL(axpy_copy_kernel, dim3(grid(n)), dim3(256), 0, s,
-                             Y.p, T, dt * a[i][j], K[j].p, n);
-        else
-          hipLaunchKernelGGL(axpy_kernel, dim3(grid(n)), dim3(256), 0, s, Y.p,
-                             K[j].p, dt * a[i][j], n);
-        first = false;
-        Yi = Y.p;
-      }
-      if (rec) {
-        ADM_HIP_ADJ(hipMemcpyAsync(rec[i].Y, Yi, sizeof(double) * n,
+    // adm_evolve's step (the gather pass applying the stage updates on the
+    // plane path), recording each stage's input and the powder before its
+    // evaluation
+    c.step_hooked(method, t, dt, T, s, [&](int i, const double *x) {
+      if (!rec) return;
+      ADM_HIP_ADJ(hipMemcpyAsync(rec[i].Y, x, sizeof(double) * n,
+                                 hipMemcpyDeviceToDevice, s));
+      if (npw)
+        ADM_HIP_ADJ(hipMemcpyAsync(rec[i].pw, c.d_powder,
+                                   sizeof(double) * npw,
                                    hipMemcpyDeviceToDevice, s));
-        if (npw)
-          ADM_HIP_ADJ(hipMemcpyAsync(rec[i].pw, c.d_powder,
-                                     sizeof(double) * npw,
-                                     hipMemcpyDeviceToDevice, s));
-        if (nfpw)
-          ADM_HIP_ADJ(hipMemcpyAsync(rec[i].fpw, c.d_face_powder,
-                                     sizeof(double) * nfpw,
-                                     hipMemcpyDeviceToDevice, s));
-      }
-      eval_time(t + cst[i] * dt);
-      c.apply(Yi, R.p, 0, s);
-      hipLaunchKernelGGL(stage_value_kernel, dim3(grid(n)), dim3(256), 0, s,
-                         K[i].p, R.p, c.sb.inv_mass, n);
-    }
-    // T' = T + dt sum b_i K_i, in stage order
-    for (int i = 0; i < n_stages; ++i)
-      hipLaunchKernelGGL(axpy_kernel, dim3(grid(n)), dim3(256), 0, s, T,
-                         K[i].p, dt * b[i], n);
+      if (nfpw)
+        ADM_HIP_ADJ(hipMemcpyAsync(rec[i].fpw, c.d_face_powder,
+                                   sizeof(double) * nfpw,
+                                   hipMemcpyDeviceToDevice, s));
+    });
     ADM_HIP_ADJ(hipGetLastError());
   }
 
@@ -1127,7 +1083,8 @@ struct EnsAdjoint {
   int N;
   int64_t n, nt;  // per member, all members
   Tableau tab;
-  DevBuf K[4], Y, R;
+  int method;
+  DevBuf K[4];  // stage adjoints of the backward sweep
   // powder of the cells (fixed slots) and of the faces (capacity per member)
   std::vector<int64_t> pw_off, fpw_off;
   int64_t npw = 0, nfpw = 0;
@@ -1135,13 +1092,10 @@ struct EnsAdjoint {
   DevBuf bpart;             // per-block partial sums of the cell term
   VjpBatch batch;
 
-  EnsAdjoint(Ensemble &e, int method, hipStream_t s_)
+  EnsAdjoint(Ensemble &e, int method_, hipStream_t s_)
       : E(e), s(s_), N(e.n_members), n(e.n_dofs), nt(e.n_dofs * e.n_members),
-        tab(method) {
+        tab(method_), method(method_) {
     for (int i = 0; i < tab.ns; ++i) K[i].alloc(nt);
-    Y.alloc(nt);
-    R.alloc(nt);
-    ADM_HIP_ADJ(hipMemsetAsync(R.p, 0, sizeof(double) * nt, s));
     pw_off.assign(N + 1, 0);
     for (int j = 0; j < N; ++j)
       pw_off[j + 1] = pw_off[j] + E.members[j]->n_powder_cells * E.members[j]->nq;
@@ -1206,33 +1160,12 @@ struct EnsAdjoint {
   // one batched step of every member (the stage sequence of
   // AdjointRun::forward_step)
   void forward_step(double t, double dt, double *T, const Rec *rec) {
-    for (int i = 0; i < tab.ns; ++i) {
-      const double *Yi = T;
-      bool first = true;
-      for (int j = 0; j < i; ++j) {
-        if (tab.a[i][j] == 0.0) continue;
-        if (first)
-          hipLaunchKernelGGL(axpy_copy_kernel, dim3(grid(nt)), dim3(256), 0, s,
-                             Y.p, T, dt * tab.a[i][j], K[j].p, nt);
-        else
-          hipLaunchKernelGGL(axpy_kernel, dim3(grid(nt)), dim3(256), 0, s, Y.p,
-                             K[j].p, dt * tab.a[i][j], nt);
-        first = false;
-        Yi = Y.p;
-      }
-      if (rec) {
-        ADM_HIP_ADJ(hipMemcpyAsync(rec[i].Y, Yi, sizeof(double) * nt,
-                                   hipMemcpyDeviceToDevice, s));
-        copy_powder(rec[i].pw, rec[i].fpw, true);
-      }
-      ens_time(E, t + tab.c[i] * dt);
-      E.apply(Yi, R.p, 0, s);
-      hipLaunchKernelGGL(stage_value_kernel, dim3(grid(nt)), dim3(256), 0, s,
-                         K[i].p, R.p, E.sb.inv_mass, nt);
-    }
-    for (int i = 0; i < tab.ns; ++i)
-      hipLaunchKernelGGL(axpy_kernel, dim3(grid(nt)), dim3(256), 0, s, T,
-                         K[i].p, dt * tab.b[i], nt);
+    E.step_hooked(method, t, dt, T, s, [&](int i, const double *x) {
+      if (!rec) return;
+      ADM_HIP_ADJ(hipMemcpyAsync(rec[i].Y, x, sizeof(double) * nt,
+                                 hipMemcpyDeviceToDevice, s));
+      copy_powder(rec[i].pw, rec[i].fpw, true);
+    });
     ADM_HIP_ADJ(hipGetLastError());
   }
 
@@ -1278,7 +1211,7 @@ int ensemble_adjoint(Ensemble &E, int method, double t0, double dt,
     if (!E.members[e.member]->device_activation_fits())
       throw AdmError(ADM_ERR_UNSUPPORTED,
                      "mesh too large for the device activation");
-  E.ensure_step_buffers(ADM_FORWARD_EULER);
+  E.ensure_step_buffers(method);
   std::stable_sort(ev.begin(), ev.end(), [](const EnsEvent &a,
                                             const EnsEvent &b) {
     return a.step < b.step;
@@ -1543,7 +1476,7 @@ int adm_adjoint_gradient(void *ptr, int method, double t0, double dt,
         return adm::set_error(ADM_ERR_INVALID, "observation DoF out of range");
     const int K = checkpoint_every > 0 ? checkpoint_every : n_steps;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    c.ensure_step_buffers(ADM_FORWARD_EULER);
+    c.ensure_step_buffers(method);
     adm::AdjointRun run(c, method, s);
     const int64_t n = run.n, npw = run.npw, nfpw = run.nfpw;
     const int n_seg = (n_steps + K - 1) / K;

@@ -1691,6 +1691,31 @@ void Context::step(int method, double t, double dt, double *T, hipStream_t s) {
       plane_path(), s);
 }
 
+void Context::step_hooked(
+    int method, double t, double dt, double *T, hipStream_t s,
+    const std::function<void(int, const double *)> &hook) {
+  int stage = 0;
+  auto at = [&](double ts, const double *x) {
+    if (hook) hook(stage, x);
+    ++stage;
+    for (size_t b = 0; b < old_segment.size(); ++b) old_segment[b] = 0;
+    set_time(ts);
+  };
+  explicit_step(
+      method, t, dt, T, sb,
+      [&](double ts, const double *x, double *r) {
+        at(ts, x);
+        apply(x, r, 0, s);
+      },
+      [&](double ts, const double *x, const StageArgs &st) {
+        at(ts, x);
+        CellArgs a = cell_args();
+        a.add = 0;
+        launch(a, face_args(), x, nullptr, 1, s, nz_active, nullptr, &st);
+      },
+      plane_path(), s);
+}
+
 // Stage time fractions of explicit_step's methods.
 static int n_stages(int method) {
   return method == ADM_FORWARD_EULER ? 1
@@ -1976,6 +2001,30 @@ void Ensemble::step(int method, double t, double dt, double *T,
         apply(x, nullptr, 0, s, &st);
       },
       members[0]->plane_path(), s);
+}
+
+void Ensemble::step_hooked(
+    int method, double t, double dt, double *T, hipStream_t s,
+    const std::function<void(int, const double *)> &hook) {
+  int stage = 0;
+  Context &c0 = *members[0];
+  auto at = [&](double ts, const double *x) {
+    if (hook) hook(stage, x);
+    ++stage;
+    for (size_t b = 0; b < c0.old_segment.size(); ++b) c0.old_segment[b] = 0;
+    set_time(ts);
+  };
+  explicit_step(
+      method, t, dt, T, sb,
+      [&](double ts, const double *x, double *r) {
+        at(ts, x);
+        apply(x, r, 0, s);
+      },
+      [&](double ts, const double *x, const StageArgs &st) {
+        at(ts, x);
+        apply(x, nullptr, 0, s, &st);
+      },
+      c0.plane_path(), s);
 }
 
 }  // namespace adm

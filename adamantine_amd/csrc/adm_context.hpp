@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -167,6 +168,11 @@ struct Context {
   void compute_inverse_mass(double *out, hipStream_t s);
   void ensure_step_buffers(int method);
   void step(int method, double t, double dt, double *T, hipStream_t s);
+  // one step as `step`, with hook(i, x) enqueued before stage i's operator
+  // evaluation (x: the stage input) and the beams set from the start of
+  // their scan paths at every stage time
+  void step_hooked(int method, double t, double dt, double *T, hipStream_t s,
+                   const std::function<void(int, const double *)> &hook);
   // n_steps steps as replays of one captured step (beams from a table)
   void evolve_graph(int method, double t, double dt, int n_steps, double *T,
                     hipStream_t s);
@@ -291,6 +297,9 @@ struct Ensemble {
   void compute_inverse_mass(double *out, hipStream_t s);
   void ensure_step_buffers(int method);
   void step(int method, double t, double dt, double *T, hipStream_t s);
+  // Context::step_hooked for every member at once
+  void step_hooked(int method, double t, double dt, double *T, hipStream_t s,
+                   const std::function<void(int, const double *)> &hook);
   void refresh_members();
 
   int n_members = 0;
