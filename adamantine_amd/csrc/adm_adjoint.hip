@@ -724,32 +724,59 @@ __global__ void __launch_bounds__(256)
 // vector kernels
 // ---------------------------------------------------------------------------
 
-// y = 1 * y + a * x
-__global__ void axpy_kernel(double *__restrict__ y,
-                            const double *__restrict__ x, double a,
-                            int64_t n) {
+// The vector work of a backward step on the first n_act entries of each
+// member's block of n (grid.y: the member): the nodes above the highest
+// active cell layer have a zero inverse mass, so their stage adjoints stay
+// zero and their T adjoint keeps its value.
+
+// K_i = (dt b_i) Tb for the S stages
+__global__ void adj_step_in_kernel(double *__restrict__ K0,
+                                   double *__restrict__ K1,
+                                   double *__restrict__ K2,
+                                   double *__restrict__ K3, double c0,
+                                   double c1, double c2, double c3, int S,
+                                   const double *__restrict__ Tb, int64_t n,
+                                   int64_t n_act) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  y[i] = 1. * y[i] + a * x[i];
+  if (i >= n_act) return;
+  const int64_t k = (int64_t)blockIdx.y * n + i;
+  const double tb = Tb[k];
+  K0[k] = c0 * tb;
+  if (S > 1) K1[k] = c1 * tb;
+  if (S > 2) K2[k] = c2 * tb;
+  if (S > 3) K3[k] = c3 * tb;
 }
 
-// y = a * x
-__global__ void scale_kernel(double *__restrict__ y,
-                             const double *__restrict__ x, double a,
-                             int64_t n) {
+// mu = 1 * K_s * inv_mass (M^-1 times the stage adjoint); Ybar = 0
+__global__ void adj_stage_in_kernel(double *__restrict__ mu,
+                                    double *__restrict__ Ybar,
+                                    const double *__restrict__ Ks,
+                                    const double *__restrict__ inv_mass,
+                                    int64_t n, int64_t n_act) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  y[i] = a * x[i];
+  if (i >= n_act) return;
+  const int64_t k = (int64_t)blockIdx.y * n + i;
+  mu[k] = 1.0 * Ks[k] * inv_mass[k];
+  Ybar[k] = 0.0;
 }
 
-// mu = a * inv_mass * v
-__global__ void scaled_mass_kernel(double *__restrict__ mu,
-                                   const double *__restrict__ v,
-                                   const double *__restrict__ inv_mass,
-                                   double a, int64_t n) {
+// Tb += Ybar, then K_j += c_j Ybar for the given K_j (the stage input's
+// adjoint to the earlier stages)
+__global__ void adj_stage_out_kernel(double *__restrict__ Tb,
+                                     const double *__restrict__ Ybar,
+                                     double *__restrict__ K0,
+                                     double *__restrict__ K1,
+                                     double *__restrict__ K2, double c0,
+                                     double c1, double c2, int64_t n,
+                                     int64_t n_act) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  mu[i] = a * v[i] * inv_mass[i];
+  if (i >= n_act) return;
+  const int64_t k = (int64_t)blockIdx.y * n + i;
+  const double y = Ybar[k];
+  Tb[k] = 1. * Tb[k] + 1.0 * y;
+  if (K0) K0[k] = 1. * K0[k] + c0 * y;
+  if (K1) K1[k] = 1. * K1[k] + c1 * y;
+  if (K2) K2[k] = 1. * K2[k] + c2 * y;
 }
 
 // ---------------------------------------------------------------------------
@@ -783,6 +810,44 @@ struct DevBuf {
     if (p) (void)hipFree(p);
   }
 };
+
+// The vector kernels of one backward step of the explicit method
+// (a, b: its tableau; K: its stage adjoints), over `members` blocks of n
+// entries of which the first n_act are touched; vjp(i) runs stage i's
+// transposed operator on mu into Ybar.
+template <class Vjp>
+void backward_step(int S, const double (&a)[4][4], const double (&b)[4],
+                   double dt, DevBuf (&K)[4], double *Tb, double *mu,
+                   double *Ybar, const double *inv_mass, int64_t n,
+                   int64_t n_act, int members, hipStream_t s, Vjp &&vjp) {
+  const dim3 g((unsigned)((n_act + 255) / 256), (unsigned)members);
+  // stage adjoints start at dt b_i Tb; Tb_new = Tb + sum Ybar_i;
+  // K_j += dt a_ij Ybar_i for j < i (reverse stage order)
+  hipLaunchKernelGGL(adj_step_in_kernel, g, dim3(256), 0, s, K[0].p, K[1].p,
+                     K[2].p, K[3].p, dt * b[0], dt * b[1], dt * b[2],
+                     dt * b[3], S, Tb, n, n_act);
+  for (int i = S - 1; i >= 0; --i) {
+    hipLaunchKernelGGL(adj_stage_in_kernel, g, dim3(256), 0, s, mu, Ybar,
+                       K[i].p, inv_mass, n, n_act);
+    vjp(i);
+    double *kj[3] = {nullptr, nullptr, nullptr};
+    double cj[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < i; ++j)
+      if (a[i][j] != 0.0) {
+        kj[j] = K[j].p;
+        cj[j] = dt * a[i][j];
+      }
+    hipLaunchKernelGGL(adj_stage_out_kernel, g, dim3(256), 0, s, Tb, Ybar,
+                       kj[0], kj[1], kj[2], cj[0], cj[1], cj[2], n, n_act);
+  }
+  ADM_HIP_ADJ(hipGetLastError());
+}
+
+// nodes in the cell layers up to nz (and the node layer above them)
+inline int64_t active_prefix(const Context &c, int nz) {
+  const int64_t layers = std::min<int64_t>(c.nn[2], (int64_t)c.p * nz + 1);
+  return (int64_t)c.nn[0] * c.nn[1] * layers;
+}
 
 // beam states of the context at time t (segment search from the start)
 inline void eval_time_reset(Context &c, double t) {
@@ -1020,8 +1085,6 @@ struct AdjointRun {
       ADM_HIP_ADJ(hipMemsetAsync(fpbar.p, 0, sizeof(double) * nfpw, s));
   }
 
-  unsigned grid(int64_t m) const { return (unsigned)((m + 255) / 256); }
-
   void eval_time(double t) { eval_time_reset(c, t); }
 
   // one forward step T -> T' (stage inputs Y_i, stage values K_i); with
@@ -1101,8 +1164,6 @@ struct EnsAdjoint {
       pw_off[j + 1] = pw_off[j] + E.members[j]->n_powder_cells * E.members[j]->nq;
     npw = pw_off[N];
   }
-
-  unsigned grid(int64_t m) const { return (unsigned)((m + 255) / 256); }
 
   // device tables once the face capacities are known
   void finish_tables(const std::vector<int64_t> &fcap) {
@@ -1364,7 +1425,6 @@ int ensemble_adjoint(Ensemble &E, int method, double t0, double dt,
     rec[k].pw = run.npw ? recP.p + k * run.npw : nullptr;
     rec[k].fpw = run.nfpw ? recF.p + k * run.nfpw : nullptr;
   }
-  const unsigned gn = run.grid(nt);
   int ev_i = (int)ev.size() - 1;
   for (int i = n_seg - 1; i >= 0; --i) {
     const int k0 = bnd[i], k1 = bnd[i + 1];
@@ -1383,24 +1443,16 @@ int ensemble_adjoint(Ensemble &E, int method, double t0, double dt,
     cp_faces(cpF[i], false);
     for (int k = k0; k < k1; ++k)
       run.forward_step(times[k], dt, T.p, &rec[(size_t)(k - k0) * S]);
+    int nz = 0;
+    for (const auto &m : E.members) nz = std::max(nz, m->nz_active);
+    const int64_t n_act = active_prefix(*E.members[0], nz);
     for (int k = k1 - 1; k >= k0; --k) {
       const EnsAdjoint::Rec *rk = &rec[(size_t)(k - k0) * S];
-      for (int st = 0; st < S; ++st)
-        hipLaunchKernelGGL(scale_kernel, dim3(gn), dim3(256), 0, s,
-                           run.K[st].p, Tb.p, dt * run.tab.b[st], nt);
-      for (int st = S - 1; st >= 0; --st) {
-        hipLaunchKernelGGL(scaled_mass_kernel, dim3(gn), dim3(256), 0, s, mu.p,
-                           run.K[st].p, E.sb.inv_mass, 1.0, nt);
-        ADM_HIP_ADJ(hipMemsetAsync(Ybar.p, 0, sizeof(double) * nt, s));
-        run.vjp(rk[st], times[k] + run.tab.c[st] * dt, mu.p, Ybar.p, pbar.p,
-                fpbar.p, partials.p);
-        hipLaunchKernelGGL(axpy_kernel, dim3(gn), dim3(256), 0, s, Tb.p,
-                           Ybar.p, 1.0, nt);
-        for (int jj = 0; jj < st; ++jj)
-          if (run.tab.a[st][jj] != 0.0)
-            hipLaunchKernelGGL(axpy_kernel, dim3(gn), dim3(256), 0, s,
-                               run.K[jj].p, Ybar.p, dt * run.tab.a[st][jj], nt);
-      }
+      backward_step(S, run.tab.a, run.tab.b, dt, run.K, Tb.p, mu.p, Ybar.p,
+                    E.sb.inv_mass, n, n_act, N, s, [&](int st) {
+                      run.vjp(rk[st], times[k] + run.tab.c[st] * dt, mu.p,
+                              Ybar.p, pbar.p, fpbar.p, partials.p);
+                    });
     }
     // the activations at the start of the segment, last first
     for (; ev_i >= 0 && ev[ev_i].step == k0; --ev_i) {
@@ -1570,7 +1622,6 @@ int adm_adjoint_gradient(void *ptr, int method, double t0, double dt,
     const double eta0 =
         c.beams.empty() ? 0.0 : c.beams[0].desc.absorption_efficiency;
     const double inv_eta0 = eta0 != 0.0 ? 1.0 / eta0 : 0.0;
-    const unsigned gn = run.grid(n);
     // backward sweep over the segments
     for (int sg = n_seg - 1; sg >= 0; --sg) {
       const int k0 = sg * K, k1 = std::min(n_steps, k0 + K);
@@ -1587,25 +1638,14 @@ int adm_adjoint_gradient(void *ptr, int method, double t0, double dt,
                                    hipMemcpyDeviceToDevice, s));
       for (int k = k0; k < k1; ++k)
         run.forward_step(times[k], dt, T.p, &rec[(size_t)(k - k0) * S]);
+      const int64_t n_act = adm::active_prefix(c, c.nz_active);
       for (int k = k1 - 1; k >= k0; --k) {
         const adm::AdjointRun::StageRecord *rk = &rec[(size_t)(k - k0) * S];
-        // stage adjoints kb_i start at dt b_i Tb; Tb_new = Tb + sum Ybar_i;
-        // kb_j += dt a_ij Ybar_i for j < i (reverse stage order)
-        for (int i = 0; i < S; ++i)
-          hipLaunchKernelGGL(adm::scale_kernel, dim3(gn), dim3(256), 0, s,
-                             run.K[i].p, Tb.p, dt * run.b[i], n);
-        for (int i = S - 1; i >= 0; --i) {
-          hipLaunchKernelGGL(adm::scaled_mass_kernel, dim3(gn), dim3(256), 0,
-                             s, mu.p, run.K[i].p, c.sb.inv_mass, 1.0, n);
-          ADM_HIP_ADJ(hipMemsetAsync(Ybar.p, 0, sizeof(double) * n, s));
-          run.vjp(rk[i], times[k] + run.cst[i] * dt, mu.p, Ybar.p, inv_eta0);
-          hipLaunchKernelGGL(adm::axpy_kernel, dim3(gn), dim3(256), 0, s, Tb.p,
-                             Ybar.p, 1.0, n);
-          for (int j = 0; j < i; ++j)
-            if (run.a[i][j] != 0.0)
-              hipLaunchKernelGGL(adm::axpy_kernel, dim3(gn), dim3(256), 0, s,
-                                 run.K[j].p, Ybar.p, dt * run.a[i][j], n);
-        }
+        adm::backward_step(S, run.a, run.b, dt, run.K, Tb.p, mu.p, Ybar.p,
+                           c.sb.inv_mass, n, n_act, 1, s, [&](int i) {
+                             run.vjp(rk[i], times[k] + run.cst[i] * dt, mu.p,
+                                     Ybar.p, inv_eta0);
+                           });
       }
     }
     copy_state(false);
