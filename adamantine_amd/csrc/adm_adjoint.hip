@@ -30,6 +30,7 @@
 #include <cstring>
 #include <string>
 #include <memory>
+#include <tuple>
 #include <vector>
 
 #include "../../include/adm.h"
@@ -724,6 +725,24 @@ __global__ void __launch_bounds__(256)
 // vector kernels
 // ---------------------------------------------------------------------------
 
+// Up to 32 device-to-device copies in one launch (grid.y: the copy).
+constexpr int kSegCopies = 32;
+struct SegCopyArgs {
+  const double *src[kSegCopies];
+  double *dst[kSegCopies];
+  int64_t len[kSegCopies];
+};
+__global__ void __launch_bounds__(256)
+    seg_copy_kernel(const SegCopyArgs a) {
+  const int k = blockIdx.y;
+  const double *__restrict__ src = a.src[k];
+  double *__restrict__ dst = a.dst[k];
+  const int64_t len = a.len[k];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 // The vector work of a backward step on the first n_act entries of each
 // member's block of n (grid.y: the member): the nodes above the highest
 // active cell layer have a zero inverse mass, so their stage adjoints stay
@@ -1197,22 +1216,42 @@ struct EnsAdjoint {
 
   // members' powder <-> a packed copy (cells: fixed slots; faces: the
   // current face count of each member, at its offset)
+  // copies (src, dst, length), batched kSegCopies to a launch
+  std::vector<std::tuple<const double *, double *, int64_t>> copies;
+  void flush_copies() {
+    for (size_t b = 0; b < copies.size(); b += kSegCopies) {
+      SegCopyArgs a;
+      std::memset(&a, 0, sizeof(a));
+      const int m = (int)std::min<size_t>(kSegCopies, copies.size() - b);
+      int64_t longest = 0;
+      for (int k = 0; k < m; ++k) {
+        a.src[k] = std::get<0>(copies[b + k]);
+        a.dst[k] = std::get<1>(copies[b + k]);
+        a.len[k] = std::get<2>(copies[b + k]);
+        longest = std::max(longest, a.len[k]);
+      }
+      if (longest == 0) continue;
+      const unsigned gx =
+          (unsigned)std::min<int64_t>((longest + 255) / 256, 2048);
+      hipLaunchKernelGGL(seg_copy_kernel, dim3(gx, (unsigned)m), dim3(256), 0,
+                         s, a);
+    }
+    copies.clear();
+    ADM_HIP_ADJ(hipGetLastError());
+  }
   void copy_powder(double *pw, double *fpw, bool save) {
     for (int j = 0; j < N; ++j) {
       Context &c = *E.members[j];
       const int64_t a = pw_off[j + 1] - pw_off[j];
       if (pw && a)
-        ADM_HIP_ADJ(hipMemcpyAsync(save ? pw + pw_off[j] : c.d_powder,
-                                   save ? c.d_powder : pw + pw_off[j],
-                                   sizeof(double) * a,
-                                   hipMemcpyDeviceToDevice, s));
+        copies.emplace_back(save ? c.d_powder : pw + pw_off[j],
+                            save ? pw + pw_off[j] : c.d_powder, a);
       const int64_t f = c.n_states == 3 ? c.n_faces * c.nfq : 0;
       if (fpw && f)
-        ADM_HIP_ADJ(hipMemcpyAsync(save ? fpw + fpw_off[j] : c.d_face_powder,
-                                   save ? c.d_face_powder : fpw + fpw_off[j],
-                                   sizeof(double) * f,
-                                   hipMemcpyDeviceToDevice, s));
+        copies.emplace_back(save ? c.d_face_powder : fpw + fpw_off[j],
+                            save ? fpw + fpw_off[j] : c.d_face_powder, f);
     }
+    flush_copies();
   }
 
   struct Rec {
@@ -1221,10 +1260,16 @@ struct EnsAdjoint {
   // one batched step of every member (the stage sequence of
   // AdjointRun::forward_step)
   void forward_step(double t, double dt, double *T, const Rec *rec) {
+    // the stage inputs' nodes up to the highest active cell layer (the
+    // transposed operator reads no other)
+    int nz = 0;
+    for (const auto &m : E.members) nz = std::max(nz, m->nz_active);
+    const int64_t n_act = active_prefix(*E.members[0], nz);
     E.step_hooked(method, t, dt, T, s, [&](int i, const double *x) {
       if (!rec) return;
-      ADM_HIP_ADJ(hipMemcpyAsync(rec[i].Y, x, sizeof(double) * nt,
-                                 hipMemcpyDeviceToDevice, s));
+      for (int j = 0; j < N; ++j)
+        copies.emplace_back(x + (int64_t)j * n, rec[i].Y + (int64_t)j * n,
+                            n_act);
       copy_powder(rec[i].pw, rec[i].fpw, true);
     });
     ADM_HIP_ADJ(hipGetLastError());
