@@ -868,6 +868,91 @@ inline int64_t active_prefix(const Context &c, int nz) {
   return (int64_t)c.nn[0] * c.nn[1] * layers;
 }
 
+// vals[j n_obs + k] = T[j n + dof[k]] (grid.y: member j)
+__global__ void gather_obs_kernel(const double *__restrict__ T, int64_t n,
+                                  const int64_t *__restrict__ dof, int n_obs,
+                                  double *__restrict__ vals) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_obs) return;
+  vals[(int64_t)blockIdx.y * n_obs + k] = T[(int64_t)blockIdx.y * n + dof[k]];
+}
+
+// Tb[j n + dof[u]] = val[j n_u + u] (grid.y: member j)
+__global__ void scatter_obs_kernel(double *__restrict__ Tb, int64_t n,
+                                   const int64_t *__restrict__ dof, int n_u,
+                                   const double *__restrict__ val) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n_u) return;
+  Tb[(int64_t)blockIdx.y * n + dof[u]] = val[(int64_t)blockIdx.y * n_u + u];
+}
+
+// The misfits J_j = 1/2 sum_k w_k (T_j[obs_dof_k] - obs_jk)^2 of the
+// `members` blocks of n in T (obs: members x n_obs), and Tb = dJ/dT: zero
+// but at the observed nodes, where each member's weighted residuals are
+// summed in observation order. Only the observed values cross the bus.
+inline void misfit_gradient(const double *T, int members, int64_t n,
+                            int n_obs, const int64_t *obs_dof,
+                            const double *obs, const double *obs_weight,
+                            double *J, double *Tb, hipStream_t s) {
+  // the distinct observed nodes, in order of first appearance
+  std::vector<int64_t> udof;
+  std::vector<int> slot((size_t)n_obs);
+  {
+    std::vector<std::pair<int64_t, int>> order((size_t)n_obs);
+    for (int k = 0; k < n_obs; ++k) order[k] = {obs_dof[k], k};
+    std::stable_sort(order.begin(), order.end(),
+                     [](const std::pair<int64_t, int> &x,
+                        const std::pair<int64_t, int> &y) {
+                       return x.first < y.first;
+                     });
+    for (size_t i = 0; i < order.size(); ++i) {
+      if (i == 0 || order[i].first != order[i - 1].first)
+        udof.push_back(order[i].first);
+      slot[order[i].second] = (int)udof.size() - 1;
+    }
+  }
+  const int n_u = (int)udof.size();
+  DevBuf d_dof, d_udof, d_vals;  // int64 tables in double-sized slots
+  d_dof.alloc((size_t)n_obs);
+  d_udof.alloc((size_t)n_u);
+  d_vals.alloc((size_t)members * std::max(n_obs, n_u));
+  int64_t *dof_p = reinterpret_cast<int64_t *>(d_dof.p);
+  int64_t *udof_p = reinterpret_cast<int64_t *>(d_udof.p);
+  ADM_HIP_ADJ(hipMemcpyAsync(dof_p, obs_dof, sizeof(int64_t) * n_obs,
+                             hipMemcpyHostToDevice, s));
+  ADM_HIP_ADJ(hipMemcpyAsync(udof_p, udof.data(), sizeof(int64_t) * n_u,
+                             hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(gather_obs_kernel,
+                     dim3((unsigned)((n_obs + 255) / 256), (unsigned)members),
+                     dim3(256), 0, s, T, n, dof_p, n_obs, d_vals.p);
+  std::vector<double> vals((size_t)members * n_obs);
+  ADM_HIP_ADJ(hipMemcpyAsync(vals.data(), d_vals.p,
+                             sizeof(double) * vals.size(),
+                             hipMemcpyDeviceToHost, s));
+  ADM_HIP_ADJ(hipStreamSynchronize(s));
+  std::vector<double> uval((size_t)members * n_u, 0.0);
+  for (int j = 0; j < members; ++j) {
+    double Jv = 0.0;
+    for (int k = 0; k < n_obs; ++k) {
+      const double w = obs_weight ? obs_weight[k] : 1.0;
+      const double r = vals[(size_t)j * n_obs + k] - obs[(size_t)j * n_obs + k];
+      Jv += 0.5 * w * r * r;
+      uval[(size_t)j * n_u + slot[k]] += w * r;
+    }
+    J[j] = Jv;
+  }
+  ADM_HIP_ADJ(hipMemsetAsync(Tb, 0, sizeof(double) * n * members, s));
+  ADM_HIP_ADJ(hipMemcpyAsync(d_vals.p, uval.data(),
+                             sizeof(double) * uval.size(),
+                             hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(scatter_obs_kernel,
+                     dim3((unsigned)((n_u + 255) / 256), (unsigned)members),
+                     dim3(256), 0, s, Tb, n, udof_p, n_u, d_vals.p);
+  ADM_HIP_ADJ(hipGetLastError());
+  // the host vectors above are read by the copies before they return
+  ADM_HIP_ADJ(hipStreamSynchronize(s));
+}
+
 // beam states of the context at time t (segment search from the start)
 inline void eval_time_reset(Context &c, double t) {
   for (size_t b = 0; b < c.old_segment.size(); ++b) c.old_segment[b] = 0;
@@ -1431,21 +1516,6 @@ int ensemble_adjoint(Ensemble &E, int method, double t0, double dt,
     ADM_HIP_ADJ(hipMemcpyAsync(T_final, T.p, sizeof(double) * nt,
                                hipMemcpyDeviceToDevice, s));
   // misfits and their gradients
-  std::vector<double> Tf((size_t)nt);
-  ADM_HIP_ADJ(hipMemcpyAsync(Tf.data(), T.p, sizeof(double) * nt,
-                             hipMemcpyDeviceToHost, s));
-  ADM_HIP_ADJ(hipStreamSynchronize(s));
-  std::vector<double> tb((size_t)nt, 0.0);
-  for (int j = 0; j < N; ++j) {
-    double Jv = 0.0;
-    for (int k = 0; k < n_obs; ++k) {
-      const double w = obs_weight ? obs_weight[k] : 1.0;
-      const double r = Tf[(size_t)j * n + obs_dof[k]] - obs[(size_t)j * n_obs + k];
-      Jv += 0.5 * w * r * r;
-      tb[(size_t)j * n + obs_dof[k]] += w * r;
-    }
-    J[j] = Jv;
-  }
   DevBuf Tb, mu, Ybar, partials, pbar, fpbar;
   Tb.alloc(nt);
   mu.alloc(nt);
@@ -1453,8 +1523,7 @@ int ensemble_adjoint(Ensemble &E, int method, double t0, double dt,
   partials.alloc(3 * (size_t)N);
   pbar.alloc(run.npw);
   fpbar.alloc(run.nfpw);
-  ADM_HIP_ADJ(hipMemcpyAsync(Tb.p, tb.data(), sizeof(double) * nt,
-                             hipMemcpyHostToDevice, s));
+  misfit_gradient(T.p, N, n, n_obs, obs_dof, obs, obs_weight, J, Tb.p, s);
   ADM_HIP_ADJ(hipMemsetAsync(partials.p, 0, sizeof(double) * 3 * N, s));
   if (run.npw) ADM_HIP_ADJ(hipMemsetAsync(pbar.p, 0, sizeof(double) * run.npw, s));
   if (run.nfpw)
@@ -1648,21 +1717,8 @@ int adm_adjoint_gradient(void *ptr, int method, double t0, double dt,
       ADM_HIP_ADJ(hipMemcpyAsync(T_final, T.p, sizeof(double) * n,
                                  hipMemcpyDeviceToDevice, s));
     // misfit and its gradient at the final time
-    std::vector<double> Tf((size_t)n);
-    ADM_HIP_ADJ(hipMemcpyAsync(Tf.data(), T.p, sizeof(double) * n,
-                               hipMemcpyDeviceToHost, s));
-    ADM_HIP_ADJ(hipStreamSynchronize(s));
-    std::vector<double> tb((size_t)n, 0.0);
-    double Jv = 0.0;
-    for (int k = 0; k < n_obs; ++k) {
-      const double w = obs_weight ? obs_weight[k] : 1.0;
-      const double r = Tf[obs_dof[k]] - obs[k];
-      Jv += 0.5 * w * r * r;
-      tb[obs_dof[k]] += w * r;
-    }
-    *J = Jv;
-    ADM_HIP_ADJ(hipMemcpyAsync(Tb.p, tb.data(), sizeof(double) * n,
-                               hipMemcpyHostToDevice, s));
+    adm::misfit_gradient(T.p, 1, n, n_obs, obs_dof, obs, obs_weight, J, Tb.p,
+                         s);
     ADM_HIP_ADJ(hipMemsetAsync(run.partials.p, 0, sizeof(double) * 3, s));
     const double eta0 =
         c.beams.empty() ? 0.0 : c.beams[0].desc.absorption_efficiency;
