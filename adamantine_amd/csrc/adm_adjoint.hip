@@ -611,10 +611,12 @@ __global__ void __launch_bounds__(64)
     for (int k = 0; k < 3; ++k) partials[3 * blockIdx.x + k] += v[k];
 }
 
-// (dA_face/dT)^T mu over the boundary faces (one lane per face, as
-// face_kernel), with the face powder recorded before the evaluation and the
-// adjoint of the carried face powder (face_powder_bar, NULL: zero, not
-// stored) as in the cell kernel.
+// (dA_face/dT)^T mu over the boundary faces, with the face powder recorded
+// before the evaluation and the adjoint of the carried face powder
+// (face_powder_bar, NULL: zero, not stored) as in the cell kernel. Lanes as
+// face_kernel: (p+1)^2 per face, lane q loading node q, evaluating point q
+// (T and mu by the same S chains as the forward face kernel) and, after the
+// points' terms are shared, integrating node q.
 template <int P, int NS, int PK>
 __global__ void __launch_bounds__(256)
     adjoint_face_kernel(const FaceArgs a, const double *__restrict__ T,
@@ -623,11 +625,17 @@ __global__ void __launch_bounds__(256)
                         double *__restrict__ face_powder_bar,
                         double *__restrict__ out, const AdjFaceArgs af) {
   constexpr int N = P + 1, NF = N * N;
-  const int64_t fi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr int FPW = FaceLanes<P>::faces_per_wave;
+  __shared__ double xf[4][2][FPW * NF];
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const int jf = lane / NF, q = lane % NF;
+  const int qa = q % N, qb = q / N;
   // one operator, or member blockIdx.y of an ensemble
   const ADM_CONSTANT MemberArgs *mb =
       a.members ? as_constant(a.members) + blockIdx.y : nullptr;
-  if (fi >= (mb ? mb->n_faces : a.n_faces)) return;
+  const int64_t fi = ((int64_t)blockIdx.x * 4 + wave) * FPW + jf;
+  // whole faces (all lanes of a face) agree
+  if (jf >= FPW || fi >= (mb ? mb->n_faces : a.n_faces)) return;
   if (mb) {
     const int64_t voff = (int64_t)blockIdx.y * a.member_stride;
     const int64_t foff = af.fpw_off[blockIdx.y];
@@ -651,17 +659,30 @@ __global__ void __launch_bounds__(256)
   const int64_t base = (int64_t)cc[0] * P + ns[1] * cc[1] * P +
                        ns[2] * cc[2] * P + (side ? P : 0) * ns[d];
   const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
-  double fu[NF], fm[NF], ft[NF], fv[NF], gv[NF];
+  const int64_t node = base + qa * ns[t1] + qb * ns[t2];
+  double *fx = &xf[wave][0][jf * NF], *fy = &xf[wave][1][jf * NF];
+  fx[q] = T[node];
+  fy[q] = mu[node];
+  wave_lds_sync();
+  // T and mu at point (qa, qb): along a, then along b
+  double ft[N], fm[N];
 #pragma unroll
-  for (int jj = 0; jj < NF; ++jj) {
-    const int64_t node = base + (jj % N) * ns[t1] + (jj / N) * ns[t2];
-    fu[jj] = T[node];
-    fm[jj] = mu[node];
+  for (int bb = 0; bb < N; ++bb) {
+    double s0 = kc.S[qa][0] * fx[N * bb], s1 = kc.S[qa][0] * fy[N * bb];
+#pragma unroll
+    for (int k = 1; k < N; ++k) {
+      s0 = __builtin_fma(kc.S[qa][k], fx[k + N * bb], s0);
+      s1 = __builtin_fma(kc.S[qa][k], fy[k + N * bb], s1);
+    }
+    ft[bb] = s0;
+    fm[bb] = s1;
   }
-  sweep<N, 0, false, false, 2>(kc.S, fu, ft);
-  sweep<N, 1, false, false, 2>(kc.S, ft, fv);
-  sweep<N, 0, false, false, 2>(kc.S, fm, ft);
-  sweep<N, 1, false, false, 2>(kc.S, ft, gv);
+  double Tq = kc.S[qb][0] * ft[0], gq = kc.S[qb][0] * fm[0];
+#pragma unroll
+  for (int k = 1; k < N; ++k) {
+    Tq = __builtin_fma(kc.S[qb][k], ft[k], Tq);
+    gq = __builtin_fma(kc.S[qb][k], fm[k], gq);
+  }
   const double sol = m.prop[mid][ADM_SOLIDUS], liq = m.prop[mid][ADM_LIQUIDUS];
   const double latent = m.latent_factor[mid];
   const double *mB = &m.merged[mid].B[0];
@@ -669,56 +690,59 @@ __global__ void __launch_bounds__(256)
   const double *fp = NS == 3 ? face_powder_old + fi * NF : nullptr;
   double *fpb = (NS == 3 && face_powder_bar) ? face_powder_bar + fi * NF
                                              : nullptr;
-#pragma unroll
-  for (int q = 0; q < NF; ++q) {
-    const double Tq = fv[q];
-    Mix<NS> x;
-    double dl;
-    bool clamped;
-    ratios_lin<NS>(Tq, fp ? fp[q] : 0.0, sol, liq, x, dl, clamped);
-    double irc, ircT, ircp;
-    const int kt = PK == 2 ? merged_search(mB, Tq) : 0;
-    inv_rho_cp_lin<NS, PK>(m, mid, mB, mC, Tq, kt, latent, x, irc, ircT,
-                           ircp);
-    // s = h (T - T_conv) + eps sigma (T + T_rad)(T^2 + T_rad^2)(T - T_rad)
-    double s1 = 0.0, s1T = 0.0, s1p = 0.0;
-    if (bt & ADM_BC_CONVECTIVE) {
-      const double cT = m.prop[mid][ADM_CONVECTION_TEMPERATURE_INFTY];
-      double h, hT, hp;
-      prop_vd<NS, PK>(m, mid, mB, mC, Tq, kt,
-                      ADM_CONVECTION_HEAT_TRANSFER_COEF, x, h, hT, hp);
-      s1 += h * (Tq - cT);
-      s1T += hT * (Tq - cT) + h;
-      s1p += hp * (Tq - cT);
-    }
-    if (bt & ADM_BC_RADIATIVE) {
-      const double rT = m.prop[mid][ADM_RADIATION_TEMPERATURE_INFTY];
-      double e, eT, ep;
-      prop_vd<NS, PK>(m, mid, mB, mC, Tq, kt, ADM_EMISSIVITY, x, e, eT, ep);
-      const double a1 = Tq + rT, a2 = Tq * Tq + rT * rT;
-      const double g = kStefanBoltzmann * a1 * a2 * (Tq - rT);
-      const double gT = kStefanBoltzmann *
-                        (a2 * (Tq - rT) + a1 * 2.0 * Tq * (Tq - rT) + a1 * a2);
-      s1 += e * g;
-      s1T += eT * g + e * gT;
-      s1p += ep * g;
-    }
-    const double w = gv[q] * kc.fjxw[d][q];
-    double c_T = (-ircT * s1 - irc * s1T) * w;
-    if constexpr (NS == 3) {
-      const double tot = (-ircp * s1 - irc * s1p) * w + (fpb ? fpb[q] : 0.0);
-      if (clamped) c_T -= dl * tot;
-      if (fpb) fpb[q] = clamped ? 0.0 : tot;
-    }
-    fm[q] = c_T;
+  Mix<NS> x;
+  double dl;
+  bool clamped;
+  ratios_lin<NS>(Tq, fp ? fp[q] : 0.0, sol, liq, x, dl, clamped);
+  double irc, ircT, ircp;
+  const int kt = PK == 2 ? merged_search(mB, Tq) : 0;
+  inv_rho_cp_lin<NS, PK>(m, mid, mB, mC, Tq, kt, latent, x, irc, ircT, ircp);
+  // s = h (T - T_conv) + eps sigma (T + T_rad)(T^2 + T_rad^2)(T - T_rad)
+  double s1 = 0.0, s1T = 0.0, s1p = 0.0;
+  if (bt & ADM_BC_CONVECTIVE) {
+    const double cT = m.prop[mid][ADM_CONVECTION_TEMPERATURE_INFTY];
+    double h, hT, hp;
+    prop_vd<NS, PK>(m, mid, mB, mC, Tq, kt, ADM_CONVECTION_HEAT_TRANSFER_COEF,
+                    x, h, hT, hp);
+    s1 += h * (Tq - cT);
+    s1T += hT * (Tq - cT) + h;
+    s1p += hp * (Tq - cT);
   }
-  sweep<N, 1, true, false, 2>(kc.S, fm, ft);
-  sweep<N, 0, true, false, 2>(kc.S, ft, fv);
-#pragma unroll
-  for (int jj = 0; jj < NF; ++jj) {
-    const int64_t node = base + (jj % N) * ns[t1] + (jj / N) * ns[t2];
-    unsafeAtomicAdd(out + node, fv[jj]);
+  if (bt & ADM_BC_RADIATIVE) {
+    const double rT = m.prop[mid][ADM_RADIATION_TEMPERATURE_INFTY];
+    double e, eT, ep;
+    prop_vd<NS, PK>(m, mid, mB, mC, Tq, kt, ADM_EMISSIVITY, x, e, eT, ep);
+    const double a1 = Tq + rT, a2 = Tq * Tq + rT * rT;
+    const double g = kStefanBoltzmann * a1 * a2 * (Tq - rT);
+    const double gT = kStefanBoltzmann *
+                      (a2 * (Tq - rT) + a1 * 2.0 * Tq * (Tq - rT) + a1 * a2);
+    s1 += e * g;
+    s1T += eT * g + e * gT;
+    s1p += ep * g;
   }
+  const double w = gq * kc.fjxw[d][q];
+  double c_T = (-ircT * s1 - irc * s1T) * w;
+  if constexpr (NS == 3) {
+    const double tot = (-ircp * s1 - irc * s1p) * w + (fpb ? fpb[q] : 0.0);
+    if (clamped) c_T -= dl * tot;
+    if (fpb) fpb[q] = clamped ? 0.0 : tot;
+  }
+  wave_lds_sync();
+  fx[q] = c_T;
+  wave_lds_sync();
+  // node (qa, qb): S^T along b, then along a
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double acc = kc.S[0][qb] * fx[i];
+#pragma unroll
+    for (int k = 1; k < N; ++k)
+      acc = __builtin_fma(kc.S[k][qb], fx[i + N * k], acc);
+    ft[i] = acc;
+  }
+  double o = kc.S[0][qa] * ft[0];
+#pragma unroll
+  for (int i = 1; i < N; ++i) o = __builtin_fma(kc.S[i][qa], ft[i], o);
+  unsafeAtomicAdd(out + node, o);
 }
 
 // ---------------------------------------------------------------------------
@@ -741,6 +765,11 @@ __global__ void __launch_bounds__(256)
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len;
        i += (int64_t)gridDim.x * blockDim.x)
     dst[i] = src[i];
+}
+
+// faces per block of the adjoint face kernel (FaceLanes<P>::faces_per_block)
+inline int FaceLanes_per_block(int p) {
+  return 4 * (64 / ((p + 1) * (p + 1)));
 }
 
 // The vector work of a backward step on the first n_act entries of each
@@ -1025,7 +1054,8 @@ inline void launch_vjp(Context &c, const double *Y, const double *pw,
                : 0) +
       sizeof(double) * 4 * cpw *
           4 * (c.p + 1) * (c.p + 1) * (c.p + 1);
-  const unsigned face_blocks = (unsigned)((fa.n_faces + 255) / 256);
+  const unsigned face_blocks = (unsigned)(
+      (fa.n_faces + FaceLanes_per_block(c.p) - 1) / FaceLanes_per_block(c.p));
   // one launch with atomic adds, or (ADM_ADJOINT_LAUNCH=colored in the
   // environment) eight launches, one per parity colour, each adding its
   // nodes' terms with a plain read-add-write (cells of one colour share no
@@ -1688,7 +1718,11 @@ int adm_adjoint_gradient(void *ptr, int method, double t0, double dt,
                                      c.d_face_powder, sizeof(double) * nfpw,
                                      hipMemcpyDeviceToDevice, s));
       }
-      run.forward_step(t, dt, T.p, nullptr);
+      // the last segment is recorded now: the backward sweep starts there
+      // without recomputing it
+      const int k_last = (n_seg - 1) * K;
+      run.forward_step(t, dt, T.p,
+                       k >= k_last ? &rec[(size_t)(k - k_last) * S] : nullptr);
       t = t + dt;
     }
     times[n_steps] = t;
@@ -1726,19 +1760,22 @@ int adm_adjoint_gradient(void *ptr, int method, double t0, double dt,
     // backward sweep over the segments
     for (int sg = n_seg - 1; sg >= 0; --sg) {
       const int k0 = sg * K, k1 = std::min(n_steps, k0 + K);
-      ADM_HIP_ADJ(hipMemcpyAsync(T.p, cpT.p + (int64_t)sg * n,
-                                 sizeof(double) * n, hipMemcpyDeviceToDevice,
-                                 s));
-      if (npw)
-        ADM_HIP_ADJ(hipMemcpyAsync(c.d_powder, cpP.p + (int64_t)sg * npw,
-                                   sizeof(double) * npw,
+      if (sg < n_seg - 1) {
+        ADM_HIP_ADJ(hipMemcpyAsync(T.p, cpT.p + (int64_t)sg * n,
+                                   sizeof(double) * n,
                                    hipMemcpyDeviceToDevice, s));
-      if (nfpw)
-        ADM_HIP_ADJ(hipMemcpyAsync(c.d_face_powder, cpF.p + (int64_t)sg * nfpw,
-                                   sizeof(double) * nfpw,
-                                   hipMemcpyDeviceToDevice, s));
-      for (int k = k0; k < k1; ++k)
-        run.forward_step(times[k], dt, T.p, &rec[(size_t)(k - k0) * S]);
+        if (npw)
+          ADM_HIP_ADJ(hipMemcpyAsync(c.d_powder, cpP.p + (int64_t)sg * npw,
+                                     sizeof(double) * npw,
+                                     hipMemcpyDeviceToDevice, s));
+        if (nfpw)
+          ADM_HIP_ADJ(hipMemcpyAsync(c.d_face_powder,
+                                     cpF.p + (int64_t)sg * nfpw,
+                                     sizeof(double) * nfpw,
+                                     hipMemcpyDeviceToDevice, s));
+        for (int k = k0; k < k1; ++k)
+          run.forward_step(times[k], dt, T.p, &rec[(size_t)(k - k0) * S]);
+      }
       const int64_t n_act = adm::active_prefix(c, c.nz_active);
       for (int k = k1 - 1; k >= k0; --k) {
         const adm::AdjointRun::StageRecord *rk = &rec[(size_t)(k - k0) * S];
