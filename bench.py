@@ -454,7 +454,8 @@ def main():
                          "2,4) instead of the autotuner's")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--only", choices=["all", "vmult", "ensemble", "copy",
-                                       "adjoint", "deposition"],
+                                       "adjoint", "deposition",
+                                       "ensemble_adjoint"],
                     default="all",
                     help="profiling aid: run one part and print its timing")
     ap.add_argument("--members", type=int, default=64)
@@ -487,6 +488,9 @@ def main():
         return 0
     if args.only == "deposition":
         print(json.dumps(deposition_bench()), flush=True)
+        return 0
+    if args.only == "ensemble_adjoint":
+        print(json.dumps(ensemble_adjoint_bench(loop_sample=1)), flush=True)
         return 0
 
     problem = plain_large()
