@@ -1349,7 +1349,16 @@ void Context::launch(const CellArgs &a, const FaceArgs &fa, const double *src,
     // the face term's slots, then the second pass: every node's sum from
     // the patch buffer and its face slots into dst
     launch_faces();
-    const dim3 ggrid((unsigned)((n_dofs + 255) / 256), M);
+    // a stage update leaves the nodes above the active cell layers as they
+    // are (the gather returns at once there), so its grid stops at them;
+    // dst = A src is zero there and is written
+    const int64_t n_gather =
+        st ? std::min<int64_t>(n_dofs,
+                               (int64_t)nn[0] * nn[1] *
+                                   std::min<int64_t>(nn[2],
+                                                     (int64_t)p * lim[2] + 1))
+           : n_dofs;
+    const dim3 ggrid((unsigned)((n_gather + 255) / 256), M);
     if (p == 1)
       hipLaunchKernelGGL(plane_gather_kernel<1>, ggrid, dim3(256), 0, s,
                          ac.plane_E, ac.plane_E_stride, dst,
