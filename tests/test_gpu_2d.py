@@ -404,3 +404,34 @@ def test_adjoint_is_three_dimensional():
     with pytest.raises(AdmError, match="three-dimensional"):
         adjoint_gradient(ph, 0.0, 1e-7, 2, T0, np.array([0], np.int64),
                          np.array([300.0]))
+
+
+def test_checkpoint_2d(tmp_path):
+    """A 2-D checkpoint taken while powder melts restarts bit for bit
+    (temperatures, point and face state)."""
+    import torch
+    from adamantine_amd.thermal import ThermalPhysicsDevice
+    prob = problem_2d(p=2, nc=(12, 8), method=RK_FOURTH_ORDER, active_rows=7)
+    X = prob.node_coordinates()
+    T0 = np.where(X[:, 1] > 0.5 * prob.extent[2], 1690.0, 500.0) + \
+        np.random.RandomState(12).uniform(0.0, 40.0, prob.n_dofs)
+    a = ThermalPhysicsDevice(prob)
+    a.setup()
+    sol = torch.tensor(T0, device="cuda")
+    t = a.evolve(0.0, 2e-7, 6, sol)
+    prefix = str(tmp_path / "ck2d")
+    a.save_checkpoint(prefix, sol)
+    _, pw = a.operator.qp_state()
+    assert ((pw > 0.0) & (pw < 1.0)).sum() > 0
+    t_a = a.evolve(t, 2e-7, 6, sol)
+    torch.cuda.synchronize()
+    b = ThermalPhysicsDevice(prob)
+    T = b.load_checkpoint(prefix)
+    t_b = b.evolve(t, 2e-7, 6, T)
+    torch.cuda.synchronize()
+    assert t_b == t_a
+    np.testing.assert_array_equal(T.cpu().numpy(), sol.cpu().numpy())
+    np.testing.assert_array_equal(b.operator.qp_state()[1],
+                                  a.operator.qp_state()[1])
+    np.testing.assert_array_equal(b.operator.face_state(),
+                                  a.operator.face_state())
