@@ -183,40 +183,39 @@ __global__ void face_count_kernel(ActGeom g, const KConst *__restrict__ kc,
 }
 
 // The cells holding box node (i0, i1, i2), in increasing cell index.
-struct NodeCells {
-  int64_t cell[8];
-  int local[8][3];  // the node's local index in each
-  int n;
-};
-
-__device__ __forceinline__ void node_cells(const ActGeom &g, int64_t node,
-                                           NodeCells &nc) {
+// f(cell, local index of the node in it) for every cell holding the node,
+// in increasing cell order (z, then y, then x): up to 2 x 2 x 2 cells, in
+// unrolled loops (no per-lane arrays, which would live in scratch memory)
+template <class F>
+__device__ __forceinline__ void for_node_cells(const ActGeom &g, int64_t node,
+                                               F &&f) {
   const int i[3] = {(int)(node % g.nn[0]), (int)((node / g.nn[0]) % g.nn[1]),
                     (int)(node / ((int64_t)g.nn[0] * g.nn[1]))};
   int lo[3], cnt[3];
+#pragma unroll
   for (int d = 0; d < 3; ++d) {
     const int c_hi = min(i[d] / g.p, g.nc[d] - 1);
     const int c_lo = (i[d] % g.p == 0 && i[d] > 0) ? i[d] / g.p - 1 : c_hi;
     lo[d] = c_lo;
     cnt[d] = c_hi - c_lo + 1;
   }
-  nc.n = 0;
-  for (int z = 0; z < cnt[2]; ++z)
-    for (int y = 0; y < cnt[1]; ++y)
-      for (int x = 0; x < cnt[0]; ++x) {
+#pragma unroll
+  for (int z = 0; z < 2; ++z)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        if (z >= cnt[2] || y >= cnt[1] || x >= cnt[0]) continue;
         const int cx = lo[0] + x, cy = lo[1] + y, cz = lo[2] + z;
-        nc.cell[nc.n] = cx + (int64_t)g.nc[0] * (cy + (int64_t)g.nc[1] * cz);
-        nc.local[nc.n][0] = i[0] - cx * g.p;
-        nc.local[nc.n][1] = i[1] - cy * g.p;
-        nc.local[nc.n][2] = i[2] - cz * g.p;
-        ++nc.n;
+        const int l[3] = {i[0] - cx * g.p, i[1] - cy * g.p, i[2] - cz * g.p};
+        f(cx + (int64_t)g.nc[0] * (cy + (int64_t)g.nc[1] * cz), l);
       }
 }
 
 // Face f = 2 d + side of a cell holds the node with local index l when
 // l[d] is 0 (side 0) or p (side 1); its slot inside the face is
 // l[t1] + (p+1) l[t2], t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2.
-__device__ __forceinline__ bool on_face(int p, int f, const int l[3],
+__device__ __forceinline__ bool on_face(int p, int f, const int (&l)[3],
                                         int &q) {
   const int d = f / 2, side = f % 2;
   if (l[d] != (side ? p : 0)) return false;
@@ -231,21 +230,20 @@ __global__ void node_count_kernel(ActGeom g, const uint32_t *__restrict__ meta,
                                   double *__restrict__ T, double new_T) {
   const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (node >= g.n_dofs) return;
-  NodeCells nc;
-  node_cells(g, node, nc);
   uint32_t slots = 0;
   bool before = false, after = false;
-  for (int k = 0; k < nc.n; ++k) {
-    const uint32_t m = meta[nc.cell[k]];
-    if (!(m & kActive)) continue;
+  for_node_cells(g, node, [&](int64_t cell, const int (&l)[3]) {
+    const uint32_t m = meta[cell];
+    if (!(m & kActive)) return;
     after = true;
     if (!(m & kNewlyActive)) before = true;
     const uint32_t fm = (m >> kFaceShift) & 63u;
+#pragma unroll
     for (int f = 0; f < 6; ++f) {
       int q;
-      if ((fm >> f) & 1u) slots += on_face(g.p, f, nc.local[k], q) ? 1u : 0u;
+      if ((fm >> f) & 1u) slots += on_face(g.p, f, l, q) ? 1u : 0u;
     }
-  }
+  });
   packed[node] = (uint64_t)slots | ((uint64_t)(slots > 0) << 32);
   if (T && after && !before) T[node] = new_T;
 }
@@ -319,22 +317,21 @@ __global__ void node_fill_kernel(ActGeom g, const uint32_t *__restrict__ meta,
   fn_offset[rank] = off;
   if (rank == n_fnodes - 1)
     fn_offset[rank + 1] = off + (int32_t)(pk & 0xffffffffull);
-  NodeCells nc;
-  node_cells(g, node, nc);
   const int NF = (g.p + 1) * (g.p + 1);
   int e = off;
-  for (int k = 0; k < nc.n; ++k) {
-    const uint32_t m = meta[nc.cell[k]];
-    if (!(m & kActive)) continue;
+  for_node_cells(g, node, [&](int64_t cell, const int (&l)[3]) {
+    const uint32_t m = meta[cell];
+    if (!(m & kActive)) return;
     const uint32_t fm = (m >> kFaceShift) & 63u;
-    int32_t fi = start[nc.cell[k]];
+    int32_t fi = start[cell];
+#pragma unroll
     for (int f = 0; f < 6; ++f) {
       if (!((fm >> f) & 1u)) continue;
       int q;
-      if (on_face(g.p, f, nc.local[k], q)) fn_slot[e++] = fi * NF + q;
+      if (on_face(g.p, f, l, q)) fn_slot[e++] = fi * NF + q;
       ++fi;
     }
-  }
+  });
 }
 
 // Tbar = 0 on the nodes the last activation added
@@ -342,15 +339,13 @@ __global__ void fresh_zero_kernel(ActGeom g, const uint32_t *__restrict__ meta,
                                   double *__restrict__ Tbar) {
   const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (node >= g.n_dofs) return;
-  NodeCells nc;
-  node_cells(g, node, nc);
   bool before = false, after = false;
-  for (int k = 0; k < nc.n; ++k) {
-    const uint32_t m = meta[nc.cell[k]];
-    if (!(m & kActive)) continue;
+  for_node_cells(g, node, [&](int64_t cell, const int (&)[3]) {
+    const uint32_t m = meta[cell];
+    if (!(m & kActive)) return;
     after = true;
     if (!(m & kNewlyActive)) before = true;
-  }
+  });
   if (after && !before) Tbar[node] = 0.0;
 }
 
