@@ -107,7 +107,9 @@ __global__ void cell_kernel(ActGeom g, uint32_t *__restrict__ meta,
   if (c >= g.n_cells) return;
   const int nq = g.nq;
   uint32_t m = meta[c] & ~kNewlyActive;
-  if (g.n_states > 1) {
+  // a cell without the "held liquid" flag holds zeros at every point: its
+  // mean is zero and nothing changes
+  if (g.n_states > 1 && liquid_dirty[c]) {
     double *lq = liquid + c * nq;
     double ls = 0.0;
     for (int q = 0; q < nq; ++q) ls += lq[q];
