@@ -229,8 +229,8 @@ __device__ __forceinline__ double beam_value_dR(const adm_beam_state &s,
 // (dA/dT)^T mu over the cells: same lane layout as cell_kernel (one x-line
 // per lane, (p+1)^2 lanes per cell, one row segment per wavefront), two
 // fields (T, mu) evaluated, atomic accumulation (or plain read-add-write
-// of the cells of one parity colour per launch), parameter partials
-// reduced per wavefront.
+// of the cells of one parity colour per launch), parameter partials summed
+// per block into block_part (reduce_partials_kernel adds them up).
 template <int P, int NS, int PK>
 __global__ void __launch_bounds__(256)
     adjoint_cell_kernel(const AdjArgs aa) {

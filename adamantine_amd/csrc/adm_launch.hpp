@@ -16,8 +16,9 @@ void launch_cell(const CellArgs &a, const double *src, double *dst,
                  int64_t n_blocks, size_t lds_bytes, unsigned n_members,
                  int waves, hipStream_t s);
 
-// The plane kernel (FE degrees 1 and 2 only): one atomic launch of
-// n_blocks blocks of `waves` wavefronts, one PX x PY cell patch each.
+// The plane kernel (FE degrees 1 and 2 only): one launch of n_blocks
+// persistent blocks of `waves` wavefronts looping over PX x PY cell
+// patches, their node sums into the patch buffer (no atomics).
 template <int P, int NS, int PK>
 void launch_plane(const CellArgs &a, const double *src, double *dst,
                   int64_t n_blocks, size_t lds_bytes, unsigned n_members,
