@@ -1059,8 +1059,8 @@ inline void launch_vjp(Context &c, const double *Y, const double *pw,
   // one launch with atomic adds, or (ADM_ADJOINT_LAUNCH=colored in the
   // environment) eight launches, one per parity colour, each adding its
   // nodes' terms with a plain read-add-write (cells of one colour share no
-  // node). Measured: equal on the 32-member 96x96x40 workload (1.06 s per
-  // 50-step gradient either way), 1.5x slower on one 64x32x16 operator
+  // node). Measured: equal on the 32-member 96x96x40 workload (0.30 s per
+  // 50-step gradient either way), 1.4x slower on one 64x32x16 operator
   // (the colour launches are too small to fill the chip).
   bool colored = false;
   if (const char *e = std::getenv("ADM_ADJOINT_LAUNCH"))
