@@ -1801,7 +1801,19 @@ void Context::evolve_graph(int method, double t, double dt, int n_steps,
           hipLaunchKernelGGL(advance_slot_kernel, dim3(1), dim3(64), 0, cs,
                              d_slot);
         },
-        [](double, const double *, const StageArgs &) {}, false, cs);
+        [&](double, const double *x, const StageArgs &st) {
+          // the plane path's fused stage update (its arguments are the
+          // same every step)
+          CellArgs a = cell_args();
+          a.add = 0;
+          a.beam_table = d_beam_table;
+          a.beam_slot = d_slot;
+          a.beam_table_len = len;
+          launch(a, face_args(), x, nullptr, 1, cs, nz_active, nullptr, &st);
+          hipLaunchKernelGGL(advance_slot_kernel, dim3(1), dim3(64), 0, cs,
+                             d_slot);
+        },
+        plane_path(), cs);
   } catch (...) {
     // end the capture before the stream goes away
     if (hipStreamEndCapture(cs, &graph) == hipSuccess && graph)
