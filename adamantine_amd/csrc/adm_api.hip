@@ -1928,6 +1928,9 @@ void Ensemble::refresh_members() {
 
 Ensemble::~Ensemble() {
   if (d_members) (void)hipFree(d_members);
+  if (d_beam_table) (void)hipFree(d_beam_table);
+  if (d_pd_table) (void)hipFree(d_pd_table);
+  if (d_slot) (void)hipFree(d_slot);
   if (d_da) (void)hipFree(d_da);
   if (h_da) (void)hipHostFree(h_da);
   if (da_done) (void)hipEventDestroy(da_done);
@@ -1973,10 +1976,8 @@ void Ensemble::set_time(double t) {
   }
 }
 
-void Ensemble::apply(const double *src, double *dst, int add, hipStream_t s,
-                     const StageArgs *st) {
-  Context &c0 = *members[0];
-  CellArgs a = c0.cell_args();
+CellArgs Ensemble::cell_args(int add) const {
+  CellArgs a = members[0]->cell_args();
   a.add = add;
   a.members = d_members;
   a.member_stride = n_dofs;
@@ -1984,14 +1985,28 @@ void Ensemble::apply(const double *src, double *dst, int add, hipStream_t s,
     a.beam_pmod[b] = pmod[b];
     a.beam_den[b] = den[b];
   }
-  FaceArgs fa = c0.face_args();
+  return a;
+}
+
+FaceArgs Ensemble::face_args() const {
+  FaceArgs fa = members[0]->face_args();
   fa.members = d_members;
   fa.member_stride = n_dofs;
   fa.n_faces = max_faces;
   fa.n_fnodes = max_fnodes;
+  return fa;
+}
+
+int Ensemble::nz_active() const {
   int nz = 0;
   for (const auto &m : members) nz = std::max(nz, m->nz_active);
-  c0.launch(a, fa, src, dst, (unsigned)n_members, s, nz, nullptr, st);
+  return nz;
+}
+
+void Ensemble::apply(const double *src, double *dst, int add, hipStream_t s,
+                     const StageArgs *st) {
+  members[0]->launch(cell_args(add), face_args(), src, dst,
+                     (unsigned)n_members, s, nz_active(), nullptr, st);
   ADM_HIP(hipGetLastError());
 }
 
@@ -2022,6 +2037,116 @@ void Ensemble::step(int method, double t, double dt, double *T,
         apply(x, nullptr, 0, s, &st);
       },
       members[0]->plane_path(), s);
+}
+
+// Context::evolve_graph for every member at once: the shared beam states
+// and, for each member's alpha, the shared power modifiers and
+// denominators of every stage come from tables the host fills beforehand.
+void Ensemble::evolve_graph(int method, double t, double dt, int n_steps,
+                            double *T, hipStream_t s) {
+  Context &c0 = *members[0];
+  const int ns = n_stages(method);
+  const size_t rows = (size_t)n_steps * ns;
+  if (rows > (size_t)INT32_MAX)
+    throw AdmError(ADM_ERR_INVALID, "too many steps for one captured loop");
+  ADM_HIP(hipStreamSynchronize(s));
+  if (rows > beam_table_rows) {
+    ADM_FREE(d_beam_table);
+    ADM_FREE(d_pd_table);
+    d_beam_table = nullptr;
+    d_pd_table = nullptr;
+    beam_table_rows = 0;
+    ADM_HIP(hipMalloc(&d_beam_table,
+                      sizeof(adm_beam_state) * ADM_MAX_BEAMS * rows));
+    ADM_HIP(hipMalloc(&d_pd_table, sizeof(double) * 2 * ADM_MAX_BEAMS * rows));
+    beam_table_rows = rows;
+  }
+  if (!d_slot) ADM_HIP(hipMalloc(&d_slot, sizeof(int32_t)));
+  std::vector<adm_beam_state> table(rows * ADM_MAX_BEAMS);
+  std::memset(table.data(), 0, sizeof(adm_beam_state) * table.size());
+  std::vector<double> pd(rows * 2 * ADM_MAX_BEAMS, 0.0);
+  double tk = t;
+  for (int k = 0; k < n_steps; ++k) {
+    for (int i = 0; i < ns; ++i) {
+      set_time(tk + stage_c(method, i) * dt);
+      const size_t row = (size_t)k * ns + i;
+      for (size_t b = 0; b < c0.beams.size(); ++b)
+        table[row * ADM_MAX_BEAMS + b] = c0.beam_states[b];
+      for (int b = 0; b < ADM_MAX_BEAMS; ++b) {
+        pd[row * 2 * ADM_MAX_BEAMS + b] = pmod[b];
+        pd[row * 2 * ADM_MAX_BEAMS + ADM_MAX_BEAMS + b] = den[b];
+      }
+    }
+    tk = tk + dt;
+  }
+  ADM_HIP(hipMemcpy(d_beam_table, table.data(),
+                    sizeof(adm_beam_state) * table.size(),
+                    hipMemcpyHostToDevice));
+  ADM_HIP(hipMemcpy(d_pd_table, pd.data(), sizeof(double) * pd.size(),
+                    hipMemcpyHostToDevice));
+  const int nz = nz_active();
+  if (c0.plane_path())
+    c0.ensure_patch_buffer(c0.patch_buffer_size(nz, (unsigned)n_members));
+  const int32_t len = (int32_t)rows;
+  auto table_args = [&](int add) {
+    CellArgs a = cell_args(add);
+    a.beam_table = d_beam_table;
+    a.beam_slot = d_slot;
+    a.beam_table_len = len;
+    a.beam_pd_table = d_pd_table;
+    return a;
+  };
+  hipStream_t cs;
+  ADM_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipError_t cap = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (cap != hipSuccess) {
+    (void)hipStreamDestroy(cs);
+    throw AdmError(ADM_ERR_HIP, std::string("graph capture: ") +
+                                    hipGetErrorString(cap));
+  }
+  try {
+    explicit_step(
+        method, t, dt, T, sb,
+        [&](double, const double *x, double *r) {
+          c0.launch(table_args(0), face_args(), x, r, (unsigned)n_members, cs,
+                    nz);
+          hipLaunchKernelGGL(advance_slot_kernel, dim3(1), dim3(64), 0, cs,
+                             d_slot);
+        },
+        [&](double, const double *x, const StageArgs &st) {
+          c0.launch(table_args(0), face_args(), x, nullptr,
+                    (unsigned)n_members, cs, nz, nullptr, &st);
+          hipLaunchKernelGGL(advance_slot_kernel, dim3(1), dim3(64), 0, cs,
+                             d_slot);
+        },
+        c0.plane_path(), cs);
+  } catch (...) {
+    if (hipStreamEndCapture(cs, &graph) == hipSuccess && graph)
+      (void)hipGraphDestroy(graph);
+    (void)hipStreamDestroy(cs);
+    throw;
+  }
+  cap = hipStreamEndCapture(cs, &graph);
+  if (cap != hipSuccess) {
+    (void)hipStreamDestroy(cs);
+    throw AdmError(ADM_ERR_HIP, std::string("graph capture: ") +
+                                    hipGetErrorString(cap));
+  }
+  hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(set_slot_kernel, dim3(1), dim3(64), 0, cs, d_slot, 0);
+    for (int k = 0; k < n_steps && e == hipSuccess; ++k)
+      e = hipGraphLaunch(exec, cs);
+    if (e == hipSuccess) e = hipStreamSynchronize(cs);
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  (void)hipStreamDestroy(cs);
+  if (e != hipSuccess)
+    throw AdmError(ADM_ERR_HIP, std::string("graph replay: ") +
+                                    hipGetErrorString(e));
 }
 
 void Ensemble::step_hooked(
@@ -2602,6 +2727,23 @@ int adm_ensemble_evolve(void *ptr, int method, double t, double dt,
     ens.step(method, t, dt, T, s);
     t = t + dt;
   }
+  if (t_end) *t_end = t;
+  ADM_API_END
+}
+
+int adm_ensemble_evolve_graph(void *ptr, int method, double t, double dt,
+                              int n_steps, double *T, double *t_end,
+                              void *stream) {
+  ADM_API_BEGIN
+  ADM_ENS(ptr)
+  if (!T) return adm::set_error(ADM_ERR_INVALID, "null solution vector");
+  if (method < ADM_FORWARD_EULER || method > ADM_RK_FOURTH_ORDER)
+    return adm::set_error(ADM_ERR_INVALID, "unknown time stepping method");
+  if (n_steps < 0) return adm::set_error(ADM_ERR_INVALID, "n_steps < 0");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  ens.ensure_step_buffers(method);
+  if (n_steps > 0) ens.evolve_graph(method, t, dt, n_steps, T, s);
+  for (int k = 0; k < n_steps; ++k) t = t + dt;
   if (t_end) *t_end = t;
   ADM_API_END
 }

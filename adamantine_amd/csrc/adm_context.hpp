@@ -300,6 +300,13 @@ struct Ensemble {
   // Context::step_hooked for every member at once
   void step_hooked(int method, double t, double dt, double *T, hipStream_t s,
                    const std::function<void(int, const double *)> &hook);
+  // n_steps steps as replays of one captured step (Context::evolve_graph)
+  void evolve_graph(int method, double t, double dt, int n_steps, double *T,
+                    hipStream_t s);
+  // the batched launches' arguments and cell layers
+  CellArgs cell_args(int add) const;
+  FaceArgs face_args() const;
+  int nz_active() const;
   void refresh_members();
 
   int n_members = 0;
@@ -321,6 +328,12 @@ struct Ensemble {
   hipEvent_t da_done = nullptr;
   double *da_device(size_t n_doubles);
   double *da_host(size_t n_doubles);
+  // captured step loops: beam states, and power modifiers / denominators,
+  // of every stage, and the stage counter
+  adm_beam_state *d_beam_table = nullptr;
+  double *d_pd_table = nullptr;
+  size_t beam_table_rows = 0;
+  int32_t *d_slot = nullptr;
 };
 
 }  // namespace adm

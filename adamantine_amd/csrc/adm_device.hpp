@@ -163,10 +163,14 @@ struct CellArgs {
   int32_t nseg;
   int32_t n_rows;
   // captured step loops: the beams come from row *beam_slot of beam_table
-  // (ADM_MAX_BEAMS states per row, beam_table_len rows) instead of `beams`
+  // (ADM_MAX_BEAMS states per row, beam_table_len rows) instead of `beams`,
+  // and an ensemble's shared power modifiers and denominators from the
+  // same row of beam_pd_table (ADM_MAX_BEAMS of each per row) instead of
+  // beam_pmod / beam_den
   const adm_beam_state *beam_table;
   const int32_t *beam_slot;
   int32_t beam_table_len;
+  const double *beam_pd_table;
   // nonzero: give each XCD a contiguous run of row segments (xcd_block)
   int32_t xcd_remap;
   // plane kernel: patches per x row and per y column of a cell layer (a

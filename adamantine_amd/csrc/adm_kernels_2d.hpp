@@ -101,20 +101,29 @@ __global__ void __launch_bounds__(256)
     constexpr int words = sizeof(adm_beam_state) * ADM_MAX_BEAMS / 8;
     const int tid = threadIdx.x;
     const adm_beam_state *bsrc = a.beams;
+    int bslot = 0;
     if (a.beam_table) {
-      int slot = *as_global(a.beam_slot);
-      slot = slot < 0 ? 0 : (slot < a.beam_table_len ? slot
-                                                     : a.beam_table_len - 1);
-      bsrc = a.beam_table + (int64_t)slot * ADM_MAX_BEAMS;
+      bslot = *as_global(a.beam_slot);
+      bslot = bslot < 0 ? 0
+                        : (bslot < a.beam_table_len ? bslot
+                                                    : a.beam_table_len - 1);
+      bsrc = a.beam_table + (int64_t)bslot * ADM_MAX_BEAMS;
     }
     if (tid < words)
       reinterpret_cast<uint64_t *>(sbeams)[tid] =
           reinterpret_cast<const uint64_t *>(bsrc)[tid];
     __syncthreads();
-    if (mb && tid < ADM_MAX_BEAMS)
-      sbeams[tid].alpha = member_alpha(sbeams[tid], mb->beam_gain[tid],
-                                       a.beam_pmod[tid], a.beam_den[tid],
-                                       kc.log_01);
+    if (mb && tid < ADM_MAX_BEAMS) {
+      double pmod = a.beam_pmod[tid], den = a.beam_den[tid];
+      if (a.beam_pd_table) {
+        const double *pd =
+            a.beam_pd_table + (int64_t)bslot * 2 * ADM_MAX_BEAMS;
+        pmod = pd[tid];
+        den = pd[ADM_MAX_BEAMS + tid];
+      }
+      sbeams[tid].alpha = member_alpha(sbeams[tid], mb->beam_gain[tid], pmod,
+                                       den, kc.log_01);
+    }
     __syncthreads();
   }
   int sources_on = a.sources_on;

@@ -166,8 +166,9 @@ __global__ void __launch_bounds__(256)
     constexpr int words = sizeof(adm_beam_state) * ADM_MAX_BEAMS / 8;
     const int tid = threadIdx.x;
     const adm_beam_state *bsrc = a.beams;
+    int bslot = 0;
     if (a.beam_table) {
-      int bslot = *as_global(a.beam_slot);
+      bslot = *as_global(a.beam_slot);
       bslot = bslot < 0 ? 0
                         : (bslot < a.beam_table_len ? bslot
                                                     : a.beam_table_len - 1);
@@ -201,10 +202,17 @@ __global__ void __launch_bounds__(256)
       }
     }
     __syncthreads();
-    if (mb && tid < ADM_MAX_BEAMS)
-      sbeams[tid].alpha = member_alpha(sbeams[tid], mb->beam_gain[tid],
-                                       a.beam_pmod[tid], a.beam_den[tid],
-                                       kc.log_01);
+    if (mb && tid < ADM_MAX_BEAMS) {
+      double pmod = a.beam_pmod[tid], den = a.beam_den[tid];
+      if (a.beam_pd_table) {
+        const double *pd =
+            a.beam_pd_table + (int64_t)bslot * 2 * ADM_MAX_BEAMS;
+        pmod = pd[tid];
+        den = pd[ADM_MAX_BEAMS + tid];
+      }
+      sbeams[tid].alpha = member_alpha(sbeams[tid], mb->beam_gain[tid], pmod,
+                                       den, kc.log_01);
+    }
     if (mb) __syncthreads();
   }
   int sources_on = a.sources_on;

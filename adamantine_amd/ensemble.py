@@ -67,6 +67,9 @@ def _bind(L):
                                         C.POINTER(C.c_int32), _dp]
     L.adm_ensemble_evolve.argtypes = [_vp, C.c_int, C.c_double, C.c_double,
                                       C.c_int, _vp, _dp, _vp]
+    L.adm_ensemble_evolve_graph.argtypes = [_vp, C.c_int, C.c_double,
+                                            C.c_double, C.c_int, _vp, _dp,
+                                            _vp]
     L.adm_ensemble_get_qp_state.argtypes = [_vp, C.c_int32, _dp, _dp]
     L.adm_ensemble_set_qp_state.argtypes = [_vp, C.c_int32, _dp, _dp]
     L.adm_ensemble_get_face_state.argtypes = [_vp, C.c_int32, _dp]
@@ -275,12 +278,16 @@ class EnsembleDevice:
                 "ms": dict(zip(configs, ms.tolist()))}
 
     def evolve(self, t: float, delta_t: float, n_steps: int, solution,
-               stream=None) -> float:
+               use_graph: bool = False, stream=None) -> float:
+        """n_steps batched steps of every member; with use_graph, as
+        replays of one captured step (adm_ensemble_evolve_graph)."""
         self._check_vec(solution)
         t_end = C.c_double()
-        check(_lib().adm_ensemble_evolve(
-            self._h, int(self.method), float(t), float(delta_t), int(n_steps),
-            _dev_ptr(solution), C.byref(t_end), _stream_ptr(stream)))
+        fn = (_lib().adm_ensemble_evolve_graph if use_graph
+              else _lib().adm_ensemble_evolve)
+        check(fn(self._h, int(self.method), float(t), float(delta_t),
+                 int(n_steps), _dev_ptr(solution), C.byref(t_end),
+                 _stream_ptr(stream)))
         return t_end.value
 
     def evolve_one_time_step(self, t, delta_t, solution, stream=None):

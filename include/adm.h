@@ -496,6 +496,14 @@ int adm_ensemble_autotune(void *ens, const double *src, double *dst,
 int adm_ensemble_evolve(void *ens, int method, double t, double dt,
                         int n_steps, double *T, double *t_end, void *stream);
 
+/* adm_ensemble_evolve as replays of one captured step (a HIP graph): the
+ * same kernels and results, the beams' states and the members' shared
+ * power modifiers of every stage read from tables the host fills before
+ * the capture. Same replacement (application/adamantine.hh:2137-2142). */
+int adm_ensemble_evolve_graph(void *ens, int method, double t, double dt,
+                              int n_steps, double *T, double *t_end,
+                              void *stream);
+
 /* Member `member`'s per-quadrature-point liquid and powder ratios (host,
  * [n_cells][n_q]; either pointer may be NULL). */
 int adm_ensemble_get_qp_state(void *ens, int32_t member, double *liquid,
