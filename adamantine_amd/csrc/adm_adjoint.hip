@@ -420,30 +420,55 @@ __global__ void __launch_bounds__(256)
     const double py = kc.origin[1] + (rc.cc[1] + kc.xq[la]) * kc.h[1];
     const double pz = kc.origin[2] + (rc.cc[2] + kc.xq[lb]) * kc.h[2];
     double cT[N], cw[3][N];
+    // wave-uniform fast path (as the forward kernels): every point of the
+    // wavefront fully solid (one material, no powder, below the solidus).
+    // The mixtures then reduce to the solid state's value and slope bit for
+    // bit (the other states enter with zero ratio and zero ratio slope),
+    // which the one-state evaluators give.
+    const int mid0 = __builtin_amdgcn_readfirstlane(mid);
+    bool warm = false;
+    if constexpr (NS > 1) {
 #pragma unroll
-    for (int q = 0; q < N; ++q) {
+      for (int q = 0; q < N; ++q) warm |= !(V[0][q] < sol);
+    }
+    const bool fast = !__any(warm || slot >= 0 || mid != mid0);
+    auto point = [&](auto solid_tag, auto q_tag) {
+      constexpr bool SOLID = decltype(solid_tag)::value;
+      constexpr int q = decltype(q_tag)::value;
       const double T = V[0][q], mu = V[1][q];
-      Mix<NS> x;
-      double dl;
-      bool clamped;
-      ratios_lin<NS>(T, pw ? pw[q] : 0.0, sol, liq, x, dl, clamped);
-      double irc, ircT, ircp;
+      double dl = 0.0;
+      bool clamped = false;
+      double irc, ircT, ircp = 0.0;
+      double kx, kxT, kxp = 0.0, ky, kyT, kyp = 0.0, kz, kzT, kzp = 0.0;
       // the merged-table interval of T, once for every property
       const int kt = PK == 2 ? merged_bin_interval(mB, mbin, T) : 0;
-      inv_rho_cp_lin<NS, PK>(m, mid, mB, mC, T, kt, latent, x, irc, ircT,
-                             ircp);
-      double kx, kxT, kxp, ky, kyT, kyp, kz, kzT, kzp;
-      prop_vd<NS, PK>(m, mid, mB, mC, T, kt, ADM_THERMAL_CONDUCTIVITY_X, x,
-                      kx, kxT, kxp);
-      if (iso) {
-        ky = kz = kx;
-        kyT = kzT = kxT;
-        kyp = kzp = kxp;
+      auto props = [&](const auto &x, auto ns_tag) {
+        constexpr int NSX = decltype(ns_tag)::value;
+        inv_rho_cp_lin<NSX, PK>(m, mid, mB, mC, T, kt, latent, x, irc, ircT,
+                                ircp);
+        prop_vd<NSX, PK>(m, mid, mB, mC, T, kt, ADM_THERMAL_CONDUCTIVITY_X, x,
+                         kx, kxT, kxp);
+        if (iso) {
+          ky = kz = kx;
+          kyT = kzT = kxT;
+          kyp = kzp = kxp;
+        } else {
+          prop_vd<NSX, PK>(m, mid, mB, mC, T, kt, ADM_THERMAL_CONDUCTIVITY_Y,
+                           x, ky, kyT, kyp);
+          prop_vd<NSX, PK>(m, mid, mB, mC, T, kt, ADM_THERMAL_CONDUCTIVITY_Z,
+                           x, kz, kzT, kzp);
+        }
+      };
+      if constexpr (SOLID) {
+        Mix<1> x1;
+        x1.r[0] = 1.0;
+        x1.rT[0] = 0.0;
+        x1.rp[0] = 0.0;
+        props(x1, std::integral_constant<int, 1>{});
       } else {
-        prop_vd<NS, PK>(m, mid, mB, mC, T, kt, ADM_THERMAL_CONDUCTIVITY_Y, x,
-                        ky, kyT, kyp);
-        prop_vd<NS, PK>(m, mid, mB, mC, T, kt, ADM_THERMAL_CONDUCTIVITY_Z, x,
-                        kz, kzT, kzp);
+        Mix<NS> x;
+        ratios_lin<NS>(T, pw ? pw[q] : 0.0, sol, liq, x, dl, clamped);
+        props(x, std::integral_constant<int, NS>{});
       }
       const double g0 = G[0][0][q], g1 = G[0][1][q], g2 = G[0][2][q];
       const double m0 = G[1][0][q], m1 = G[1][1][q], m2 = G[1][2][q];
@@ -473,7 +498,7 @@ __global__ void __launch_bounds__(256)
       // partials at fixed carried powder (T) and with respect to it (p)
       double c_T = -jxw * (ircT * mKg + irc * mKg_of(kxT, kyT, kzT));
       double c_p = 0.0;
-      if constexpr (NS == 3)
+      if constexpr (NS == 3 && !SOLID)
         if (pw) c_p = -jxw * (ircp * mKg + irc * mKg_of(kxp, kyp, kzp));
       ps += -irc * jxw * mKg;
       if (a.sources_on) {
@@ -486,11 +511,11 @@ __global__ void __launch_bounds__(256)
           qs += v;
         }
         c_T += mu * ircT * qs * jxw;
-        if constexpr (NS == 3) c_p += mu * ircp * qs * jxw;
+        if constexpr (NS == 3 && !SOLID) c_p += mu * ircp * qs * jxw;
         pe += mu * irc * q0 * inv_eta0 * jxw;
         pr += mu * irc * beam_value_dR(sbeams[0], q0, px, py, pz, kc.log_01) * jxw;
       }
-      if constexpr (NS == 3) {
+      if constexpr (NS == 3 && !SOLID) {
         if (pw) {
           // the carried powder after this evaluation: its adjoint (later
           // evaluations) plus this evaluation's dependence on it goes to T
@@ -501,7 +526,15 @@ __global__ void __launch_bounds__(256)
         }
       }
       cT[q] = c_T;
-    }
+    };
+    if (fast)
+      static_for<0, N>([&](auto q) {
+        point(std::integral_constant<bool, true>{}, q);
+      });
+    else
+      static_for<0, N>([&](auto q) {
+        point(std::integral_constant<bool, false>{}, q);
+      });
     // integration as in cell_kernel
     double t[N];
 #pragma unroll
