@@ -417,7 +417,8 @@ GeomSnap::~GeomSnap() {
   (void)hipFree(fn_rank);
 }
 
-void Context::save_geometry(GeomSnap &g, hipStream_t s) const {
+void Context::save_geometry(GeomSnap &g, hipStream_t s,
+                            bool host_copy) const {
   auto keep = [&](auto *&dst, const auto *src, size_t count) {
     using T = std::remove_const_t<std::remove_pointer_t<decltype(src)>>;
     (void)hipFree(dst);
@@ -444,9 +445,15 @@ void Context::save_geometry(GeomSnap &g, hipStream_t s) const {
   g.n_fnodes = n_fnodes;
   g.n_slots = n_slots;
   g.nz_active = nz_active;
-  g.active = active;
-  g.dep_cos = dep_cos;
-  g.dep_sin = dep_sin;
+  g.host_copy = host_copy;
+  g.log_cell.clear();
+  g.log_old.clear();
+  g.log_new.clear();
+  if (host_copy) {
+    g.active = active;
+    g.dep_cos = dep_cos;
+    g.dep_sin = dep_sin;
+  }
 }
 
 void Context::restore_geometry(const GeomSnap &g, hipStream_t s) {
@@ -473,10 +480,29 @@ void Context::restore_geometry(const GeomSnap &g, hipStream_t s) {
   n_faces = g.n_faces;
   n_fnodes = g.n_fnodes;
   nz_active = g.nz_active;
-  active = g.active;
-  dep_cos = g.dep_cos;
-  dep_sin = g.dep_sin;
+  if (g.host_copy) {
+    active = g.active;
+    dep_cos = g.dep_cos;
+    dep_sin = g.dep_sin;
+  } else {
+    for (size_t i = 0; i < g.log_cell.size(); ++i) {
+      const int64_t c = g.log_cell[i];
+      active[c] = 0;
+      dep_cos[c] = g.log_old[2 * i];
+      dep_sin[c] = g.log_old[2 * i + 1];
+    }
+  }
   faces_stale = true;
+}
+
+void Context::redo_host_geometry(const GeomSnap &g) {
+  for (size_t i = 0; i < g.log_cell.size(); ++i) {
+    const int64_t c = g.log_cell[i];
+    active[c] = 1;
+    dep_cos[c] = g.log_new[2 * i];
+    dep_sin[c] = g.log_new[2 * i + 1];
+  }
+  nz_active = g.nz_after;
 }
 
 void Context::activation_adjoint(double *Tbar, double *pbar,
@@ -628,15 +654,25 @@ void Context::activate(int n_boxes, const double *boxes,
   n_fnodes = n_nodes;
   faces_stale = true;
   // host bookkeeping of the activity and deposition directions
+  GeomSnap *const lg = geom_log;
+  geom_log = nullptr;
   for (int64_t c = 0; c < n_cells; ++c) {
     const int32_t b = w.h_boc[c];
     if (box_of_cell_out) box_of_cell_out[c] = b;
     if (b < 0 || active[c]) continue;
+    if (lg) {
+      lg->log_cell.push_back(c);
+      lg->log_old.push_back(dep_cos[c]);
+      lg->log_old.push_back(dep_sin[c]);
+      lg->log_new.push_back(box_cos[b]);
+      lg->log_new.push_back(box_sin[b]);
+    }
     active[c] = 1;
     dep_cos[c] = box_cos[b];
     dep_sin[c] = box_sin[b];
     nz_active = std::max(nz_active, (int)(c / ((int64_t)nc[0] * nc[1])) + 1);
   }
+  if (lg) lg->nz_after = nz_active;
   if (sb.inv_mass) compute_inverse_mass(sb.inv_mass, s);
 }
 

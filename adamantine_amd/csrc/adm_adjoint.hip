@@ -1524,9 +1524,17 @@ int ensemble_adjoint(Ensemble &E, int method, double t0, double dt,
       const EnsEvent &e = ev[next_ev];
       snap[next_ev].reset(new GeomSnap());
       ADM_HIP_ADJ(hipStreamSynchronize(s));
-      E.members[e.member]->save_geometry(*snap[next_ev], s);
-      run.activate(e, T.p);
       Context &c = *E.members[e.member];
+      // device arrays copied, the host part logged by the activation
+      c.save_geometry(*snap[next_ev], s, false);
+      c.geom_log = snap[next_ev].get();
+      try {
+        run.activate(e, T.p);
+      } catch (...) {
+        c.geom_log = nullptr;
+        throw;
+      }
+      c.geom_log = nullptr;
       if (c.n_states == 3)
         fcap[e.member] = std::max(fcap[e.member], c.n_faces * c.nfq);
       ++next_ev;
@@ -1557,7 +1565,9 @@ int ensemble_adjoint(Ensemble &E, int method, double t0, double dt,
   for (int j = 0; j < N; ++j)
     if (moved[j]) {
       fin_geom[j].reset(new GeomSnap());
-      E.members[j]->save_geometry(*fin_geom[j], s);
+      // the host part comes back by redoing the member's logged
+      // activations
+      E.members[j]->save_geometry(*fin_geom[j], s, false);
     }
   DevBuf finP, finF;
   finP.alloc(run.npw);
@@ -1654,6 +1664,8 @@ int ensemble_adjoint(Ensemble &E, int method, double t0, double dt,
   for (int j = 0; j < N; ++j) {
     Context &c = *E.members[j];
     if (moved[j]) {
+      for (size_t e = 0; e < ev.size(); ++e)
+        if (ev[e].member == j) c.redo_host_geometry(*snap[e]);
       c.restore_geometry(*fin_geom[j], s);
       c.compute_inverse_mass(E.sb.inv_mass + (int64_t)j * n, s);
     }

@@ -82,8 +82,17 @@ struct GeomSnap {
   int32_t *fn_offset = nullptr, *fn_slot = nullptr, *fn_rank = nullptr;
   int64_t n_faces = 0, n_fnodes = 0, n_slots = 0;
   int nz_active = 0;
+  // host part: full copies of the activity and deposition directions
+  // (host_copy), or the log of the one activation that followed the
+  // snapshot -- the cells it activated with their {cos, sin} before and
+  // after, and nz_active after -- which undoes (restore_geometry) and
+  // redoes (redo_host_geometry) it without copying every cell's entries
+  bool host_copy = true;
   std::vector<uint8_t> active;
   std::vector<double> dep_cos, dep_sin;
+  std::vector<int64_t> log_cell;
+  std::vector<double> log_old, log_new;
+  int nz_after = 0;
   GeomSnap() = default;
   GeomSnap(const GeomSnap &) = delete;
   GeomSnap &operator=(const GeomSnap &) = delete;
@@ -130,8 +139,14 @@ struct Context {
   void sync_faces_host();
   // the activation geometry, saved and put back (the inverse mass is the
   // caller's: it recomputes it)
-  void save_geometry(GeomSnap &g, hipStream_t s) const;
+  // host_copy false: the host part is left to the log of the next
+  // activation (geom_log) or, with no activation after it, to the logs of
+  // the activations redone on top of it
+  void save_geometry(GeomSnap &g, hipStream_t s, bool host_copy = true) const;
   void restore_geometry(const GeomSnap &g, hipStream_t s);
+  void redo_host_geometry(const GeomSnap &g);
+  // set: the next activate() logs its host changes into this snapshot
+  GeomSnap *geom_log = nullptr;
   // adjoint of the last activation, with the geometry after it (the cells
   // it activated carry kNewlyActive): Tbar = 0 on the nodes it added, and
   // the carried powder's adjoint through the cell-mean transfer, pbar
