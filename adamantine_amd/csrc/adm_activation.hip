@@ -31,6 +31,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <type_traits>
 #include <vector>
@@ -628,8 +629,37 @@ void Context::activate(int n_boxes, const double *boxes,
   ADM_HIP_ACT(hipGetLastError());
   ADM_HIP_ACT(hipMemcpyAsync(w.h_totals, w.totals, sizeof(int64_t) * 3,
                              hipMemcpyDeviceToHost, s));
-  ADM_HIP_ACT(hipMemcpyAsync(w.h_boc, w.boc, sizeof(int32_t) * n_cells,
-                             hipMemcpyDeviceToHost, s));
+  // The cells the host bookkeeping below visits: with boxes and no
+  // per-cell output, only the cell layers the boxes' z extents meet, with
+  // one layer of margin on either side (box_kernel's closed-box test finds
+  // no box for a cell outside them); otherwise every cell.
+  int64_t c_lo = 0, c_hi = n_cells;
+  if (!box_of_cell && !box_of_cell_out && boxes && n_boxes > 0) {
+    // (the extents' lower and higher ends, whichever bound holds them)
+    double zlo = boxes[2], zhi = boxes[2];
+    bool finite = true;
+    for (int b = 0; b < n_boxes; ++b) {
+      const double a0 = boxes[6 * b + 2], a1 = boxes[6 * b + 5];
+      finite = finite && std::isfinite(a0) && std::isfinite(a1);
+      zlo = std::min(zlo, std::min(a0, a1));
+      zhi = std::max(zhi, std::max(a0, a1));
+    }
+    const double o = kc.origin[2], h = kc.h[2];
+    if (finite && h > 0.0) {
+      const double f0 = std::floor((zlo - o) / h) - 1.0;
+      const double f1 = std::floor((zhi - o) / h) + 1.0;
+      const double top = (double)(nc[2] - 1);
+      const int64_t z0 = (int64_t)std::min(std::max(f0, 0.0), top);
+      const int64_t z1 = (int64_t)std::min(std::max(f1, 0.0), top);
+      const int64_t layer = (int64_t)nc[0] * nc[1];
+      c_lo = z0 * layer;
+      c_hi = z1 >= z0 ? (z1 + 1) * layer : c_lo;
+    }
+  }
+  if (c_hi > c_lo)
+    ADM_HIP_ACT(hipMemcpyAsync(w.h_boc + c_lo, w.boc + c_lo,
+                               sizeof(int32_t) * (c_hi - c_lo),
+                               hipMemcpyDeviceToHost, s));
   ADM_HIP_ACT(hipStreamSynchronize(s));
   const int64_t nf = w.h_totals[0], n_slots = w.h_totals[1],
                 n_nodes = w.h_totals[2];
@@ -656,7 +686,7 @@ void Context::activate(int n_boxes, const double *boxes,
   // host bookkeeping of the activity and deposition directions
   GeomSnap *const lg = geom_log;
   geom_log = nullptr;
-  for (int64_t c = 0; c < n_cells; ++c) {
+  for (int64_t c = c_lo; c < c_hi; ++c) {
     const int32_t b = w.h_boc[c];
     if (box_of_cell_out) box_of_cell_out[c] = b;
     if (b < 0 || active[c]) continue;
