@@ -1496,7 +1496,10 @@ int ensemble_adjoint(Ensemble &E, int method, double t0, double dt,
   T.alloc(nt);
   cpT.alloc((size_t)nt * n_seg);
   cpP.alloc((size_t)run.npw * n_seg);
-  std::vector<std::vector<DevBuf>> cpF(n_seg);
+  // face powder at each segment start, packed over the members (the face
+  // counts of that moment give the offsets)
+  std::vector<DevBuf> cpF(n_seg);
+  std::vector<std::vector<int64_t>> cpF_off(n_seg);
   std::vector<int64_t> fcap(N, 0);
   for (int j = 0; j < N; ++j)
     fcap[j] = E.members[j]->n_states == 3
@@ -1505,18 +1508,24 @@ int ensemble_adjoint(Ensemble &E, int method, double t0, double dt,
   ADM_HIP_ADJ(hipMemcpyAsync(T.p, T0, sizeof(double) * nt,
                              hipMemcpyDeviceToDevice, s));
   size_t next_ev = 0;
-  auto cp_faces = [&](std::vector<DevBuf> &dst, bool save) {
-    if (save) dst.resize(N);
-    for (int j = 0; j < N; ++j) {
-      Context &c = *E.members[j];
-      const int64_t f = c.n_states == 3 ? c.n_faces * c.nfq : 0;
-      if (save) dst[j].alloc((size_t)f);
-      if (f)
-        ADM_HIP_ADJ(hipMemcpyAsync(save ? dst[j].p : c.d_face_powder,
-                                   save ? c.d_face_powder : dst[j].p,
-                                   sizeof(double) * f,
-                                   hipMemcpyDeviceToDevice, s));
+  auto cp_faces = [&](int i, bool save) {
+    std::vector<int64_t> &off = cpF_off[i];
+    if (save) {
+      off.assign(N + 1, 0);
+      for (int j = 0; j < N; ++j) {
+        const Context &c = *E.members[j];
+        off[j + 1] = off[j] + (c.n_states == 3 ? c.n_faces * c.nfq : 0);
+      }
+      cpF[i].alloc((size_t)off[N]);
     }
+    for (int j = 0; j < N; ++j) {
+      const int64_t f = off[j + 1] - off[j];
+      if (!f) continue;
+      Context &c = *E.members[j];
+      run.copies.emplace_back(save ? c.d_face_powder : cpF[i].p + off[j],
+                              save ? cpF[i].p + off[j] : c.d_face_powder, f);
+    }
+    run.flush_copies();
   };
   for (int i = 0; i < n_seg; ++i) {
     const int k0 = bnd[i], k1 = bnd[i + 1];
@@ -1547,11 +1556,11 @@ int ensemble_adjoint(Ensemble &E, int method, double t0, double dt,
       Context &c = *E.members[j];
       const int64_t a = run.pw_off[j + 1] - run.pw_off[j];
       if (a)
-        ADM_HIP_ADJ(hipMemcpyAsync(cpP.p + (int64_t)i * run.npw + run.pw_off[j],
-                                   c.d_powder, sizeof(double) * a,
-                                   hipMemcpyDeviceToDevice, s));
+        run.copies.emplace_back(
+            c.d_powder, cpP.p + (int64_t)i * run.npw + run.pw_off[j], a);
     }
-    cp_faces(cpF[i], true);
+    run.flush_copies();
+    cp_faces(i, true);
     for (int k = k0; k < k1; ++k) run.forward_step(times[k], dt, T.p, nullptr);
   }
   if (next_ev != ev.size())
@@ -1573,17 +1582,24 @@ int ensemble_adjoint(Ensemble &E, int method, double t0, double dt,
   finP.alloc(run.npw);
   finF.alloc(run.nfpw);
   run.copy_powder(finP.p, finF.p, true);
-  std::vector<DevBuf> finL(N), finD(N);
-  for (int j = 0; j < N; ++j) {
-    Context &c = *E.members[j];
-    if (!c.d_liquid) continue;
-    finL[j].alloc((size_t)c.n_cells * c.nq);
-    ADM_HIP_ADJ(hipMemcpyAsync(finL[j].p, c.d_liquid,
-                               sizeof(double) * c.n_cells * c.nq,
-                               hipMemcpyDeviceToDevice, s));
-    finD[j].alloc((size_t)(c.n_cells + 7) / 8);
-    ADM_HIP_ADJ(hipMemcpyAsync(finD[j].p, c.d_liquid_dirty, (size_t)c.n_cells,
-                               hipMemcpyDeviceToDevice, s));
+  // the members' liquid ratios and liquid flags, packed (member j's at
+  // j times the member's length)
+  DevBuf finL, finD;
+  const int64_t nlq = E.members[0]->d_liquid
+                          ? E.members[0]->n_cells * E.members[0]->nq : 0;
+  const int64_t ndw = (E.members[0]->n_cells + 7) / 8;  // flags, in doubles
+  if (nlq) {
+    finL.alloc((size_t)nlq * N);
+    finD.alloc((size_t)ndw * N);
+    for (int j = 0; j < N; ++j) {
+      Context &c = *E.members[j];
+      if (!c.d_liquid) continue;  // (the members share the phase model)
+      run.copies.emplace_back(c.d_liquid, finL.p + (int64_t)j * nlq, nlq);
+      ADM_HIP_ADJ(hipMemcpyAsync(finD.p + (int64_t)j * ndw, c.d_liquid_dirty,
+                                 (size_t)c.n_cells, hipMemcpyDeviceToDevice,
+                                 s));
+    }
+    run.flush_copies();
   }
   if (T_final)
     ADM_HIP_ADJ(hipMemcpyAsync(T_final, T.p, sizeof(double) * nt,
@@ -1622,12 +1638,11 @@ int ensemble_adjoint(Ensemble &E, int method, double t0, double dt,
       Context &c = *E.members[j];
       const int64_t a = run.pw_off[j + 1] - run.pw_off[j];
       if (a)
-        ADM_HIP_ADJ(hipMemcpyAsync(c.d_powder,
-                                   cpP.p + (int64_t)i * run.npw + run.pw_off[j],
-                                   sizeof(double) * a,
-                                   hipMemcpyDeviceToDevice, s));
+        run.copies.emplace_back(
+            cpP.p + (int64_t)i * run.npw + run.pw_off[j], c.d_powder, a);
     }
-    cp_faces(cpF[i], false);
+    run.flush_copies();
+    cp_faces(i, false);
     for (int k = k0; k < k1; ++k)
       run.forward_step(times[k], dt, T.p, &rec[(size_t)(k - k0) * S]);
     int nz = 0;
@@ -1669,15 +1684,14 @@ int ensemble_adjoint(Ensemble &E, int method, double t0, double dt,
       c.restore_geometry(*fin_geom[j], s);
       c.compute_inverse_mass(E.sb.inv_mass + (int64_t)j * n, s);
     }
-    if (c.d_liquid) {
-      ADM_HIP_ADJ(hipMemcpyAsync(c.d_liquid, finL[j].p,
-                                 sizeof(double) * c.n_cells * c.nq,
-                                 hipMemcpyDeviceToDevice, s));
-      ADM_HIP_ADJ(hipMemcpyAsync(c.d_liquid_dirty, finD[j].p,
+    if (nlq && c.d_liquid) {
+      run.copies.emplace_back(finL.p + (int64_t)j * nlq, c.d_liquid, nlq);
+      ADM_HIP_ADJ(hipMemcpyAsync(c.d_liquid_dirty, finD.p + (int64_t)j * ndw,
                                  (size_t)c.n_cells, hipMemcpyDeviceToDevice,
                                  s));
     }
   }
+  run.flush_copies();
   E.refresh_members();
   run.copy_powder(finP.p, finF.p, false);
   ens_time(E, times[n_steps]);
