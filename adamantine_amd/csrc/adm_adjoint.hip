@@ -800,6 +800,31 @@ __global__ void __launch_bounds__(256)
     dst[i] = src[i];
 }
 
+// device copies (src, dst, length in doubles) in launches of kSegCopies:
+// one kernel launch costs the host less than one runtime copy call, so a
+// sequence of short copies leaves the device fewer idle gaps
+using SegCopy = std::tuple<const double *, double *, int64_t>;
+inline void launch_seg_copies(const std::vector<SegCopy> &copies,
+                              hipStream_t s) {
+  for (size_t b = 0; b < copies.size(); b += kSegCopies) {
+    SegCopyArgs a;
+    std::memset(&a, 0, sizeof(a));
+    const int m = (int)std::min<size_t>(kSegCopies, copies.size() - b);
+    int64_t longest = 0;
+    for (int k = 0; k < m; ++k) {
+      a.src[k] = std::get<0>(copies[b + k]);
+      a.dst[k] = std::get<1>(copies[b + k]);
+      a.len[k] = std::get<2>(copies[b + k]);
+      longest = std::max(longest, a.len[k]);
+    }
+    if (longest == 0) continue;
+    const unsigned gx =
+        (unsigned)std::min<int64_t>((longest + 255) / 256, 2048);
+    hipLaunchKernelGGL(seg_copy_kernel, dim3(gx, (unsigned)m), dim3(256), 0,
+                       s, a);
+  }
+}
+
 // faces per block of the adjoint face kernel (FaceLanes<P>::faces_per_block)
 inline int FaceLanes_per_block(int p) {
   return 4 * (64 / ((p + 1) * (p + 1)));
@@ -1264,18 +1289,14 @@ struct AdjointRun {
     // adm_evolve's step (the gather pass applying the stage updates on the
     // plane path), recording each stage's input and the powder before its
     // evaluation
+    std::vector<SegCopy> cp;
     c.step_hooked(method, t, dt, T, s, [&](int i, const double *x) {
       if (!rec) return;
-      ADM_HIP_ADJ(hipMemcpyAsync(rec[i].Y, x, sizeof(double) * n,
-                                 hipMemcpyDeviceToDevice, s));
-      if (npw)
-        ADM_HIP_ADJ(hipMemcpyAsync(rec[i].pw, c.d_powder,
-                                   sizeof(double) * npw,
-                                   hipMemcpyDeviceToDevice, s));
-      if (nfpw)
-        ADM_HIP_ADJ(hipMemcpyAsync(rec[i].fpw, c.d_face_powder,
-                                   sizeof(double) * nfpw,
-                                   hipMemcpyDeviceToDevice, s));
+      cp.clear();
+      cp.emplace_back(x, rec[i].Y, n);
+      if (npw) cp.emplace_back(c.d_powder, rec[i].pw, npw);
+      if (nfpw) cp.emplace_back(c.d_face_powder, rec[i].fpw, nfpw);
+      launch_seg_copies(cp, s);
     });
     ADM_HIP_ADJ(hipGetLastError());
   }
@@ -1365,25 +1386,9 @@ struct EnsAdjoint {
   // members' powder <-> a packed copy (cells: fixed slots; faces: the
   // current face count of each member, at its offset)
   // copies (src, dst, length), batched kSegCopies to a launch
-  std::vector<std::tuple<const double *, double *, int64_t>> copies;
+  std::vector<SegCopy> copies;
   void flush_copies() {
-    for (size_t b = 0; b < copies.size(); b += kSegCopies) {
-      SegCopyArgs a;
-      std::memset(&a, 0, sizeof(a));
-      const int m = (int)std::min<size_t>(kSegCopies, copies.size() - b);
-      int64_t longest = 0;
-      for (int k = 0; k < m; ++k) {
-        a.src[k] = std::get<0>(copies[b + k]);
-        a.dst[k] = std::get<1>(copies[b + k]);
-        a.len[k] = std::get<2>(copies[b + k]);
-        longest = std::max(longest, a.len[k]);
-      }
-      if (longest == 0) continue;
-      const unsigned gx =
-          (unsigned)std::min<int64_t>((longest + 255) / 256, 2048);
-      hipLaunchKernelGGL(seg_copy_kernel, dim3(gx, (unsigned)m), dim3(256), 0,
-                         s, a);
-    }
+    launch_seg_copies(copies, s);
     copies.clear();
     ADM_HIP_ADJ(hipGetLastError());
   }
