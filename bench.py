@@ -30,9 +30,14 @@ LAUNCH_MODE_NAMES = {0: "line_atomic", 1: "line_colored", 2: "plane"}
 # scaled by a 256 MiB copy; the profile named beside each value holds the
 # command and the counter values. None: not profiled.
 MEASURED_TRAFFIC_BYTES = {
-    "line_colored": 355.6e6,   # profiles/hbm_traffic_plain_large.md
-    "line_atomic": 176.0e6,    # profiles/hbm_traffic_plain_large.md
-    "plane": 127.1e6,          # profiles/hbm_traffic_plane_plain_large.md
+    "line_colored": 355.6e6,
+    "line_atomic": 176.0e6,
+    "plane": 127.1e6,
+}
+TRAFFIC_PROFILE = {
+    "line_colored": "profiles/hbm_traffic_plain_large.md",
+    "line_atomic": "profiles/hbm_traffic_plain_large.md",
+    "plane": "profiles/hbm_traffic_plane_plain_large.md",
 }
 
 
@@ -581,6 +586,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS,
                      "traffic": MEASURED_TRAFFIC_BYTES.get(launch["mode"]),
+                     "traffic_source": TRAFFIC_PROFILE.get(launch["mode"]),
                      "bytes_per_launch": bytes_per_vmult},
         "rk4_step_ms": rk4_ms,
         "wall_s": wall,
