@@ -629,12 +629,12 @@ void Context::activate(int n_boxes, const double *boxes,
   ADM_HIP_ACT(hipGetLastError());
   ADM_HIP_ACT(hipMemcpyAsync(w.h_totals, w.totals, sizeof(int64_t) * 3,
                              hipMemcpyDeviceToHost, s));
-  // The cells the host bookkeeping below visits: with boxes and no
-  // per-cell output, only the cell layers the boxes' z extents meet, with
-  // one layer of margin on either side (box_kernel's closed-box test finds
-  // no box for a cell outside them); otherwise every cell.
+  // The cells the host bookkeeping below visits: with boxes, only the cell
+  // layers the boxes' z extents meet, with one layer of margin on either
+  // side (box_kernel's closed-box test finds no box for a cell outside
+  // them: their box_of_cell_out entries are -1); otherwise every cell.
   int64_t c_lo = 0, c_hi = n_cells;
-  if (!box_of_cell && !box_of_cell_out && boxes && n_boxes > 0) {
+  if (!box_of_cell && boxes && n_boxes > 0) {
     // (the extents' lower and higher ends, whichever bound holds them)
     double zlo = boxes[2], zhi = boxes[2];
     bool finite = true;
@@ -686,6 +686,11 @@ void Context::activate(int n_boxes, const double *boxes,
   // host bookkeeping of the activity and deposition directions
   GeomSnap *const lg = geom_log;
   geom_log = nullptr;
+  if (box_of_cell_out) {
+    std::fill(box_of_cell_out, box_of_cell_out + c_lo, -1);
+    std::fill(box_of_cell_out + std::max(c_hi, c_lo), box_of_cell_out + n_cells,
+              -1);
+  }
   for (int64_t c = c_lo; c < c_hi; ++c) {
     const int32_t b = w.h_boc[c];
     if (box_of_cell_out) box_of_cell_out[c] = b;
