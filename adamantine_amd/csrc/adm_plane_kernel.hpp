@@ -304,15 +304,19 @@ __global__ void __launch_bounds__(256)
   const int ix = cw % PX, iy = cw / PX;
   // The blocks loop over the patches (persistent grid: the prologue above
   // runs once per block). Blocks are dispatched round-robin over the 8
-  // XCDs (block b on XCD b % 8); each XCD takes one contiguous eighth of the
-  // patches in the y-major order, a slab of patch rows through every cell
-  // layer, so patches that share node lines share that XCD's L2 and every
-  // XCD holds its share of the slow layers (an eighth of the layers in
-  // the x-y-z order gave the top one, the powder, to one XCD: 0.120 ms per
-  // application against 0.100 in the plain order).
+  // XCDs (block b on XCD b % 8). One operator: each XCD takes one
+  // contiguous eighth of the patches in the y-major order, a slab of patch
+  // rows through every cell layer, so patches that share node lines share
+  // that XCD's L2 and every XCD holds its share of the slow layers (an
+  // eighth of the layers in the x-y-z order gave the top one, the powder,
+  // to one XCD: 0.120 ms per application of the 128x64x32 Q2 workload,
+  // against 0.100 in the plain order and 0.098 in y-major slabs). Batched
+  // members (grid y): the plain order, block after block through the
+  // patches (64 members of 64x32x16: 3.10 ms per RK4 step against 3.20 in
+  // y-major slabs and 3.75 in layer slabs).
   const int W = blockDim.x >> 6;
   int first, stride, end;
-  const bool ymajor = a.xcd_remap && gridDim.x >= 8;
+  const bool ymajor = a.xcd_remap && gridDim.x >= 8 && gridDim.y == 1;
   const int nzl = in_sgpr(a.n_rows / (npx * npy));
   if (ymajor) {
     const int x = blockIdx.x & 7, k = blockIdx.x >> 3;
